@@ -1,0 +1,235 @@
+// ref_driver.cpp -- C-ABI harness around the compiled CAVIAR2 reference.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile into oracle/_ref/
+// together with the reference sources (compiled where they lie, never copied).
+// It drives the reference's own classes exactly as a user program does
+// (tests/t001_lj.cpp:23-99) and exposes setup / step / getters so that the
+// fixture generator and the CPU tests can compare the oracle restatement with
+// the reference on identical inputs.
+//
+// The reference's default-constructed members leave fields uninitialised
+// (Domain::nprocs_*/grid_index_*/me, Atom_data::n_r_df,
+// stochastic_force_present, make_ghost_velocity, record_owned_*,
+// Md_simulator::initialized/use_time, the Langevin seeds); this harness sets
+// every one of them before use.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "caviar2/caviar2.hpp"
+#include "caviar2/atom_data.hpp"
+#include "caviar2/domain.hpp"
+#include "caviar2/force/lj.hpp"
+#include "caviar2/md_simulator.hpp"
+#include "caviar2/neighborlist.hpp"
+
+using caviar2::Vector3d;
+
+namespace {
+
+// Read-only access to Neighborlist::pos_old (protected) through a
+// pointer-to-member of a derived class.
+struct NlSpy : caviar2::Neighborlist {
+  static const std::vector<Vector3d<double>> &pos_old_of(const caviar2::Neighborlist &nl) {
+    return nl.*(&NlSpy::pos_old);
+  }
+};
+
+struct RefSim {
+  caviar2::Caviar2 *cv = nullptr;
+  caviar2::force_field::Lj *lj = nullptr;
+  int integrator = 0;
+};
+
+}  // namespace
+
+extern "C" {
+
+// integrator: 0 = Velocity_verlet, 1 = Velocity_verlet_langevin
+void *ref_sim_create(const double lo[3], const double hi[3], const int bc[3], double ghost_cutoff,
+                     double nl_cutoff, double nl_coef, int use_cell_list, int rebuild_test,
+                     int ntypes, const double *eps, const double *sigma,
+                     const double *cutoff_list, const double *mass, double lj_cutoff,
+                     int integrator, double dt, double gamma, double kbt, double k_b,
+                     int n_r_df, int64_t n, const double *pos, const double *vel,
+                     const int32_t *type) {
+  RefSim *s = new RefSim();
+  s->cv = new caviar2::Caviar2();
+  caviar2::Caviar2 &c = *s->cv;
+  s->integrator = integrator;
+
+  c.domain.nprocs_x = c.domain.nprocs_y = c.domain.nprocs_z = 1;
+  c.domain.grid_index_x = c.domain.grid_index_y = c.domain.grid_index_z = 0;
+  c.domain.me = 0;
+  c.domain.nprocs = 1;
+  c.domain.set_lower_global(Vector3d<double>{lo[0], lo[1], lo[2]});
+  c.domain.set_upper_global(Vector3d<double>{hi[0], hi[1], hi[2]});
+  c.domain.set_boundary_condition(Vector3d<int>{bc[0], bc[1], bc[2]});
+  c.domain.generate();
+
+  c.atom_data.set_record_owned_position_old(false);
+  c.atom_data.set_record_owned_velocity_old(false);
+  c.atom_data.set_record_owned_acceleration_old(false);
+  c.atom_data.make_ghost_velocity = false;
+  c.atom_data.stochastic_force_present = (integrator == 1);
+  c.atom_data.set_n_r_df(n_r_df);
+  c.atom_data.set_ghost_cutoff(ghost_cutoff);
+  for (int64_t i = 0; i < n; ++i) {
+    c.atom_data.add_atom((size_t)i, (size_t)type[i],
+                         Vector3d<double>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]},
+                         Vector3d<double>{vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]});
+  }
+  for (int t = 0; t < ntypes; ++t) c.atom_data.add_type_mass(t, mass[t]);
+  c.atom_data.set_k_b(k_b);
+  c.atom_data.set_temperature_process(true);
+
+  c.neighborlist.set_cutoff(nl_cutoff);
+  c.neighborlist.set_cutoff_extra_coef(nl_coef);
+  c.neighborlist.set_dt(dt);
+  if (use_cell_list) c.neighborlist.set_make_verlet_list_from_cell_list(true);
+  if (rebuild_test) c.neighborlist.set_rebuild_test(true);
+
+  s->lj = new caviar2::force_field::Lj(&c);
+  s->lj->set_cutoff(lj_cutoff);
+  for (int a = 0; a < ntypes; ++a)
+    for (int b = 0; b < ntypes; ++b) {
+      s->lj->set_sigma(a, b, sigma[a * ntypes + b]);
+      s->lj->set_epsilon(a, b, eps[a * ntypes + b]);
+    }
+  if (cutoff_list) {
+    std::vector<std::vector<double>> cl(ntypes, std::vector<double>(ntypes));
+    for (int a = 0; a < ntypes; ++a)
+      for (int b = 0; b < ntypes; ++b) cl[a][b] = cutoff_list[a * ntypes + b];
+    s->lj->set_cutoff_list(cl);
+  }
+  c.add_force_field(s->lj);
+
+  caviar2::Md_simulator &md = c.md_simulator;
+  md.initialized = false;
+  md.use_time = false;
+  md.current_step = 0;
+  md.time = 0.0;
+  md.set_integrator_type(integrator == 1 ? caviar2::Integrator_t::Velocity_verlet_langevin
+                                         : caviar2::Integrator_t::Velocity_verlet);
+  md.set_initial_step(0);
+  md.set_final_step(1);
+  md.set_dt(dt);
+  md.langevin_param.friction = gamma;
+  md.langevin_param.kbt = kbt;
+  md.langevin_param.kb = 0.0;
+  md.langevin_param.temperature = 0.0;
+  md.langevin_param.rnd_tools_x.seed(1);
+  md.langevin_param.rnd_tools_y.seed(2);
+  md.langevin_param.rnd_tools_z.seed(3);
+  return s;
+}
+
+void ref_sim_destroy(void *h) {
+  RefSim *s = static_cast<RefSim *>(h);
+  delete s->lj;
+  delete s->cv;
+  delete s;
+}
+
+// Md_simulator::step() setup branch (md_simulator.cpp:445-467) without the
+// first integration step.
+void ref_sim_setup(void *h) {
+  RefSim *s = static_cast<RefSim *>(h);
+  caviar2::Md_simulator &md = s->cv->md_simulator;
+  md.initialize();
+  s->cv->atom_data.reset_virial();
+  md.setup();
+}
+
+// One Md_simulator::step(); returns 1 if the Verlet list was rebuilt.
+int ref_sim_step(void *h) {
+  RefSim *s = static_cast<RefSim *>(h);
+  s->cv->md_simulator.step();
+  s->cv->md_simulator.current_step++;
+  const auto &po = NlSpy::pos_old_of(s->cv->neighborlist);
+  const auto &p = s->cv->atom_data.atom_struct_owned.position;
+  if (po.size() != p.size()) return 1;
+  return std::memcmp(po.data(), p.data(), p.size() * sizeof(Vector3d<double>)) == 0 ? 1 : 0;
+}
+
+int64_t ref_sim_natoms(void *h) {
+  return (int64_t) static_cast<RefSim *>(h)->cv->atom_data.atom_struct_owned.position.size();
+}
+
+static void copy_vec(const std::vector<Vector3d<double>> &v, double *out) {
+  if (out) std::memcpy(out, v.data(), v.size() * sizeof(Vector3d<double>));
+}
+
+void ref_sim_get(void *h, double *pos, double *vel, double *acc) {
+  const auto &a = static_cast<RefSim *>(h)->cv->atom_data.atom_struct_owned;
+  copy_vec(a.position, pos);
+  copy_vec(a.velocity, vel);
+  copy_vec(a.acceleration, acc);
+}
+
+void ref_sim_thermo(void *h, double *ke, double *temperature, double *virial) {
+  caviar2::Atom_data &ad = static_cast<RefSim *>(h)->cv->atom_data;
+  if (ke) *ke = ad.kinetic_energy();
+  if (temperature) *temperature = ad.temperature();
+  if (virial) *virial = ad.virialForce;
+}
+
+// Langevin noise of the last step (md_simulator.cpp:748-757), AoS 3n.
+void ref_sim_eta(void *h, double *eta) {
+  const auto &L = static_cast<RefSim *>(h)->cv->md_simulator.langevin_param;
+  for (size_t i = 0; i < L.eta_x.size(); ++i) {
+    eta[3 * i] = L.eta_x[i];
+    eta[3 * i + 1] = L.eta_y[i];
+    eta[3 * i + 2] = L.eta_z[i];
+  }
+}
+
+int64_t ref_sim_nghost(void *h) {
+  return (int64_t) static_cast<RefSim *>(h)->cv->atom_data.atom_struct_ghost.position.size();
+}
+
+void ref_sim_ghosts(void *h, double *gpos, int64_t *gid) {
+  const auto &g = static_cast<RefSim *>(h)->cv->atom_data.atom_struct_ghost;
+  copy_vec(g.position, gpos);
+  if (gid)
+    for (size_t i = 0; i < g.id.size(); ++i) gid[i] = (int64_t)g.id[i];
+}
+
+// Neighbour list as CSR; returns the number of entries (writes at most cap).
+int64_t ref_sim_nlist(void *h, int64_t cap, int64_t *offs, uint32_t *nbr) {
+  const auto &nl = static_cast<RefSim *>(h)->cv->neighborlist.neighlist;
+  int64_t k = 0;
+  for (size_t i = 0; i < nl.size(); ++i) {
+    if (offs) offs[i] = k;
+    for (unsigned j : nl[i]) {
+      if (k < cap) nbr[k] = j;
+      ++k;
+    }
+  }
+  if (offs) offs[nl.size()] = k;
+  return k;
+}
+
+// Atom_data::add_random_velocity (atom_data.cpp:713-734) on a fresh
+// 1-type system: the libstdc++ stream the drop-in host API must reproduce.
+void ref_random_velocity(int64_t n, unsigned seed, double amplitude, double *vel) {
+  caviar2::Caviar2 c;
+  c.domain.nprocs_x = c.domain.nprocs_y = c.domain.nprocs_z = 1;
+  c.domain.grid_index_x = c.domain.grid_index_y = c.domain.grid_index_z = 0;
+  c.domain.me = 0;
+  c.domain.nprocs = 1;
+  c.domain.set_lower_global(Vector3d<double>{0, 0, 0});
+  c.domain.set_upper_global(Vector3d<double>{1, 1, 1});
+  c.domain.set_boundary_condition(Vector3d<int>{1, 1, 1});
+  c.domain.generate();
+  c.atom_data.set_record_owned_position_old(false);
+  c.atom_data.set_record_owned_velocity_old(false);
+  c.atom_data.set_record_owned_acceleration_old(false);
+  for (int64_t i = 0; i < n; ++i)
+    c.atom_data.add_atom((size_t)i, 0, Vector3d<double>{0.5, 0.5, 0.5}, Vector3d<double>{0, 0, 0});
+  c.atom_data.add_type_mass(0, 1.0);
+  c.atom_data.add_random_velocity(seed, amplitude);
+  copy_vec(c.atom_data.atom_struct_owned.velocity, vel);
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""Pins for the oracle (the CPU restatement the HIP path is checked against).
+
+1. Bit-exact against golden fixtures written by the compiled reference
+   (tests/golden/make_golden.py).
+2. Against the compiled reference directly on fresh random inputs, where
+   oracle/_ref is built (this container).
+3. Closed forms that a plausible slip (dropped term, wrong sign, index or
+   operand order) would break: the dimer force, FCC shell counts, lattice
+   symmetry, Newton's third law, and the potential energy as the integral of
+   the force.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import sysdef_from
+from cv2md.systems import fcc_lattice, ka_mixture, lj_fluid
+
+
+def _box(s):
+    return O.make_box(s["lo"], s["hi"], s["bc"])
+
+
+@pytest.mark.parametrize("name", ["c1_setup_brute", "c1_setup_cell", "ka_setup_cell"])
+def test_setup_matches_reference(golden, name):
+    d = golden(name)
+    s = sysdef_from(d)
+    box = _box(s)
+    pos, _ = O.wrap(s["pos"], box)
+    assert np.array_equal(pos, d["pos"])
+    gpos, owner, code = O.ghosts(pos, box, s["ghost_cutoff"])
+    assert np.array_equal(gpos, d["gpos"])
+    assert np.array_equal(owner, d["gid"])  # ids are insertion indices
+    if s["use_cell_list"]:
+        offs, nbr = O.nlist_cell(pos, gpos, box, s["nl_cutoff"], s["nl_coef"])
+    else:
+        offs, nbr = O.nlist_brute(pos, gpos, s["nl_cutoff"], s["nl_coef"])
+    assert np.array_equal(offs, d["offs"]) and np.array_equal(nbr, d["nbr"])
+    n = len(pos)
+    ids = np.arange(n)
+    acc, vir, pe = O.lj(pos, s["type"], ids, gpos, s["type"][owner], owner, offs, nbr,
+                        s["eps"], s["sigma"], s["lj_cutoff"], s["mass"], s.get("cutoff_list"))
+    assert np.array_equal(acc, d["acc"])
+    assert vir == d["virial"]
+    assert O.kinetic_energy(s["vel"], s["type"], s["mass"], s["n_r_df"]) == d["ke"]
+
+
+@pytest.mark.parametrize("name", ["c1_traj_brute", "c1_traj_cell", "dimer_traj", "grid512_traj",
+                                  "langevin_traj"])
+def test_trajectory_matches_reference(golden, name):
+    d = golden(name)
+    s = sysdef_from(d)
+    keep = set(int(k) for k in d["keep"])
+    with O.OracleSim(s) as o:
+        o.setup()
+        for k in range(1, int(d["nsteps"]) + 1):
+            eta = d["eta"][k - 1] if "eta" in d else None
+            assert o.step(eta) == bool(d["rebuilt"][k - 1]), f"rebuild flag differs at step {k}"
+            if k in keep:
+                p, v, a = o.state()
+                assert np.array_equal(p, d[f"pos_{k}"]), f"pos differs at step {k}"
+                assert np.array_equal(v, d[f"vel_{k}"]), f"vel differs at step {k}"
+                assert np.array_equal(a, d[f"acc_{k}"]), f"acc differs at step {k}"
+            if s["integrator"] == 0:
+                assert o.thermo()["ke"] == d["ke"][k - 1]
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_systems_match_compiled_reference(seed):
+    rng = np.random.default_rng(seed)
+    s = ka_mixture(int(rng.integers(300, 700)), rho=float(rng.uniform(0.7, 1.2)), seed=seed)
+    s["integrator"] = 0
+    s["n_r_df"] = 3
+    s["use_cell_list"] = seed % 2
+    with O.RefSim(s) as r, O.OracleSim(s) as o:
+        r.setup(); o.setup()
+        for _ in range(15):
+            assert r.step() == o.step()
+        pr, vr, ar = r.state(); po, vo, ao = o.state()
+        assert np.array_equal(pr, po) and np.array_equal(vr, vo) and np.array_equal(ar, ao)
+        assert r.thermo()["virial"] == o.thermo()["virial"]
+
+
+def test_dimer_closed_form():
+    # F on i = forceCoef*(r_j - r_i), forceCoef = 4e(-12 s^12/r^14 + 6 s^6/r^8 + 12 s^12/rc^14 - 6 s^6/rc^8)
+    for r in (0.95, 1.0, 2.0 ** (1 / 6), 1.5, 2.4):
+        pos = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+        offs = np.array([0, 1, 1]); nbr = np.array([1], np.uint32)
+        acc, vir, pe = O.lj(pos, [0, 0], [0, 1], None, None, None, offs, nbr,
+                            [[1.0]], [[1.0]], 2.5, [1.0])
+        rc = 2.5
+        fc = 4 * (-12 / r ** 14 + 6 / r ** 8 + 12 / rc ** 14 - 6 / rc ** 8)
+        assert acc[0, 0] == pytest.approx(fc * r, rel=1e-14, abs=1e-14)
+        assert acc[1, 0] == pytest.approx(-fc * r, rel=1e-14, abs=1e-14)
+        assert vir == pytest.approx(-fc * r * r, rel=1e-14, abs=1e-14)
+        u = 4 * (r ** -12 - r ** -6) + 4 * (6 * rc ** -12 - 3 * rc ** -6) * (r / rc) ** 2 \
+            - 4 * (7 * rc ** -12 - 4 * rc ** -6)
+        assert pe == pytest.approx(u, rel=1e-13, abs=1e-13)
+
+
+def test_pair_energy_is_integral_of_force():
+    # U(rc) = 0 and dU/dr = forceCoef * r (central finite differences)
+    rc2 = 2.5 ** 2
+    assert abs(O.pair_energy(rc2, 1.0, 1.0, rc2)) < 1e-15
+    for eps, sig in ((1.0, 1.0), (1.5, 0.8), (0.5, 0.88)):
+        for r in np.linspace(0.8 * sig, 2.4 * sig, 9):
+            h = 1e-6
+            du = (O.pair_energy((r + h) ** 2, eps, sig, rc2) - O.pair_energy((r - h) ** 2, eps, sig, rc2)) / (2 * h)
+            fc = 4 * eps * (-12 * sig ** 12 / r ** 14 + 6 * sig ** 6 / r ** 8 + 12 * sig ** 12 / rc2 ** 7 - 6 * sig ** 6 / rc2 ** 4)
+            assert du == pytest.approx(fc * r, rel=1e-6, abs=1e-7)
+
+
+def test_fcc_shells_and_symmetry():
+    pos, L = fcc_lattice(6, 0.8442)
+    box = O.make_box([0, 0, 0], [L, L, L])
+    gpos, owner, _ = O.ghosts(pos, box, 2.8000000000000003)
+    for cut, coef, expect in ((2.5, 1.12, 78), (2.5, 1.0, 54)):
+        offs, nbr = O.nlist_cell(pos, gpos, box, cut, coef)
+        # half list + all ghosts: count pairs per atom from both sides
+        n = len(pos)
+        cnt = np.diff(offs).copy()
+        own = nbr[nbr < n]
+        np.add.at(cnt, own, 1)
+        assert np.all(cnt == expect), (cut * coef, np.unique(cnt))
+    offs, nbr = O.nlist_brute(pos, gpos, 2.5, 1.12)
+    acc, vir, pe = O.lj(pos, np.zeros(len(pos)), np.arange(len(pos)), gpos, np.zeros(len(gpos)), owner,
+                        offs, nbr, [[1.0]], [[1.0]], 2.5, [1.0])
+    assert np.abs(acc).max() < 1e-11                     # lattice symmetry
+    # PE per atom = sum over shells n_k U(r_k) / 2
+    a = (4 / 0.8442) ** (1 / 3)
+    shells = {1: 12, 2: 6, 3: 24, 4: 12}  # r^2 = k a^2 / 2 (the k=5 shell, r=2.656, lies beyond rc)
+    u = sum(m * O.pair_energy(k * a * a / 2, 1.0, 1.0, 6.25) for k, m in shells.items())
+    assert pe / len(pos) == pytest.approx(u / 2, rel=1e-12)
+
+
+def test_newton_third_law_and_wrap():
+    s = lj_fluid(5, seed=7)
+    rng = np.random.default_rng(3)
+    box = _box(s)
+    p = s["pos"] + rng.normal(0, 0.05, s["pos"].shape)
+    p[0, 0] = -1e-9          # just below lower: wraps by +L
+    p[1, 1] = s["hi"][1]     # exactly upper: stays (strict >)
+    pw, flag = O.wrap(p, box)
+    assert flag and pw[0, 0] == -1e-9 + s["hi"][0] and pw[1, 1] == s["hi"][1]
+    gpos, owner, code = O.ghosts(pw, box, 2.8)
+    L = s["hi"] - s["lo"]
+    shift = np.stack([(code % 3) - 1, (code // 3) % 3 - 1, code // 9 - 1], 1)
+    assert np.array_equal(gpos, pw[owner] - shift * L)
+    offs, nbr = O.nlist_brute(pw, gpos, 2.5, 1.12)
+    acc, _, _ = O.lj(pw, np.zeros(len(pw)), np.arange(len(pw)), gpos, np.zeros(len(gpos)), owner,
+                     offs, nbr, [[1.0]], [[1.0]], 2.5, [1.0])
+    assert np.abs(acc.sum(0)).max() < 1e-9
+
+
+def test_cell_list_equals_brute_force_as_sets():
+    s = ka_mixture(800, rho=0.9, seed=11)
+    box = _box(s)
+    pos, _ = O.wrap(s["pos"], box)
+    gpos, _, _ = O.ghosts(pos, box, s["ghost_cutoff"])
+    a = O.nlist_brute(pos, gpos, s["nl_cutoff"], s["nl_coef"])
+    b = O.nlist_cell(pos, gpos, box, s["nl_cutoff"], s["nl_coef"])
+    n = len(pos)
+    for i in range(n):
+        assert sorted(a[1][a[0][i]:a[0][i + 1]]) == sorted(b[1][b[0][i]:b[0][i + 1]])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
+def test_random_velocity_fixture(golden):
+    d = golden("random_velocity")
+    assert np.array_equal(O.ref_random_velocity(int(d["n"]), int(d["seed"]), float(d["amplitude"])), d["vel"])
