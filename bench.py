@@ -1,0 +1,189 @@
+"""Benchmark of the MD step on one MI355X.
+
+Workload (default): 2,048,000-atom LJ fluid, 80^3 FCC cells, rho*=0.8442,
+T*=1.0, rc=2.5, skin 0.3, Langevin gamma=1.0, dt=0.005, FP32 pair
+arithmetic with 64-bit fixed-point accumulation, FP64 state.  The same
+workload is also timed with the plain FP64 kernels; the FP32 forces are
+compared with the FP64 ones at one configuration.  A bounded sample of the
+same workload runs on the host CPU (the compiled reference where available,
+else the oracle restatement) for the cpu_baseline entry.
+
+Prints one JSON line.  Only --gpus 1 is supported (one GPU per box).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "caviar-v2_amd"))
+
+from cv2md import device as D  # noqa: E402
+from cv2md.systems import lj_fluid  # noqa: E402
+
+METRIC = "atom-timesteps/s, 2M-atom LJ: FP32+fixed-point vs FP64 GPU at force err ≤1e-6"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(ncell):
+    return lj_fluid(ncell, rho=0.8442, temperature=1.0, seed=1003, integrator=1, gamma=1.0)
+
+
+def run_gpu(s, precision, steps, warmup, seed=7):
+    n = len(s["pos"])
+    with D.Context.from_sysdef(s, precision=precision) as c:
+        c.compute_forces()
+        c.run_langevin(warmup, s["dt"], s["gamma"], s["kbt"], seed)
+        c.synchronize()
+        c.set_timing(True)
+        st0 = c.stats()
+        t0 = time.perf_counter()
+        c.run_langevin(steps, s["dt"], s["gamma"], s["kbt"], seed)
+        c.synchronize()
+        t1 = time.perf_counter()
+        tm = c.timing()
+        st = c.stats()
+        th = c.thermo(n_r_df=0)
+        pos = c.download()[0]
+    el = t1 - t0
+    return dict(value=n * steps / el, ms_per_step=1e3 * el / steps, timing=tm,
+                rebuilds=st["rebuilds"] - st0["rebuilds"], mean_list=st["list_entries"] / n,
+                max_list=st["max_neighbors"], capacity=st["capacity"],
+                temperature=2 * th["ke"] / (3 * n), pos=pos)
+
+
+def force_error(s, pos):
+    """FP32+fixed-point accelerations against the FP64 kernel at one configuration."""
+    s2 = dict(s, pos=pos)
+    acc = {}
+    for prec in (D.CV2_PREC_FP64, D.CV2_PREC_FP32_FIXED):
+        with D.Context.from_sysdef(s2, precision=prec) as c:
+            c.compute_forces()
+            acc[prec] = c.download()[2]
+    ref = acc[D.CV2_PREC_FP64]
+    d = np.linalg.norm(acc[D.CV2_PREC_FP32_FIXED] - ref, axis=1)
+    m = np.linalg.norm(ref, axis=1)
+    rms = float(np.sqrt((m ** 2).mean()))
+    return dict(max_rel=float((d / np.maximum(m, rms)).max()), rms_rel=float(np.sqrt((d ** 2).mean()) / rms),
+                metric="|a32-a64|/max(|a64|, rms|a64|), per atom")
+
+
+def cpu_baseline(s, budget_s):
+    """Bounded sample of the same workload on one host core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    n = len(s["pos"])
+    s = dict(s, use_cell_list=1)
+    kind = "reference" if O.ref_available() else "port"
+    sim = O.RefSim(s) if kind == "reference" else O.OracleSim(s)
+    rng = np.random.default_rng(5)
+    t_setup = time.perf_counter()
+    sim.setup()
+    t_setup = time.perf_counter() - t_setup
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        if kind == "reference":
+            sim.step()
+        else:
+            sim.step(rng.normal(size=(n, 3)))
+        steps += 1
+        if time.perf_counter() - t0 > budget_s or steps >= 50:
+            break
+    el = time.perf_counter() - t0
+    sim.close()
+    return dict(value=n * steps / el, unit="atom-timesteps/s", cores=1, kind=kind,
+                sample=f"{steps} Langevin steps of the same {n}-atom system after an untimed setup "
+                       f"({t_setup:.1f} s), cell-list Verlet path, 1 thread, 1 process (MPI ranks: 1)",
+                cpu=platform.processor() or platform.machine(), nproc=os.cpu_count())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--ncell", type=int, default=80, help="FCC cells per edge (80 -> 2,048,000 atoms)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); multi-GPU sharding is a design "
+              "note in DESIGN.md, not a measured configuration", file=sys.stderr)
+        sys.exit(2)
+
+    s = workload(a.ncell)
+    n = len(s["pos"])
+    log(f"workload: {n} atoms, L={s['hi'][0]:.4f}")
+    r32 = run_gpu(s, D.CV2_PREC_FP32_FIXED, a.steps, a.warmup)
+    log(f"fp32+fx: {r32['ms_per_step']:.4f} ms/step, rebuilds {r32['rebuilds']}, <n> {r32['mean_list']:.2f}")
+    r64 = None
+    if not a.no_fp64:
+        r64 = run_gpu(s, D.CV2_PREC_FP64, a.steps, a.warmup)
+        log(f"fp64: {r64['ms_per_step']:.4f} ms/step, rebuilds {r64['rebuilds']}")
+    ferr = force_error(s, r32["pos"])
+    log(f"force error: {ferr}")
+
+    tm = r32["timing"]
+    t_force = tm["force_ms"] / max(tm["force_calls"], 1) * 1e-3
+    nl = r32["mean_list"]
+    # q (int4) + nnbr + list entries + acc (3 x f64) per atom; neighbour
+    # positions are the same array, counted once
+    b_force = n * (16 + 4 + 4 * nl + 24)
+    achieved = b_force / t_force / 1e9
+    out = {
+        "metric": METRIC,
+        "value": r32["value"],
+        "unit": "atom-timesteps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"LJ fluid {n} atoms ({a.ncell}^3 FCC), rho*=0.8442, T*=1.0, rc=2.5, skin 0.3, "
+                        "Langevin gamma=1.0, dt=0.005; FP32 pair arithmetic, 64-bit fixed-point "
+                        "accumulation, FP64 state",
+            "natoms": n,
+            "precision": "fp32+fixed64",
+            "mean_list_length": nl,
+            "list": "full (both directions), transposed ELL",
+            "rebuilds_in_timed_steps": r32["rebuilds"],
+        },
+        "fp64_gpu": None if r64 is None else {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
+                                             "force_ms": r64["timing"]["force_ms"] / max(r64["timing"]["force_calls"], 1),
+                                             "rebuilds_in_timed_steps": r64["rebuilds"]},
+        "speedup_fp32fx_vs_fp64": None if r64 is None else r32["value"] / r64["value"],
+        "force_error_vs_fp64": ferr,
+        "phase_ms_per_step": {k: tm[k] / a.steps for k in ("force_ms", "build_ms", "integrate_ms")},
+        "temperature_end": r32["temperature"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_lj_f32fx", "bytes_per_launch": b_force,
+                     "avg_launch_ms": t_force * 1e3},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(s, a.cpu_seconds)
+        log(f"cpu: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

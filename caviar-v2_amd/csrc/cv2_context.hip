@@ -1,0 +1,725 @@
+// cv2_context.hip -- the C ABI (include/cv2_md.h): context, buffers, step
+// sequencing.  All device work of a context is queued on its own stream;
+// the host waits only where a result must come back (thermo, download,
+// error flags at the end of a run).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cv2_internal.h"
+#include "cv2_md.h"
+
+using namespace cv2;
+
+struct cv2_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // configuration
+  bool box_set = false, lj_set = false, nl_set = false;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  int bc[3] = {1, 1, 1};
+  double ghost_cut = 0.0;
+  double nl_cutoff = 0.0, nl_coef = 1.0;
+  unsigned nl_flags = 0;
+  int precision = CV2_PREC_FP64;
+  LJDev lj{};
+  LJDev *lj_dev = nullptr;
+  // state
+  DevState s{};
+  BoxDev b{};
+  int64_t n = 0, ng = 0;
+  int64_t pos_cap = 0;  // capacity of the owned+ghost position arrays
+  bool host_halo = false;
+  bool forces_valid = false;
+  bool need_build = true;  // configuration changed since the last forced build
+  bool atoms = false;
+  int64_t steps = 0;
+  double virial = 0.0, pe = 0.0;
+  int *h_flags = nullptr;
+  double *h_red = nullptr;
+  long long *h_fx = nullptr;
+  int partial_blocks = 0;
+  // HIP-event timing of kernel groups (cv2_set_timing)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct Span {
+    hipEvent_t a, b;
+    int kind;  // 0 force, 1 build, 2 integrate
+  };
+  std::vector<Span> spans;
+  double t_ms[3] = {0, 0, 0};
+  int64_t force_calls = 0;
+};
+
+namespace {
+
+hipEvent_t next_event(cv2_ctx *c) {
+  if (c->ev_used == c->ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->ev_pool.push_back(e);
+  }
+  return c->ev_pool[c->ev_used++];
+}
+
+// Open a timed span on the context stream (no-op when timing is off).
+int span_begin(cv2_ctx *c) {
+  if (!c->timing) return -1;
+  hipEvent_t a = next_event(c), b = next_event(c);
+  if (!a || !b) return -1;
+  (void)hipEventRecord(a, c->stream);
+  c->spans.push_back({a, b, 0});
+  return (int)c->spans.size() - 1;
+}
+
+void span_end(cv2_ctx *c, int k, int kind) {
+  if (k < 0) return;
+  c->spans[k].kind = kind;
+  (void)hipEventRecord(c->spans[k].b, c->stream);
+  if (kind == 0) c->force_calls++;
+}
+
+// Fold finished spans into the totals (stream must be synchronized).
+void spans_collect(cv2_ctx *c) {
+  for (const auto &s : c->spans) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) c->t_ms[s.kind] += ms;
+  }
+  c->spans.clear();
+  c->ev_used = 0;
+}
+
+}  // namespace
+
+namespace {
+
+int fail(cv2_ctx *c, int code, const std::string &m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(ctx, CV2_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+  if (*p) {
+    hipError_t e = hipFree(*p);
+    *p = nullptr;
+    if (e != hipSuccess) return e;
+  }
+  if (count == 0) count = 1;
+  return hipMalloc((void **)p, sizeof(T) * count);
+}
+
+void free_state(DevState &s) {
+  void *ptrs[] = {s.pos, s.pos_alt, s.pos_ref, s.q, s.vel, s.vel_alt, s.acc, s.acc_alt, s.tag,
+                  s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.cell_count, s.cell_start,
+                  s.gcell_count, s.gcell_start, s.gcell_items, s.nbr, s.nnbr, s.flags,
+                  s.partials, s.fx_acc, s.reduce_out};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  s = DevState{};
+}
+
+// Cell grid and box constants; requires box, neighbour settings.
+int setup_box(cv2_ctx *ctx) {
+  BoxDev &b = ctx->b;
+  b = BoxDev{};
+  const double ce = ctx->nl_cutoff * ctx->nl_coef;  // neighborlist.cpp:317
+  b.ce_sq = ce * ce;
+  b.thresh_sq = 0.25 * (ce - ctx->nl_cutoff) * (ce - ctx->nl_cutoff);  // :318
+  b.ghost_cut = ctx->ghost_cut;
+  int64_t ncell = 1;
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = ctx->lo[d];
+    b.hi[d] = ctx->hi[d];
+    b.L[d] = ctx->hi[d] - ctx->lo[d];
+    b.halfL[d] = 0.5 * b.L[d];
+    b.bc[d] = ctx->host_halo ? 0 : ctx->bc[d];
+    const double margin = b.bc[d] ? 0.0 : ctx->ghost_cut;
+    b.glo[d] = b.lo[d] - margin;
+    const double ext = b.L[d] + 2.0 * margin;
+    int nc = (int)std::floor(ext / (ce * (1.0 + 1e-12)));
+    nc = std::max(1, std::min(nc, 1024));
+    b.nc[d] = nc;
+    b.cinv[d] = nc / ext;
+    ncell *= nc;
+    b.qscale[d] = 4294967296.0 / b.L[d];
+    b.qinv[d] = b.L[d] / 4294967296.0;
+    if (ctx->bc[d] && !ctx->host_halo && b.L[d] <= 2.0 * ce)
+      return fail(ctx, CV2_ERR_BAD_ARG, "periodic box must be wider than twice the list cutoff");
+  }
+  if (ncell > (1 << 26)) return fail(ctx, CV2_ERR_BAD_ARG, "cell grid too large");
+  if (ncell != ctx->s.ncell) {
+    ctx->s.ncell = (int)ncell;
+    HIPCHK(dalloc(&ctx->s.cell_count, ncell + 1));
+    HIPCHK(dalloc(&ctx->s.cell_start, ncell + 1));
+    HIPCHK(dalloc(&ctx->s.gcell_count, ncell + 1));
+    HIPCHK(dalloc(&ctx->s.gcell_start, ncell + 1));
+    HIPCHK(hipMemsetAsync(ctx->s.gcell_start, 0, sizeof(int) * (ncell + 1), ctx->stream));
+  }
+  return CV2_OK;
+}
+
+int ensure_ready(cv2_ctx *ctx) {
+  if (!ctx->box_set || !ctx->lj_set || !ctx->nl_set || !ctx->atoms)
+    return fail(ctx, CV2_ERR_STATE, "box, LJ parameters, neighbour settings and atoms must be set first");
+  return CV2_OK;
+}
+
+int check_flags(cv2_ctx *ctx) {
+  HIPCHK(hipMemcpyAsync(ctx->h_flags, ctx->s.flags, sizeof(int) * F_COUNT, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int *f = ctx->h_flags;
+  if (f[F_OVERFLOW])
+    return fail(ctx, CV2_ERR_CAPACITY,
+                "neighbour list capacity " + std::to_string(ctx->s.cap) + " exceeded (longest list " +
+                    std::to_string(f[F_MAXNBR]) + ")");
+  if (f[F_NAN]) return fail(ctx, CV2_ERR_NAN, "NaN acceleration");
+  if (f[F_FXOVF]) return fail(ctx, CV2_ERR_FIXED_OVERFLOW, "fixed-point force range exceeded (r < sigma/2)");
+  return CV2_OK;
+}
+
+int fetch_energy(cv2_ctx *ctx) {
+  if (ctx->precision == CV2_PREC_FP64) {
+    HIPCHK(hipMemcpyAsync(ctx->h_red, ctx->s.reduce_out, sizeof(double) * 2, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->virial = ctx->h_red[0];
+    ctx->pe = ctx->h_red[1];
+  } else {
+    HIPCHK(hipMemcpyAsync(ctx->h_fx, ctx->s.fx_acc, sizeof(long long) * 2, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->virial = (double)ctx->h_fx[0] / kEnergyScale;
+    ctx->pe = (double)ctx->h_fx[1] / kEnergyScale;
+  }
+  return CV2_OK;
+}
+
+void queue_build(cv2_ctx *ctx, bool force) {
+  force = force || ctx->need_build;
+  ctx->need_build = false;
+  if (force || (ctx->nl_flags & CV2_NL_REBUILD_EVERY_STEP))
+    launch_set_flag(ctx->s.flags, F_REBUILD, 1, ctx->stream);
+  launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo);
+}
+
+int alloc_positions(cv2_ctx *ctx, int64_t ntot) {
+  if (ntot <= ctx->pos_cap) return CV2_OK;
+  const int64_t cap = ntot;
+  DevState &s = ctx->s;
+  // keep the owned part when growing for ghosts
+  std::vector<double4> keep;
+  std::vector<int> keep_id;
+  if (ctx->pos_cap > 0 && ctx->n > 0) {
+    keep.resize(ctx->n);
+    keep_id.resize(ctx->n);
+    HIPCHK(hipMemcpyAsync(keep.data(), s.pos, sizeof(double4) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(keep_id.data(), s.id, sizeof(int) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  HIPCHK(dalloc(&s.pos, cap));
+  HIPCHK(dalloc(&s.pos_ref, cap));
+  HIPCHK(dalloc(&s.q, cap));
+  HIPCHK(dalloc(&s.id, cap));
+  HIPCHK(dalloc(&s.cell, cap));
+  HIPCHK(dalloc(&s.slot, cap));
+  HIPCHK(dalloc(&s.gcell_items, cap));
+  if (!keep.empty()) {
+    HIPCHK(hipMemcpyAsync(s.pos, keep.data(), sizeof(double4) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(s.id, keep_id.data(), sizeof(int) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->pos_cap = cap;
+  return CV2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cv2_create(cv2_ctx **out, int device) {
+  if (!out) return CV2_ERR_BAD_ARG;
+  *out = nullptr;
+  cv2_ctx *ctx = new cv2_ctx();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void **)&ctx->lj_dev, sizeof(LJDev));
+  if (e == hipSuccess) e = hipMalloc((void **)&ctx->s.flags, sizeof(int) * F_COUNT);
+  if (e == hipSuccess) e = hipMemset(ctx->s.flags, 0, sizeof(int) * F_COUNT);
+  if (e == hipSuccess) e = hipMalloc((void **)&ctx->s.reduce_out, sizeof(double) * 16);
+  if (e == hipSuccess) e = hipMalloc((void **)&ctx->s.fx_acc, sizeof(long long) * 4);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_flags, sizeof(int) * F_COUNT, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_red, sizeof(double) * 16, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_fx, sizeof(long long) * 4, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    fprintf(stderr, "cv2_create: %s\n", hipGetErrorString(e));
+    cv2_destroy(ctx);
+    return CV2_ERR_HIP;
+  }
+  *out = ctx;
+  return CV2_OK;
+}
+
+int cv2_destroy(cv2_ctx *ctx) {
+  if (!ctx) return CV2_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_state(ctx->s);
+  if (ctx->lj_dev) (void)hipFree(ctx->lj_dev);
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
+  if (ctx->h_red) (void)hipHostFree(ctx->h_red);
+  if (ctx->h_fx) (void)hipHostFree(ctx->h_fx);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CV2_OK;
+}
+
+const char *cv2_last_error(const cv2_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cv2_set_box(cv2_ctx *ctx, const double lo[3], const double hi[3], const int bc[3],
+                double ghost_cutoff) {
+  if (!ctx || !lo || !hi || !bc) return fail(ctx, CV2_ERR_BAD_ARG, "null argument");
+  for (int d = 0; d < 3; ++d) {
+    if (!(hi[d] > lo[d])) return fail(ctx, CV2_ERR_BAD_ARG, "upper must exceed lower on every axis");
+    if (bc[d] != 0 && bc[d] != 1) return fail(ctx, CV2_ERR_BAD_ARG, "boundary condition must be 0 or 1");
+    if (bc[d] == 1 && hi[d] - lo[d] <= 2.0 * ghost_cutoff)
+      return fail(ctx, CV2_ERR_BAD_ARG, "periodic box must be wider than twice the ghost cutoff");
+    ctx->lo[d] = lo[d];
+    ctx->hi[d] = hi[d];
+    ctx->bc[d] = bc[d];
+  }
+  if (ghost_cutoff < 0) return fail(ctx, CV2_ERR_BAD_ARG, "ghost cutoff must be non-negative");
+  ctx->ghost_cut = ghost_cutoff;
+  ctx->box_set = true;
+  ctx->need_build = true;
+  ctx->forces_valid = false;
+  if (ctx->nl_set) return setup_box(ctx);
+  return CV2_OK;
+}
+
+int cv2_set_lj(cv2_ctx *ctx, int ntypes, const double *eps, const double *sigma,
+               const double *cutoff_list, double cutoff, const double *mass) {
+  if (!ctx || !eps || !sigma || !mass) return fail(ctx, CV2_ERR_BAD_ARG, "null argument");
+  if (ntypes < 1 || ntypes > kMaxTypes) return fail(ctx, CV2_ERR_BAD_ARG, "ntypes must be 1..8");
+  if (cutoff < 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "Force field cutoff must be non-negative.");
+  LJDev &L = ctx->lj;
+  L = LJDev{};
+  L.nt = ntypes;
+  for (int t = 0; t < ntypes; ++t) {
+    if (!(mass[t] > 0.0)) return fail(ctx, CV2_ERR_BAD_ARG, "masses must be positive");
+    L.mass[t] = mass[t];
+    L.mass_inv[t] = 1.0 / mass[t];
+  }
+  for (int k = 0; k < ntypes * ntypes; ++k) {
+    double rc = cutoff;
+    if (cutoff_list) {
+      if (cutoff_list[k] < 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "Cutoff list values must be non-negative.");
+      rc = cutoff_list[k];
+    }
+    const double c = rc * rc;
+    const double rcinv = 1.0 / c;
+    const double s2 = sigma[k] * sigma[k];
+    const double rho_c2 = s2 * rcinv, rho_c6 = rho_c2 * rho_c2 * rho_c2, rho_c12 = rho_c6 * rho_c6;
+    L.eps4[k] = 4.0 * eps[k];
+    L.sig2[k] = s2;
+    L.rc2[k] = eps[k] == 0.0 ? -1.0 : c;  // eps == 0 pairs are skipped (lj.cpp:470-472)
+    L.fshift[k] = 12.0 * rho_c12 * rcinv - 6.0 * rho_c6 * rcinv;
+    L.ecut[k] = (6.0 * rho_c12 - 3.0 * rho_c6) * rcinv;
+    L.ecut2[k] = 7.0 * rho_c12 - 4.0 * rho_c6;
+    L.eps4f[k] = (float)L.eps4[k];
+    L.sig2f[k] = (float)L.sig2[k];
+    L.rc2f[k] = (float)L.rc2[k];
+    L.fshiftf[k] = (float)L.fshift[k];
+  }
+  ctx->lj_set = true;
+  ctx->need_build = true;
+  ctx->forces_valid = false;
+  hipError_t e = hipMemcpyAsync(ctx->lj_dev, &ctx->lj, sizeof(LJDev), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(ctx, CV2_ERR_HIP, hipGetErrorString(e));
+  return CV2_OK;
+}
+
+int cv2_set_neighbor(cv2_ctx *ctx, double cutoff, double extra_coef, unsigned flags) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  if (cutoff <= 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "cutoff have to non-negative.");
+  if (extra_coef < 1.0) return fail(ctx, CV2_ERR_BAD_ARG, "cutoff_extra_coef must be larger than 1.0");
+  ctx->nl_cutoff = cutoff;
+  ctx->nl_coef = extra_coef;
+  ctx->nl_flags = flags;
+  ctx->nl_set = true;
+  ctx->need_build = true;
+  ctx->forces_valid = false;
+  if (ctx->box_set) return setup_box(ctx);
+  return CV2_OK;
+}
+
+int cv2_set_precision(cv2_ctx *ctx, int precision) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  if (precision != CV2_PREC_FP64 && precision != CV2_PREC_FP32_FIXED)
+    return fail(ctx, CV2_ERR_BAD_ARG, "unknown precision");
+  ctx->precision = precision;
+  ctx->forces_valid = false;
+  return CV2_OK;
+}
+
+int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *vel,
+                     const int32_t *type, const int64_t *id) {
+  if (!ctx || n < 0 || (n > 0 && !pos)) return fail(ctx, CV2_ERR_BAD_ARG, "bad atom arrays");
+  if (!ctx->box_set || !ctx->nl_set || !ctx->lj_set)
+    return fail(ctx, CV2_ERR_STATE, "set box, LJ and neighbour parameters before uploading atoms");
+  if (n + ctx->ng > (int64_t)kIndexMask) return fail(ctx, CV2_ERR_BAD_ARG, "too many atoms (2^26 max)");
+  (void)hipSetDevice(ctx->device);
+  DevState &s = ctx->s;
+  const int ni = (int)n;
+  // host staging in device layout
+  std::vector<double4> p4(n);
+  std::vector<double> v3(3 * n, 0.0), zero(3 * n, 0.0);
+  std::vector<int> tg(n), idv(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const int t = type ? type[i] : 0;
+    if (t < 0 || t >= ctx->lj.nt) return fail(ctx, CV2_ERR_BAD_ARG, "atom type out of range");
+    p4[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], (double)t);
+    if (vel)
+      for (int d = 0; d < 3; ++d) v3[d * n + i] = vel[3 * i + d];
+    tg[i] = (int)i;
+    idv[i] = id ? (int)id[i] : (int)i;
+  }
+  ctx->pos_cap = 0;
+  ctx->n = n;
+  s.n = ni;
+  {
+    int rc = alloc_positions(ctx, n + ctx->ng);
+    if (rc) return rc;
+  }
+  HIPCHK(dalloc(&s.pos_alt, n));
+  HIPCHK(dalloc(&s.vel, 3 * n));
+  HIPCHK(dalloc(&s.vel_alt, 3 * n));
+  HIPCHK(dalloc(&s.acc, 3 * n));
+  HIPCHK(dalloc(&s.acc_alt, 3 * n));
+  HIPCHK(dalloc(&s.tag, n));
+  HIPCHK(dalloc(&s.tag_alt, n));
+  HIPCHK(dalloc(&s.id_alt, n));
+  HIPCHK(dalloc(&s.perm, n));
+  HIPCHK(dalloc(&s.nnbr, n));
+  // list capacity from the mean density: 1.6x the expected count + 64
+  const double vol = (ctx->hi[0] - ctx->lo[0]) * (ctx->hi[1] - ctx->lo[1]) * (ctx->hi[2] - ctx->lo[2]);
+  const double ce = ctx->nl_cutoff * ctx->nl_coef;
+  const double expect = 4.18879020478639 * ce * ce * ce * (double)n / vol;
+  int cap = (int)std::ceil(1.6 * expect + 64.0);
+  cap = (cap + 15) / 16 * 16;
+  s.cap = cap;
+  s.stride = (int)((n + 63) / 64 * 64);
+  HIPCHK(dalloc(&s.nbr, (size_t)s.cap * s.stride));
+  const int nb = (int)((n + 255) / 256) + 8;
+  if (nb > ctx->partial_blocks) {
+    HIPCHK(dalloc(&s.partials, (size_t)nb * 8));
+    ctx->partial_blocks = nb;
+  }
+  HIPCHK(hipMemcpyAsync(s.pos, p4.data(), sizeof(double4) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(s.vel, v3.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(s.acc, zero.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(s.tag, tg.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(s.id, idv.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(s.flags, 0, sizeof(int) * F_COUNT, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->atoms = true;
+  ctx->need_build = true;
+  ctx->forces_valid = false;
+  ctx->steps = 0;
+  return CV2_OK;
+}
+
+int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *type,
+                   const int64_t *id) {
+  if (!ctx || ng < 0 || (ng > 0 && !pos)) return fail(ctx, CV2_ERR_BAD_ARG, "bad ghost arrays");
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload owned atoms before ghosts");
+  (void)hipSetDevice(ctx->device);
+  const bool count_changed = ng != ctx->ng || !ctx->host_halo;
+  if (!ctx->host_halo) {
+    ctx->host_halo = true;
+    int rc = setup_box(ctx);
+    if (rc) return rc;
+  }
+  {
+    int rc = alloc_positions(ctx, ctx->n + ng);
+    if (rc) return rc;
+  }
+  std::vector<double4> p4(ng);
+  std::vector<int> idv(ng);
+  for (int64_t g = 0; g < ng; ++g) {
+    const int t = type ? type[g] : 0;
+    if (t < 0 || t >= ctx->lj.nt) return fail(ctx, CV2_ERR_BAD_ARG, "ghost type out of range");
+    p4[g] = make_double4(pos[3 * g], pos[3 * g + 1], pos[3 * g + 2], (double)t);
+    idv[g] = id ? (int)id[g] : (int)(ctx->n + g);
+  }
+  DevState &s = ctx->s;
+  HIPCHK(hipMemcpyAsync(s.pos + ctx->n, p4.data(), sizeof(double4) * ng, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(s.id + ctx->n, idv.data(), sizeof(int) * ng, hipMemcpyHostToDevice, ctx->stream));
+  ctx->ng = ng;
+  s.ng = (int)ng;
+  if (count_changed) ctx->need_build = true;  // neighborlist.cpp:118-121
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // p4/idv are stack buffers
+  return CV2_OK;
+}
+
+int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  int rc = ensure_ready(ctx);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  int before = 0;
+  if (rebuilt) {
+    HIPCHK(hipMemcpy(&before, ctx->s.flags + F_REBUILDS, sizeof(int), hipMemcpyDeviceToHost));
+  }
+  if (!force) launch_kick_drift(ctx->s, ctx->b, 0.0, ctx->stream);  // displacement check only
+  queue_build(ctx, force != 0);
+  ctx->forces_valid = false;
+  rc = check_flags(ctx);
+  if (rebuilt) *rebuilt = ctx->h_flags[F_REBUILDS] != before;
+  return rc;
+}
+
+int cv2_compute_forces(cv2_ctx *ctx) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  int rc = ensure_ready(ctx);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->need_build) queue_build(ctx, true);
+  const int t = span_begin(ctx);
+  launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, true, ctx->stream);
+  span_end(ctx, t, 0);
+  rc = check_flags(ctx);
+  if (rc) return rc;
+  rc = fetch_energy(ctx);
+  if (rc) return rc;
+  ctx->forces_valid = true;
+  return CV2_OK;
+}
+
+int cv2_run_vv(cv2_ctx *ctx, int64_t nsteps, double dt) {
+  if (!ctx || nsteps < 0 || dt < 0) return fail(ctx, CV2_ERR_BAD_ARG, "bad step arguments");
+  int rc = ensure_ready(ctx);
+  if (rc) return rc;
+  if (!ctx->forces_valid) {
+    rc = cv2_compute_forces(ctx);
+    if (rc) return rc;
+  }
+  for (int64_t k = 0; k < nsteps; ++k) {
+    int t = span_begin(ctx);
+    launch_kick_drift(ctx->s, ctx->b, dt, ctx->stream);
+    span_end(ctx, t, 2);
+    t = span_begin(ctx);
+    queue_build(ctx, false);
+    span_end(ctx, t, 1);
+    t = span_begin(ctx);
+    launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
+    span_end(ctx, t, 0);
+    t = span_begin(ctx);
+    launch_kick(ctx->s, dt, ctx->stream);
+    span_end(ctx, t, 2);
+    if (ctx->spans.size() > 4000) {
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      spans_collect(ctx);
+    }
+  }
+  ctx->steps += nsteps;
+  rc = check_flags(ctx);
+  if (rc) return rc;
+  if (nsteps > 0) return fetch_energy(ctx);
+  return CV2_OK;
+}
+
+int cv2_run_langevin(cv2_ctx *ctx, int64_t nsteps, double dt, double gamma, double kbt,
+                     uint64_t seed) {
+  if (!ctx || nsteps < 0 || dt < 0) return fail(ctx, CV2_ERR_BAD_ARG, "bad step arguments");
+  if (kbt < 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "kbt have to non-negative.");
+  int rc = ensure_ready(ctx);
+  if (rc) return rc;
+  if (!ctx->forces_valid) {
+    rc = cv2_compute_forces(ctx);
+    if (rc) return rc;
+  }
+  // md_simulator.cpp:398-405
+  const double la = (2.0 - gamma * dt) / (2.0 + gamma * dt);
+  const double lb = std::sqrt(kbt * gamma * 0.5 * dt);
+  const double lc = 2.0 * dt / (2.0 + gamma * dt);
+  for (int64_t k = 0; k < nsteps; ++k) {
+    const uint64_t step = (uint64_t)(ctx->steps + k);
+    int t = span_begin(ctx);
+    launch_langevin_1(ctx->s, ctx->b, ctx->lj_dev, dt, lb, lc, seed, step, ctx->stream);
+    span_end(ctx, t, 2);
+    t = span_begin(ctx);
+    queue_build(ctx, false);
+    span_end(ctx, t, 1);
+    t = span_begin(ctx);
+    launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
+    span_end(ctx, t, 0);
+    t = span_begin(ctx);
+    launch_langevin_2(ctx->s, ctx->lj_dev, dt, la, lb, seed, step, ctx->stream);
+    span_end(ctx, t, 2);
+    if (ctx->spans.size() > 4000) {
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      spans_collect(ctx);
+    }
+  }
+  ctx->steps += nsteps;
+  rc = check_flags(ctx);
+  if (rc) return rc;
+  if (nsteps > 0) return fetch_energy(ctx);
+  return CV2_OK;
+}
+
+int cv2_thermo(cv2_ctx *ctx, int n_r_df, double *ke, double *pe, double *virial) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  int rc = ensure_ready(ctx);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  if (ke) {
+    launch_thermo(ctx->s, ctx->lj_dev, ctx->stream);
+    HIPCHK(hipMemcpyAsync(ctx->h_red, ctx->s.reduce_out, sizeof(double) * 8, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const double *r = ctx->h_red + 2;  // sum m v^2, sum m v (3), sum m
+    double e = r[0];
+    if (n_r_df >= 1 && n_r_df <= 3 && r[4] > 0.0)  // kinetic_energy.cpp:56-70
+      e -= (r[1] * r[1] + r[2] * r[2] + r[3] * r[3]) / r[4];
+    *ke = 0.5 * e;
+  }
+  if (pe) *pe = ctx->pe;
+  if (virial) *virial = ctx->virial;
+  return CV2_OK;
+}
+
+int cv2_download(cv2_ctx *ctx, double *pos, double *vel, double *acc) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "no atoms");
+  (void)hipSetDevice(ctx->device);
+  const int64_t n = ctx->n;
+  std::vector<double4> p4(n);
+  std::vector<double> v(3 * n), a(3 * n);
+  std::vector<int> tg(n);
+  DevState &s = ctx->s;
+  HIPCHK(hipMemcpyAsync(tg.data(), s.tag, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+  if (pos) HIPCHK(hipMemcpyAsync(p4.data(), s.pos, sizeof(double4) * n, hipMemcpyDeviceToHost, ctx->stream));
+  if (vel) HIPCHK(hipMemcpyAsync(v.data(), s.vel, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+  if (acc) HIPCHK(hipMemcpyAsync(a.data(), s.acc, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t o = tg[k];
+    if (pos) {
+      pos[3 * o] = p4[k].x;
+      pos[3 * o + 1] = p4[k].y;
+      pos[3 * o + 2] = p4[k].z;
+    }
+    for (int d = 0; d < 3; ++d) {
+      if (vel) vel[3 * o + d] = v[d * n + k];
+      if (acc) acc[3 * o + d] = a[d * n + k];
+    }
+  }
+  return CV2_OK;
+}
+
+int cv2_get_pairs(cv2_ctx *ctx, int64_t cap, int64_t *npairs, int64_t *pairs) {
+  if (!ctx || !npairs || (cap > 0 && !pairs)) return fail(ctx, CV2_ERR_BAD_ARG, "bad pair buffer");
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "no atoms");
+  (void)hipSetDevice(ctx->device);
+  const int64_t n = ctx->n;
+  DevState &s = ctx->s;
+  std::vector<int> cnt(n), tg(n);
+  std::vector<uint32_t> nb((size_t)s.cap * s.stride);
+  HIPCHK(hipMemcpyAsync(cnt.data(), s.nnbr, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(tg.data(), s.tag, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(nb.data(), s.nbr, sizeof(uint32_t) * nb.size(), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int64_t k = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    for (int m = 0; m < cnt[i]; ++m) {
+      const uint32_t e = nb[(size_t)m * s.stride + i];
+      const int64_t j = e & kIndexMask;
+      const uint32_t img = e >> 26;
+      const int code = (int)(img & 3u) + 3 * (int)((img >> 2) & 3u) + 9 * (int)(img >> 4);
+      int64_t tj;
+      if (j >= n) {
+        tj = j;  // host-halo ghost: n + host index
+      } else {
+        tj = tg[j];
+        if (img == kNoImage && tj < tg[i]) continue;  // half list: owned j > i
+      }
+      if (k < cap) {
+        pairs[3 * k] = tg[i];
+        pairs[3 * k + 1] = tj;
+        pairs[3 * k + 2] = code;
+      }
+      ++k;
+    }
+  }
+  *npairs = k;
+  return CV2_OK;
+}
+
+int cv2_get_stats(cv2_ctx *ctx, cv2_stats *out) {
+  if (!ctx || !out) return CV2_ERR_BAD_ARG;
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(hipMemcpyAsync(ctx->h_flags, ctx->s.flags, sizeof(int) * F_COUNT, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *out = cv2_stats{};
+  out->natoms = ctx->n;
+  out->nghost = ctx->ng;
+  out->steps = ctx->steps;
+  out->rebuilds = ctx->h_flags[F_REBUILDS];
+  out->list_entries = (uint32_t)ctx->h_flags[F_ENTRIES];
+  out->max_neighbors = ctx->h_flags[F_MAXNBR];
+  out->capacity = ctx->s.cap;
+  for (int d = 0; d < 3; ++d) out->ncell[d] = ctx->b.nc[d];
+  out->precision = ctx->precision;
+  return CV2_OK;
+}
+
+int cv2_synchronize(cv2_ctx *ctx) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return CV2_OK;
+}
+
+int cv2_set_timing(cv2_ctx *ctx, int enable) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  spans_collect(ctx);
+  ctx->timing = enable != 0;
+  if (enable) {
+    ctx->t_ms[0] = ctx->t_ms[1] = ctx->t_ms[2] = 0.0;
+    ctx->force_calls = 0;
+  }
+  return CV2_OK;
+}
+
+int cv2_get_timing(cv2_ctx *ctx, cv2_timing *out) {
+  if (!ctx || !out) return CV2_ERR_BAD_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  spans_collect(ctx);
+  out->force_ms = ctx->t_ms[0];
+  out->force_calls = ctx->force_calls;
+  out->build_ms = ctx->t_ms[1];
+  out->integrate_ms = ctx->t_ms[2];
+  return CV2_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// cv2_internal.h -- shared device/host declarations of the MD-step library.
+//
+// HBM layout (all arrays in cell-sorted order of the owned atoms, rebuilt at
+// every neighbour-list build; host-halo ghosts follow the owned atoms in
+// host order):
+//   pos   double4[n_owned + n_ghost]  x, y, z, type (as double)     32 B/atom
+//   vel   double[3][n_owned]          SoA                            24 B/atom
+//   acc   double[3][n_owned]          SoA                            24 B/atom
+//   q     int4[n_owned + n_ghost]     fixed-point x, y, z, type      16 B/atom
+//                                     (FP32 path: periodic axes span 2^32)
+//   tag   int32[n_owned]              upload index (inverse of the sort)
+//   id    int32[n_owned + n_ghost]    user id (virial/energy pair rule, noise key)
+//   pos_ref double4[n_owned+n_ghost]  positions at the last build
+//   nbr   uint32[cap][stride]         transposed ELL Verlet list: entry
+//                                     j | (vx+1)<<26 | (vy+1)<<28 | (vz+1)<<30,
+//                                     the image of j being pos_j - v*L
+//   nnbr  int32[n_owned]              list length per atom
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cv2 {
+
+constexpr int kMaxTypes = 8;
+constexpr uint32_t kIndexMask = (1u << 26) - 1u;
+constexpr uint32_t kNoImage = 0x15u;  // (1,1,1) in 2-bit fields: no shift on any axis
+
+// Flags word layout in device memory.
+enum Flag : int {
+  F_REBUILD = 0,   // a rebuild is pending (set by the displacement check)
+  F_OVERFLOW = 1,  // some list exceeded its capacity
+  F_MAXNBR = 2,    // longest list of the last build
+  F_NAN = 3,       // NaN acceleration seen
+  F_FXOVF = 4,     // fixed-point overflow seen
+  F_REBUILDS = 5,  // number of builds done on the device
+  F_ENTRIES = 6,   // list entries of the last build (low 32 bits)
+  F_COUNT = 8
+};
+
+struct BoxDev {
+  double lo[3], hi[3], L[3], halfL[3];
+  double glo[3];   // cell-grid origin
+  double cinv[3];  // cells per unit length
+  double ghost_cut;
+  double ce_sq;     // (cutoff * extra_coef)^2, the list predicate
+  double thresh_sq; // squared half-skin displacement that triggers a rebuild
+  double qscale[3]; // fixed-point units per length (2^32 / L on periodic axes)
+  double qinv[3];   // length per fixed-point unit
+  int bc[3];        // 1 = periodic (images generated on the device)
+  int nc[3];
+};
+
+struct LJDev {
+  int nt;
+  double eps4[kMaxTypes * kMaxTypes];   // 4 * eps
+  double sig2[kMaxTypes * kMaxTypes];   // sigma^2
+  double rc2[kMaxTypes * kMaxTypes];    // cutoff^2 (per pair with cutoff_list)
+  double fshift[kMaxTypes * kMaxTypes]; // 12 rho_c^12 / rc^2 - 6 rho_c^6 / rc^2
+  double ecut[kMaxTypes * kMaxTypes];   // U(r) = 4e[(s6^2 - s6) + eA r^2 - eB]: eA
+  double ecut2[kMaxTypes * kMaxTypes];  //                                     eB
+  double mass_inv[kMaxTypes];
+  double mass[kMaxTypes];
+  // FP32 copies for the mixed-precision kernel
+  float eps4f[kMaxTypes * kMaxTypes], sig2f[kMaxTypes * kMaxTypes];
+  float rc2f[kMaxTypes * kMaxTypes], fshiftf[kMaxTypes * kMaxTypes];
+};
+
+// Fixed-point scales of the FP32 path: forces (and accelerations) in units of
+// 2^-32, energies and virials in units of 2^-24.
+constexpr double kForceScale = 4294967296.0;
+constexpr double kEnergyScale = 16777216.0;
+
+struct DevState {
+  int n;        // owned atoms
+  int ng;       // host-halo ghosts
+  int stride;   // ELL stride (>= n, multiple of 64)
+  int cap;      // ELL slots per atom
+  double4 *pos, *pos_alt, *pos_ref;
+  int4 *q;
+  double *vel, *vel_alt;  // 3 * n each (SoA: x block, y block, z block)
+  double *acc, *acc_alt;
+  int *tag, *tag_alt, *id, *id_alt;
+  int *cell, *slot, *perm;     // per owned atom during a build
+  int *cell_count, *cell_start; // ncell (+1)
+  int *gcell_count, *gcell_start, *gcell_items; // host-halo ghost cell list
+  uint32_t *nbr;
+  int *nnbr;
+  int *flags;
+  double *partials;            // per-block reduction partials
+  long long *fx_acc;           // fixed-point reductions (energy, virial)
+  double *reduce_out;          // final reductions
+  int ncell;
+};
+
+// ---- launchers (defined in the .hip files) ----
+void launch_pack_q(const DevState &s, const BoxDev &b, hipStream_t st);
+void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo);
+void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
+                   int precision, bool energy, hipStream_t st);
+void launch_kick_drift(DevState &s, const BoxDev &b, double dt, hipStream_t st);
+void launch_kick(DevState &s, double dt, hipStream_t st);
+void launch_langevin_1(DevState &s, const BoxDev &b, const LJDev *lj_dev, double dt, double lb,
+                       double lc, uint64_t seed, uint64_t step, hipStream_t st);
+void launch_langevin_2(DevState &s, const LJDev *lj_dev, double dt, double la, double lb,
+                       uint64_t seed, uint64_t step, hipStream_t st);
+void launch_thermo(DevState &s, const LJDev *lj_dev, hipStream_t st);
+void launch_set_flag(int *flags, int which, int value, hipStream_t st);
+
+}  // namespace cv2

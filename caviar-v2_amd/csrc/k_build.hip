@@ -1,0 +1,395 @@
+// k_build.hip -- periodic wrap, cell sort and Verlet-list build on the device.
+//
+// A build runs only when the device flag F_REBUILD is set (by the
+// displacement check in the integrator kernels, or by the host for a forced
+// build); every kernel below returns at once otherwise, so a whole MD step
+// can be queued without a host round trip.
+//
+// Reference behaviour restated here:
+//   wrap        Domain::fix_position (domain.cpp:443-493)
+//   cells       Neighborlist::update_cell_list / binlist_index
+//               (neighborlist.cpp:392-455) -- own grid, see below
+//   list        Neighborlist::update_verlet_list (neighborlist.cpp:247-297)
+//               with ghost images as exchange_ghost_single_md_domain.cpp:67-215
+//               makes them.
+//
+// Pair-set parity: the predicate is evaluated exactly as the reference does,
+// in FP64 with no contraction: the ghost coordinate is formed first,
+// g = p_j - v*L (one rounding), then dr = g - p_i and
+// r^2 = (dx*dx + dy*dy) + dz*dz < (cutoff*coef)^2.  Ghost images are not
+// stored: for each candidate the only image that can lie within the list
+// cutoff (the minimum image, boxes being wider than twice the cutoff) is
+// generated on the fly, and kept only if the reference would have created
+// that ghost (owner within ghost_cutoff of the face, same priority rule).
+// The cell grid is only a candidate filter (cells at least as wide as the
+// list cutoff), so the pair set does not depend on it.
+#include "cv2_internal.h"
+
+namespace cv2 {
+
+namespace {
+
+__device__ inline int axis_cell(double p, double glo, double cinv, int nc) {
+  int c = (int)floor((p - glo) * cinv);
+  return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__device__ inline int cell_of(const double4 &p, const BoxDev &b) {
+  const int cx = axis_cell(p.x, b.glo[0], b.cinv[0], b.nc[0]);
+  const int cy = axis_cell(p.y, b.glo[1], b.cinv[1], b.nc[1]);
+  const int cz = axis_cell(p.z, b.glo[2], b.cinv[2], b.nc[2]);
+  return cx + b.nc[0] * (cy + b.nc[1] * cz);
+}
+
+// Wrap, bin, count.  One thread per owned atom.
+__global__ void k_wrap_bin(double4 *__restrict__ pos, int n, BoxDev b, int *__restrict__ cell,
+                           int *__restrict__ slot, int *__restrict__ cell_count,
+                           const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 p = pos[i];
+  double *c[3] = {&p.x, &p.y, &p.z};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (b.bc[d] != 1) continue;
+    if (*c[d] < b.lo[d])
+      *c[d] += b.L[d];
+    else if (*c[d] > b.hi[d])
+      *c[d] -= b.L[d];
+  }
+  pos[i] = p;
+  const int ci = cell_of(p, b);
+  cell[i] = ci;
+  slot[i] = atomicAdd(&cell_count[ci], 1);
+}
+
+// Exclusive scan of cell counts (one work-group; ncell up to a few million).
+__global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ count,
+                                               int *__restrict__ start, int ncell,
+                                               const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (ncell + 1023) / 1024;
+  const int b0 = t * chunk;
+  const int b1 = min(ncell, b0 + chunk);
+  int s = 0;
+  for (int c = b0; c < b1; ++c) s += count[c];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int c = b0; c < b1; ++c) {
+    start[c] = run;
+    run += count[c];
+  }
+  if (t == 1023) start[ncell] = part[1023];
+}
+
+__global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
+                          const int *__restrict__ start, int *__restrict__ perm, int n,
+                          const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  perm[start[cell[i]] + slot[i]] = i;
+}
+
+// Order atoms inside each cell by upload index so that the sorted order,
+// and with it every list and sum, is independent of atomic arrival order.
+__global__ void k_sort_cells(int *__restrict__ perm, const int *__restrict__ start,
+                             const int *__restrict__ tag, int ncell,
+                             const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int a = start[c], e = start[c + 1];
+  for (int k = a + 1; k < e; ++k) {
+    const int v = perm[k];
+    const int tv = tag[v];
+    int m = k - 1;
+    while (m >= a && tag[perm[m]] > tv) {
+      perm[m + 1] = perm[m];
+      --m;
+    }
+    perm[m + 1] = v;
+  }
+}
+
+// Gather every per-atom array into the new order; remember build positions.
+__global__ void k_permute(const int *__restrict__ perm, int n, const double4 *__restrict__ pos,
+                          double4 *__restrict__ pos_out, double4 *__restrict__ pos_ref,
+                          const double *__restrict__ vel, double *__restrict__ vel_out,
+                          const double *__restrict__ acc, double *__restrict__ acc_out,
+                          const int *__restrict__ tag, int *__restrict__ tag_out,
+                          const int *__restrict__ id, int *__restrict__ id_out,
+                          const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int o = perm[k];
+  const double4 p = pos[o];
+  pos_out[k] = p;
+  pos_ref[k] = p;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vel_out[d * n + k] = vel[d * n + o];
+    acc_out[d * n + k] = acc[d * n + o];
+  }
+  tag_out[k] = tag[o];
+  id_out[k] = id[o];
+}
+
+__global__ void k_copy_back(int n, const double4 *__restrict__ pos_alt, double4 *__restrict__ pos,
+                            const double *__restrict__ vel_alt, double *__restrict__ vel,
+                            const double *__restrict__ acc_alt, double *__restrict__ acc,
+                            const int *__restrict__ tag_alt, int *__restrict__ tag,
+                            const int *__restrict__ id_alt, int *__restrict__ id,
+                            const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  pos[k] = pos_alt[k];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vel[d * n + k] = vel_alt[d * n + k];
+    acc[d * n + k] = acc_alt[d * n + k];
+  }
+  tag[k] = tag_alt[k];
+  id[k] = id_alt[k];
+}
+
+// Host-halo ghosts keep host order; they get their own CSR cell list and
+// their build positions.
+__global__ void k_ghost_bin(const double4 *__restrict__ pos, double4 *__restrict__ pos_ref,
+                            int n, int ng, BoxDev b, int *__restrict__ cell,
+                            int *__restrict__ slot, int *__restrict__ cell_count,
+                            const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const double4 p = pos[n + g];
+  pos_ref[n + g] = p;
+  const int c = cell_of(p, b);
+  cell[g] = c;
+  slot[g] = atomicAdd(&cell_count[c], 1);
+}
+
+__global__ void k_ghost_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
+                                const int *__restrict__ start, int *__restrict__ items, int ng,
+                                const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  items[start[cell[g]] + slot[g]] = g;
+}
+
+__global__ void k_ghost_sort_cells(int *__restrict__ items, const int *__restrict__ start,
+                                   int ncell, const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int a = start[c], e = start[c + 1];
+  for (int k = a + 1; k < e; ++k) {
+    const int v = items[k];
+    int m = k - 1;
+    while (m >= a && items[m] > v) {
+      items[m + 1] = items[m];
+      --m;
+    }
+    items[m + 1] = v;
+  }
+}
+
+// Candidate cells along one axis: c-1, c, c+1 (periodic wrap, or clamped
+// for open axes), duplicates removed when the grid has fewer than 3 cells.
+__device__ inline int axis_stencil(int c, int nc, int periodic, int out[3]) {
+  if (nc <= 3 && periodic) {
+    for (int k = 0; k < nc; ++k) out[k] = k;
+    return nc;
+  }
+  int m = 0;
+  for (int d = -1; d <= 1; ++d) {
+    int v = c + d;
+    if (periodic) {
+      v = v < 0 ? v + nc : (v >= nc ? v - nc : v);
+    } else if (v < 0 || v >= nc) {
+      continue;
+    }
+    out[m++] = v;
+  }
+  return m;
+}
+
+// Image of candidate coordinate pj relative to pi on a periodic axis, and
+// whether the reference would have created that ghost
+// (exchange_ghost_single_md_domain.cpp:69-100).
+__device__ inline bool axis_image(double pi, double pj, double lo, double hi, double L,
+                                  double halfL, double g, int periodic, int &v, double &out) {
+  v = 0;
+  out = pj;
+  if (!periodic) return true;
+  const double d = pj - pi;
+  if (d > halfL)
+    v = 1;
+  else if (d < -halfL)
+    v = -1;
+  if (v == 0) return true;
+  const int flag = (pj < lo + g) ? -1 : ((pj > hi - g) ? 1 : 0);
+  if (flag != v) return false;
+  out = __dsub_rn(pj, (double)v * L);
+  return true;
+}
+
+// Verlet list: one thread per owned atom, full list (both directions).
+__global__ void __launch_bounds__(256)
+    k_nlist(const double4 *__restrict__ pos, int n, BoxDev b, const int *__restrict__ cstart,
+            const int *__restrict__ gstart, const int *__restrict__ gitems,
+            uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
+            int *__restrict__ flags, int host_halo) {
+  if (flags[F_REBUILD] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pi = pos[i];
+  const int cx = axis_cell(pi.x, b.glo[0], b.cinv[0], b.nc[0]);
+  const int cy = axis_cell(pi.y, b.glo[1], b.cinv[1], b.nc[1]);
+  const int cz = axis_cell(pi.z, b.glo[2], b.cinv[2], b.nc[2]);
+  int sx[3], sy[3], sz[3];
+  const int nx = axis_stencil(cx, b.nc[0], b.bc[0], sx);
+  const int ny = axis_stencil(cy, b.nc[1], b.bc[1], sy);
+  const int nz = axis_stencil(cz, b.nc[2], b.bc[2], sz);
+  int cnt = 0;
+  for (int a = 0; a < nz; ++a)
+    for (int bb = 0; bb < ny; ++bb)
+      for (int c = 0; c < nx; ++c) {
+        const int cell = sx[c] + b.nc[0] * (sy[bb] + b.nc[1] * sz[a]);
+        const int j1 = cstart[cell + 1];
+        for (int j = cstart[cell]; j < j1; ++j) {
+          if (j == i) continue;
+          const double4 pj = pos[j];
+          int vx, vy, vz;
+          double gx, gy, gz;
+          if (!axis_image(pi.x, pj.x, b.lo[0], b.hi[0], b.L[0], b.halfL[0], b.ghost_cut, b.bc[0], vx, gx)) continue;
+          if (!axis_image(pi.y, pj.y, b.lo[1], b.hi[1], b.L[1], b.halfL[1], b.ghost_cut, b.bc[1], vy, gy)) continue;
+          if (!axis_image(pi.z, pj.z, b.lo[2], b.hi[2], b.L[2], b.halfL[2], b.ghost_cut, b.bc[2], vz, gz)) continue;
+          const double dx = __dsub_rn(gx, pi.x), dy = __dsub_rn(gy, pi.y), dz = __dsub_rn(gz, pi.z);
+          const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+          if (r2 < b.ce_sq) {
+            if (cnt < cap)
+              nbr[(size_t)cnt * stride + i] =
+                  (uint32_t)j | ((uint32_t)(vx + 1) << 26) | ((uint32_t)(vy + 1) << 28) | ((uint32_t)(vz + 1) << 30);
+            ++cnt;
+          }
+        }
+        if (host_halo) {
+          const int g1 = gstart[cell + 1];
+          for (int t = gstart[cell]; t < g1; ++t) {
+            const int j = n + gitems[t];
+            const double4 pj = pos[j];
+            const double dx = __dsub_rn(pj.x, pi.x), dy = __dsub_rn(pj.y, pi.y), dz = __dsub_rn(pj.z, pi.z);
+            const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+            if (r2 < b.ce_sq) {
+              if (cnt < cap) nbr[(size_t)cnt * stride + i] = (uint32_t)j | (kNoImage << 26);
+              ++cnt;
+            }
+          }
+        }
+      }
+  nnbr[i] = cnt < cap ? cnt : cap;
+  if (cnt > cap) atomicOr(&flags[F_OVERFLOW], 1);
+  atomicMax(&flags[F_MAXNBR], cnt);
+  atomicAdd(&flags[F_ENTRIES], cnt < cap ? cnt : cap);
+}
+
+__global__ void k_build_done(int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  flags[F_REBUILD] = 0;
+  flags[F_REBUILDS] += 1;
+}
+
+__global__ void k_build_begin(int *__restrict__ flags, int *__restrict__ cell_count, int ncell,
+                              int *__restrict__ gcount) {
+  if (flags[F_REBUILD] == 0) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncell) {
+    cell_count[c] = 0;
+    if (gcount) gcount[c] = 0;
+  }
+  if (c == 0) {
+    flags[F_MAXNBR] = 0;
+    flags[F_ENTRIES] = 0;
+  }
+}
+
+// Fixed-point copy of positions for the FP32 pair kernel: (x - lo) * 2^32 / L
+// reduced modulo 2^32, so that int32 differences are minimum-image
+// separations on periodic axes.
+__global__ void k_pack_q(const double4 *__restrict__ pos, int4 *__restrict__ q, int ntot, BoxDev b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntot) return;
+  const double4 p = pos[i];
+  int4 r;
+  r.x = (int)(unsigned)(unsigned long long)llrint((p.x - b.lo[0]) * b.qscale[0]);
+  r.y = (int)(unsigned)(unsigned long long)llrint((p.y - b.lo[1]) * b.qscale[1]);
+  r.z = (int)(unsigned)(unsigned long long)llrint((p.z - b.lo[2]) * b.qscale[2]);
+  r.w = (int)p.w;
+  q[i] = r;
+}
+
+__global__ void k_set_flag(int *flags, int which, int value) { flags[which] = value; }
+
+inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+void launch_set_flag(int *flags, int which, int value, hipStream_t st) {
+  k_set_flag<<<1, 1, 0, st>>>(flags, which, value);
+}
+
+void launch_pack_q(const DevState &s, const BoxDev &b, hipStream_t st) {
+  const int ntot = s.n + s.ng;
+  if (ntot == 0) return;
+  k_pack_q<<<nblk(ntot, 256), 256, 0, st>>>(s.pos, s.q, ntot, b);
+}
+
+void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) {
+  const int n = s.n;
+  const int nc = s.ncell;
+  const bool ghosts = host_halo && s.ng > 0;
+  k_build_begin<<<nblk(nc, 256), 256, 0, st>>>(s.flags, s.cell_count, nc,
+                                                ghosts ? s.gcell_count : nullptr);
+  if (n > 0) {
+    k_wrap_bin<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell, s.slot, s.cell_count, s.flags);
+    k_scan<<<1, 1024, 0, st>>>(s.cell_count, s.cell_start, nc, s.flags);
+    k_scatter<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.slot, s.cell_start, s.perm, n, s.flags);
+    k_sort_cells<<<nblk(nc, 256), 256, 0, st>>>(s.perm, s.cell_start, s.tag, nc, s.flags);
+    k_permute<<<nblk(n, 256), 256, 0, st>>>(s.perm, n, s.pos, s.pos_alt, s.pos_ref, s.vel,
+                                            s.vel_alt, s.acc, s.acc_alt, s.tag, s.tag_alt, s.id,
+                                            s.id_alt, s.flags);
+    // The host cannot know whether the device ran the build, so the new
+    // order is copied back instead of swapping buffer pointers.
+    k_copy_back<<<nblk(n, 256), 256, 0, st>>>(n, s.pos_alt, s.pos, s.vel_alt, s.vel, s.acc_alt,
+                                              s.acc, s.tag_alt, s.tag, s.id_alt, s.id, s.flags);
+  }
+  if (ghosts) {
+    k_ghost_bin<<<nblk(s.ng, 256), 256, 0, st>>>(s.pos, s.pos_ref, n, s.ng, b, s.cell + n,
+                                                  s.slot + n, s.gcell_count, s.flags);
+    k_scan<<<1, 1024, 0, st>>>(s.gcell_count, s.gcell_start, nc, s.flags);
+    k_ghost_scatter<<<nblk(s.ng, 256), 256, 0, st>>>(s.cell + n, s.slot + n, s.gcell_start,
+                                                      s.gcell_items, s.ng, s.flags);
+    k_ghost_sort_cells<<<nblk(nc, 256), 256, 0, st>>>(s.gcell_items, s.gcell_start, nc, s.flags);
+  }
+  if (n > 0)
+    k_nlist<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell_start, s.gcell_start,
+                                          s.gcell_items, s.nbr, s.nnbr, s.stride, s.cap, s.flags,
+                                          ghosts ? 1 : 0);
+  k_build_done<<<1, 1, 0, st>>>(s.flags);
+}
+
+}  // namespace cv2

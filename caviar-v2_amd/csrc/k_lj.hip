@@ -1,0 +1,272 @@
+// k_lj.hip -- Lennard-Jones accelerations, virial and energy on the device.
+//
+// Force law of force_field::Lj::calculate_acceleration (lj.cpp:393-517):
+//   forceCoef = 4 eps (-12 rho^12/r^2 + 6 rho^6/r^2 + 12 rho_c^12/rc^2 - 6 rho_c^6/rc^2),
+//   rho^2 = sigma^2/r^2, rho_c^2 = sigma^2/rc^2, pairs with r^2 > rc^2 skipped,
+//   F_i = forceCoef * (r_j - r_i), a_i += F_i / m_i, W += -forceCoef r^2 once per pair.
+// The potential energy is the U(r) whose U'(r)/r is forceCoef, with U(rc)=0
+// (the reference computes no energy):
+//   U = 4 eps [(rho^12 - rho^6) + (6 rho_c^12 - 3 rho_c^6) r^2/rc^2 - (7 rho_c^12 - 4 rho_c^6)].
+//
+// Both kernels walk the full Verlet list (each pair from both sides), one
+// thread per owned atom, so every atom's sum is private to one thread: no
+// atomics on forces.  Virial and energy count a pair once, from the side
+// with the lower sorted index (host-halo ghosts: the lower id, as
+// lj.cpp:487 does).
+//
+// k_lj_f64     FP64 pair arithmetic, FP64 accumulation.
+// k_lj_f32fx   FP32 pair arithmetic on separations formed exactly from
+//              fixed-point coordinates; per-pair force components are
+//              converted to 64-bit fixed point (2^-32 units) inside one FP64
+//              FMA and summed as integers, so the per-atom sum is exact and
+//              independent of summation order.  Energy and virial use 2^-24
+//              units and integer atomics for the global sums.
+#include "cv2_internal.h"
+
+namespace cv2 {
+
+namespace {
+
+constexpr int kBlock = 256;
+// 1.5 * 2^52: adding it to a double of magnitude < 2^51 rounds to an integer
+// held in the low mantissa bits, so bits(t) - bits(kMagic) is that integer.
+constexpr double kMagic = 6755399441055744.0;
+constexpr long long kMagicBits = 0x4338000000000000LL;
+
+// Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD gets a
+// contiguous range of (spatially sorted) atoms and shares its L2 working set.
+__device__ inline int xcd_block(int nblk_real) {
+  const int nb = gridDim.x;  // multiple of 8
+  const int per = nb >> 3;
+  const int b = blockIdx.x;
+  const int logical = (b & 7) * per + (b >> 3);
+  return logical < nblk_real ? logical : -1;
+}
+
+__device__ inline double block_sum(double v, double *sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r += sh[k];
+  return r;
+}
+
+__device__ inline long long block_sum_i64(long long v, long long *sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  long long r = 0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r += sh[k];
+  return r;
+}
+
+template <bool MULTI, bool ENERGY, bool HALO>
+__global__ void __launch_bounds__(kBlock)
+    k_lj_f64(const double4 *__restrict__ pos, const uint32_t *__restrict__ nbr,
+             const int *__restrict__ nnbr, int stride, int n, const int *__restrict__ id,
+             const LJDev *__restrict__ lj, BoxDev b, double *__restrict__ acc,
+             double *__restrict__ partials, int *__restrict__ flags, int nblk_real) {
+  __shared__ double sh[kBlock / 64];
+  const int blk = xcd_block(nblk_real);
+  if (blk < 0) return;
+  const int i = blk * kBlock + threadIdx.x;
+  double vir = 0.0, pe = 0.0;
+  if (i < n) {
+    const double4 pi = pos[i];
+    const int ti = MULTI ? (int)pi.w : 0;
+    const int nt = MULTI ? lj->nt : 1;
+    const int cnt = nnbr[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int id_i = (ENERGY && HALO) ? id[i] : 0;
+    for (int k = 0; k < cnt; ++k) {
+      const uint32_t e = nbr[(size_t)k * stride + i];
+      const int j = (int)(e & kIndexMask);
+      const double4 pj = pos[j];
+      double gx = pj.x, gy = pj.y, gz = pj.z;
+      const uint32_t img = e >> 26;
+      if (img != kNoImage) {
+        gx -= (double)((int)(img & 3u) - 1) * b.L[0];
+        gy -= (double)((int)((img >> 2) & 3u) - 1) * b.L[1];
+        gz -= (double)((int)(img >> 4) - 1) * b.L[2];
+      }
+      const double dx = gx - pi.x, dy = gy - pi.y, dz = gz - pi.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const int tp = MULTI ? ti * nt + (int)pj.w : 0;
+      if (r2 > lj->rc2[tp]) continue;
+      const double r2i = 1.0 / r2;
+      const double s2 = lj->sig2[tp] * r2i;
+      const double s6 = s2 * s2 * s2;
+      const double eps4 = lj->eps4[tp];
+      const double fc = eps4 * (r2i * (s6 * (-12.0 * s6 + 6.0)) + lj->fshift[tp]);
+      fx += fc * dx;
+      fy += fc * dy;
+      fz += fc * dz;
+      if (ENERGY) {
+        const bool once = HALO && j >= n ? (id_i < id[j]) : (i < j);
+        if (once) {
+          vir -= fc * r2;
+          pe += eps4 * ((s6 * s6 - s6) + lj->ecut[tp] * r2 - lj->ecut2[tp]);
+        }
+      }
+    }
+    const double mi = lj->mass_inv[ti];
+    const double ax = fx * mi, ay = fy * mi, az = fz * mi;
+    acc[i] = ax;
+    acc[n + i] = ay;
+    acc[2 * n + i] = az;
+    if (isnan(ax) || isnan(ay) || isnan(az)) atomicOr(&flags[F_NAN], 1);
+  }
+  if (ENERGY) {
+    const double v = block_sum(vir, sh);
+    __syncthreads();
+    const double u = block_sum(pe, sh);
+    if (threadIdx.x == 0) {
+      partials[2 * blk] = v;
+      partials[2 * blk + 1] = u;
+    }
+  }
+}
+
+template <bool MULTI, bool ENERGY, bool HALO>
+__global__ void __launch_bounds__(kBlock)
+    k_lj_f32fx(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
+               const int *__restrict__ nnbr, int stride, int n, const int *__restrict__ id,
+               const LJDev *__restrict__ lj, BoxDev b, double *__restrict__ acc,
+               long long *__restrict__ fx_acc, int *__restrict__ flags, int nblk_real) {
+  __shared__ long long sh[kBlock / 64];
+  const int blk = xcd_block(nblk_real);
+  if (blk < 0) return;
+  const int i = blk * kBlock + threadIdx.x;
+  // separation in units of 2^-32 length, then r^2 scaled back by 2^-64
+  const double qs0 = b.qinv[0] * kForceScale, qs1 = b.qinv[1] * kForceScale,
+               qs2 = b.qinv[2] * kForceScale;
+  long long vir = 0, pe = 0;
+  if (i < n) {
+    const int4 qi = q[i];
+    const int ti = MULTI ? qi.w : 0;
+    const int nt = MULTI ? lj->nt : 1;
+    const int cnt = nnbr[i];
+    unsigned long long sx = 0, sy = 0, sz = 0;
+    int terms = 0;
+    bool ovf = false;
+    const int id_i = (ENERGY && HALO) ? id[i] : 0;
+    for (int k = 0; k < cnt; ++k) {
+      const uint32_t e = nbr[(size_t)k * stride + i];
+      const int j = (int)(e & kIndexMask);
+      const int4 qj = q[j];
+      const double dxs = (double)(qj.x - qi.x) * qs0;
+      const double dys = (double)(qj.y - qi.y) * qs1;
+      const double dzs = (double)(qj.z - qi.z) * qs2;
+      const float r2 = (float)((dxs * dxs + dys * dys + dzs * dzs) * 5.421010862427522e-20);
+      const int tp = MULTI ? ti * nt + qj.w : 0;
+      if (r2 > lj->rc2f[tp]) continue;
+      const float sig2 = lj->sig2f[tp];
+      ovf |= r2 < 0.25f * sig2;
+      const float r2i = __builtin_amdgcn_rcpf(r2);
+      const float s2 = sig2 * r2i;
+      const float s6 = s2 * s2 * s2;
+      const float eps4 = lj->eps4f[tp];
+      const float fc = eps4 * (r2i * (s6 * (-12.0f * s6 + 6.0f)) + lj->fshiftf[tp]);
+      const double fcd = (double)fc;
+      sx += (unsigned long long)__double_as_longlong(fma(fcd, dxs, kMagic));
+      sy += (unsigned long long)__double_as_longlong(fma(fcd, dys, kMagic));
+      sz += (unsigned long long)__double_as_longlong(fma(fcd, dzs, kMagic));
+      ++terms;
+      if (ENERGY) {
+        const bool once = HALO && j >= n ? (id_i < id[j]) : (i < j);
+        if (once) {
+          const double r2d = (double)r2;
+          vir += __double_as_longlong(fma(-fcd * r2d, kEnergyScale, kMagic)) - kMagicBits;
+          const float u = eps4 * ((s6 * s6 - s6) + (float)lj->ecut[tp] * r2 - (float)lj->ecut2[tp]);
+          pe += __double_as_longlong(fma((double)u, kEnergyScale, kMagic)) - kMagicBits;
+        }
+      }
+    }
+    const unsigned long long base = (unsigned long long)terms * (unsigned long long)kMagicBits;
+    const double mi = lj->mass_inv[ti] * (1.0 / kForceScale);
+    acc[i] = (double)(long long)(sx - base) * mi;
+    acc[n + i] = (double)(long long)(sy - base) * mi;
+    acc[2 * n + i] = (double)(long long)(sz - base) * mi;
+    if (ovf) atomicOr(&flags[F_FXOVF], 1);
+  }
+  if (ENERGY) {
+    const long long v = block_sum_i64(vir, sh);
+    __syncthreads();
+    const long long u = block_sum_i64(pe, sh);
+    if (threadIdx.x == 0) {
+      atomicAdd((unsigned long long *)&fx_acc[0], (unsigned long long)v);
+      atomicAdd((unsigned long long *)&fx_acc[1], (unsigned long long)u);
+    }
+  }
+}
+
+// Fixed-order sum of per-block partials (deterministic).
+__global__ void __launch_bounds__(1024) k_reduce_partials(const double *__restrict__ partials,
+                                                         int nblk, int width,
+                                                         double *__restrict__ out) {
+  __shared__ double sh[1024];
+  for (int c = 0; c < width; ++c) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += 1024) s += partials[(size_t)k * width + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_i64(long long *p, int k) {
+  if ((int)threadIdx.x < k) p[threadIdx.x] = 0;
+}
+
+template <bool MULTI, bool ENERGY, bool HALO>
+void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precision, hipStream_t st) {
+  const int real = (s.n + kBlock - 1) / kBlock;
+  const int grid = ((real + 7) / 8) * 8;
+  if (precision == 0) {
+    k_lj_f64<MULTI, ENERGY, HALO><<<grid, kBlock, 0, st>>>(s.pos, s.nbr, s.nnbr, s.stride, s.n,
+                                                           s.id, lj_dev, b, s.acc, s.partials,
+                                                           s.flags, real);
+    if (ENERGY) k_reduce_partials<<<1, 1024, 0, st>>>(s.partials, real, 2, s.reduce_out);
+  } else {
+    if (ENERGY) k_zero_i64<<<1, 64, 0, st>>>(s.fx_acc, 2);
+    k_lj_f32fx<MULTI, ENERGY, HALO><<<grid, kBlock, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n,
+                                                             s.id, lj_dev, b, s.acc, s.fx_acc,
+                                                             s.flags, real);
+  }
+}
+
+}  // namespace
+
+void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
+                   int precision, bool energy, hipStream_t st) {
+  if (s.n == 0) return;
+  const bool multi = lj.nt > 1;
+  const bool halo = s.ng > 0;
+  if (precision != 0) launch_pack_q(s, b, st);
+#define CV2_LJ(M, E, H)                                      \
+  if (multi == M && energy == E && halo == H) {              \
+    lj_dispatch<M, E, H>(s, b, lj_dev, precision, st);       \
+    return;                                                  \
+  }
+  CV2_LJ(false, false, false)
+  CV2_LJ(false, true, false)
+  CV2_LJ(true, false, false)
+  CV2_LJ(true, true, false)
+  CV2_LJ(false, false, true)
+  CV2_LJ(false, true, true)
+  CV2_LJ(true, false, true)
+  CV2_LJ(true, true, true)
+#undef CV2_LJ
+}
+
+}  // namespace cv2

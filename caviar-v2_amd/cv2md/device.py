@@ -1,0 +1,217 @@
+"""ctypes binding of the C ABI in include/cv2_md.h (lib/libcv2md.so).
+
+There is no CPU fallback: if the HIP library is missing or no GPU is
+visible, constructing a Context raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libcv2md.so")
+
+CV2_PREC_FP64 = 0
+CV2_PREC_FP32_FIXED = 1
+CV2_NL_REBUILD_EVERY_STEP = 1
+
+ERRORS = {1: "HIP", 2: "BAD_ARG", 3: "NAN", 4: "FIXED_OVERFLOW", 5: "CAPACITY", 6: "STATE"}
+
+_d = C.POINTER(C.c_double)
+_i32 = C.POINTER(C.c_int32)
+_i64 = C.POINTER(C.c_int64)
+
+
+class Stats(C.Structure):
+    _fields_ = [("natoms", C.c_int64), ("nghost", C.c_int64), ("steps", C.c_int64),
+                ("rebuilds", C.c_int64), ("list_entries", C.c_int64), ("max_neighbors", C.c_int32),
+                ("capacity", C.c_int32), ("ncell", C.c_int32 * 3), ("precision", C.c_int32)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("force_ms", C.c_double), ("force_calls", C.c_int64), ("build_ms", C.c_double),
+                ("integrate_ms", C.c_double)]
+
+
+class Cv2Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cv2 error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libcv2md.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} not built (run __graft_entry__.build() or make -C caviar-v2_amd)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        sig = {
+            "cv2_create": ([C.POINTER(vp), C.c_int], C.c_int),
+            "cv2_destroy": ([vp], C.c_int),
+            "cv2_last_error": ([vp], C.c_char_p),
+            "cv2_set_box": ([vp, _d, _d, _i32, C.c_double], C.c_int),
+            "cv2_set_lj": ([vp, C.c_int, _d, _d, _d, C.c_double, _d], C.c_int),
+            "cv2_set_neighbor": ([vp, C.c_double, C.c_double, C.c_uint], C.c_int),
+            "cv2_set_precision": ([vp, C.c_int], C.c_int),
+            "cv2_upload_atoms": ([vp, C.c_int64, _d, _d, _i32, _i64], C.c_int),
+            "cv2_set_ghosts": ([vp, C.c_int64, _d, _i32, _i64], C.c_int),
+            "cv2_build_neighbors": ([vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
+            "cv2_compute_forces": ([vp], C.c_int),
+            "cv2_run_vv": ([vp, C.c_int64, C.c_double], C.c_int),
+            "cv2_run_langevin": ([vp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint64], C.c_int),
+            "cv2_thermo": ([vp, C.c_int, _d, _d, _d], C.c_int),
+            "cv2_download": ([vp, _d, _d, _d], C.c_int),
+            "cv2_get_pairs": ([vp, C.c_int64, _i64, _i64], C.c_int),
+            "cv2_get_stats": ([vp, C.POINTER(Stats)], C.c_int),
+            "cv2_synchronize": ([vp], C.c_int),
+            "cv2_set_timing": ([vp, C.c_int], C.c_int),
+            "cv2_get_timing": ([vp, C.POINTER(Timing)], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device-resident MD system (see include/cv2_md.h)."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.cv2_create(C.byref(h), device)
+        if rc:
+            raise Cv2Error(rc, "cv2_create failed (no GPU?)")
+        self.h = h
+        self.n = 0
+
+    def _chk(self, rc):
+        if rc:
+            raise Cv2Error(rc, self.L.cv2_last_error(self.h).decode())
+
+    # ---- configuration
+    def set_box(self, lo, hi, bc=(1, 1, 1), ghost_cutoff=0.0):
+        lo = _f64(lo); hi = _f64(hi); b = np.ascontiguousarray(bc, np.int32)
+        self._chk(self.L.cv2_set_box(self.h, _p(lo, _d), _p(hi, _d), _p(b, _i32), float(ghost_cutoff)))
+
+    def set_lj(self, eps, sigma, cutoff, mass, cutoff_list=None):
+        e = _f64(np.atleast_2d(eps)); s = _f64(np.atleast_2d(sigma))
+        cl = _f64(np.atleast_2d(cutoff_list)) if cutoff_list is not None else None
+        m = _f64(np.atleast_1d(mass))
+        self._chk(self.L.cv2_set_lj(self.h, e.shape[0], _p(e, _d), _p(s, _d), _p(cl, _d), float(cutoff), _p(m, _d)))
+
+    def set_neighbor(self, cutoff, extra_coef, flags=0):
+        self._chk(self.L.cv2_set_neighbor(self.h, float(cutoff), float(extra_coef), int(flags)))
+
+    def set_precision(self, prec):
+        self._chk(self.L.cv2_set_precision(self.h, int(prec)))
+
+    def upload(self, pos, vel=None, type_=None, id_=None):
+        p = _f64(pos); v = _f64(vel)
+        t = np.ascontiguousarray(type_, np.int32) if type_ is not None else None
+        i = np.ascontiguousarray(id_, np.int64) if id_ is not None else None
+        self.n = len(p)
+        self._chk(self.L.cv2_upload_atoms(self.h, self.n, _p(p, _d), _p(v, _d), _p(t, _i32), _p(i, _i64)))
+
+    def set_ghosts(self, pos, type_=None, id_=None):
+        p = _f64(pos)
+        t = np.ascontiguousarray(type_, np.int32) if type_ is not None else None
+        i = np.ascontiguousarray(id_, np.int64) if id_ is not None else None
+        self._chk(self.L.cv2_set_ghosts(self.h, len(p), _p(p, _d), _p(t, _i32), _p(i, _i64)))
+
+    @classmethod
+    def from_sysdef(cls, s: dict, precision=CV2_PREC_FP64, device=0, rebuild_every_step=False):
+        c = cls(device)
+        c.set_box(s["lo"], s["hi"], s.get("bc", (1, 1, 1)), s["ghost_cutoff"])
+        c.set_lj(s["eps"], s["sigma"], s["lj_cutoff"], s["mass"], s.get("cutoff_list"))
+        c.set_neighbor(s["nl_cutoff"], s["nl_coef"], CV2_NL_REBUILD_EVERY_STEP if rebuild_every_step else 0)
+        c.set_precision(precision)
+        c.upload(s["pos"], s.get("vel"), s.get("type"))
+        return c
+
+    # ---- work
+    def build(self, force=True):
+        r = C.c_int(0)
+        self._chk(self.L.cv2_build_neighbors(self.h, int(force), C.byref(r)))
+        return bool(r.value)
+
+    def compute_forces(self):
+        self._chk(self.L.cv2_compute_forces(self.h))
+
+    def run_vv(self, nsteps, dt):
+        self._chk(self.L.cv2_run_vv(self.h, int(nsteps), float(dt)))
+
+    def run_langevin(self, nsteps, dt, gamma, kbt, seed):
+        self._chk(self.L.cv2_run_langevin(self.h, int(nsteps), float(dt), float(gamma), float(kbt), int(seed)))
+
+    def synchronize(self):
+        self._chk(self.L.cv2_synchronize(self.h))
+
+    # ---- results
+    def thermo(self, n_r_df=3):
+        ke = C.c_double(); pe = C.c_double(); vir = C.c_double()
+        self._chk(self.L.cv2_thermo(self.h, int(n_r_df), C.byref(ke), C.byref(pe), C.byref(vir)))
+        return dict(ke=ke.value, pe=pe.value, virial=vir.value)
+
+    def download(self):
+        p = np.empty((self.n, 3)); v = np.empty((self.n, 3)); a = np.empty((self.n, 3))
+        self._chk(self.L.cv2_download(self.h, _p(p, _d), _p(v, _d), _p(a, _d)))
+        return p, v, a
+
+    def pairs(self):
+        """(i, j, image_code) rows in the reference's half-list convention."""
+        k = C.c_int64()
+        self._chk(self.L.cv2_get_pairs(self.h, 0, C.byref(k), None))
+        out = np.empty((max(k.value, 1), 3), np.int64)
+        self._chk(self.L.cv2_get_pairs(self.h, k.value, C.byref(k), _p(out, _i64)))
+        return out[: k.value]
+
+    def stats(self):
+        s = Stats()
+        self._chk(self.L.cv2_get_stats(self.h, C.byref(s)))
+        return dict(natoms=s.natoms, nghost=s.nghost, steps=s.steps, rebuilds=s.rebuilds,
+                    list_entries=s.list_entries, max_neighbors=s.max_neighbors, capacity=s.capacity,
+                    ncell=tuple(s.ncell), precision=s.precision)
+
+    def set_timing(self, enable=True):
+        self._chk(self.L.cv2_set_timing(self.h, int(bool(enable))))
+
+    def timing(self):
+        t = Timing()
+        self._chk(self.L.cv2_get_timing(self.h, C.byref(t)))
+        return dict(force_ms=t.force_ms, force_calls=t.force_calls, build_ms=t.build_ms,
+                    integrate_ms=t.integrate_ms)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cv2_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

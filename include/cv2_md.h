@@ -1,0 +1,158 @@
+/*
+ * cv2_md.h -- C ABI of the MI355X-native CAVIAR2 molecular-dynamics step.
+ *
+ * One opaque context holds one periodic box (or one domain's share plus a
+ * host-supplied ghost halo) resident in HBM: cell-sorted atom state,
+ * cell list, Verlet list and thermodynamic accumulators.  Every call returns
+ * CV2_OK (0) or an error code; cv2_last_error() gives the message.  Host
+ * arrays passed in are copied, never retained.  Calls on one context are
+ * ordered on its own HIP stream and are not thread-safe; separate contexts
+ * are independent.
+ *
+ * The reference exposes this path only as C++ classes (no C ABI); each entry
+ * point names the reference interface it stands in for.
+ */
+#ifndef CV2_MD_H
+#define CV2_MD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define CV2_OK 0
+#define CV2_ERR_HIP 1            /* HIP runtime failure */
+#define CV2_ERR_BAD_ARG 2        /* invalid argument (reference: std::runtime_error / error_all) */
+#define CV2_ERR_NAN 3            /* NaN acceleration (md_simulator.cpp:646-652) */
+#define CV2_ERR_FIXED_OVERFLOW 4 /* fixed-point force/energy out of range */
+#define CV2_ERR_CAPACITY 5       /* neighbour-list capacity exceeded */
+#define CV2_ERR_STATE 6          /* call out of order (e.g. run before upload) */
+
+/* force precision forms */
+#define CV2_PREC_FP64 0          /* FP64 pair arithmetic, FP64 accumulation */
+#define CV2_PREC_FP32_FIXED 1    /* FP32 pair arithmetic, 64-bit fixed-point accumulation */
+
+/* neighbour-list flags (cv2_set_neighbor) */
+#define CV2_NL_REBUILD_EVERY_STEP 1u /* Neighborlist::set_rebuild_test (neighborlist.cpp:107-110) */
+
+typedef struct cv2_ctx cv2_ctx;
+
+typedef struct {
+  int64_t natoms, nghost;   /* owned atoms; host-halo ghosts */
+  int64_t steps;            /* integration steps taken */
+  int64_t rebuilds;         /* neighbour-list builds */
+  int64_t list_entries;     /* entries in the current list (full list) */
+  int32_t max_neighbors;    /* longest list of the current build */
+  int32_t capacity;         /* list slots per atom */
+  int32_t ncell[3];         /* cell grid */
+  int32_t precision;
+} cv2_stats;
+
+/* Context lifetime.  Stands in for Caviar2 (caviar2.hpp:39-66), which owns
+ * Atom_data, Domain, Neighborlist and Md_simulator by value. */
+int cv2_create(cv2_ctx **ctx, int device);
+int cv2_destroy(cv2_ctx *ctx);
+const char *cv2_last_error(const cv2_ctx *ctx);
+
+/* Box and ghost cutoff.  Domain::set_lower_global/set_upper_global/
+ * set_boundary_condition + generate (domain.hpp:38-42, domain.cpp:295-328)
+ * and Atom_data::set_ghost_cutoff (atom_data.cpp:302-305).  Periodic axes
+ * need hi-lo > 2*ghost_cutoff. */
+int cv2_set_box(cv2_ctx *ctx, const double lo[3], const double hi[3], const int bc[3],
+                double ghost_cutoff);
+
+/* Lennard-Jones parameters.  force_field::Lj::set_cutoff / set_epsilon /
+ * set_sigma / set_cutoff_list (lj.hpp:67-79, lj.cpp:74-140) and
+ * Atom_data::add_type_mass (atom_data.cpp:368-380).  eps, sigma and
+ * cutoff_list (nullable) are ntypes*ntypes row-major [t_i][t_j]; mass has
+ * ntypes entries. */
+int cv2_set_lj(cv2_ctx *ctx, int ntypes, const double *eps, const double *sigma,
+               const double *cutoff_list, double cutoff, const double *mass);
+
+/* Verlet-list parameters.  Neighborlist::set_cutoff /
+ * set_cutoff_extra_coef / set_rebuild_test (neighborlist.hpp:46-54,
+ * neighborlist.cpp:202-234).  The list holds pairs closer than
+ * cutoff*extra_coef; it is rebuilt when an atom has moved more than half
+ * the skin since the last build (neighborlist.cpp:100-148, :318). */
+int cv2_set_neighbor(cv2_ctx *ctx, double cutoff, double extra_coef, unsigned flags);
+
+/* CV2_PREC_FP64 or CV2_PREC_FP32_FIXED. */
+int cv2_set_precision(cv2_ctx *ctx, int precision);
+
+/* Owned atoms.  Atom_data::add_atom(id, type, pos, vel) (atom_data.cpp:603-637)
+ * for n atoms at once; pos and vel are AoS xyz (Vector3d<double>); id may be
+ * NULL (id = index, as add_unique_atom_molecule.cpp:31 assigns). */
+int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *vel,
+                     const int32_t *type, const int64_t *id);
+
+/* Host-supplied ghost halo for one domain's share (the images that
+ * exchange_ghost_mpi_shared_atoms.cpp:274-311 would deliver).  Switches the
+ * context to host-halo mode: the box is then treated as open and only these
+ * ghosts provide neighbours across its faces.  Call again each step with the
+ * refreshed halo; a changed count forces a rebuild (neighborlist.cpp:118-121). */
+int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *type,
+                   const int64_t *id);
+
+/* Neighbour-list build.  Neighborlist::build(bool) (neighborlist.cpp:77-98):
+ * wraps positions into the box (exchange_owned_single_md_domain.cpp:29-57),
+ * re-sorts atoms by cell and rebuilds the list when force != 0 or when
+ * update_is_needed() holds.  *rebuilt (nullable) reports whether it did. */
+int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt);
+
+/* Accelerations, virial and potential energy at the current positions.
+ * Atom_data::reset_owned_acceleration + force_field::Lj::calculate_acceleration
+ * (atom_data.cpp:424-432, lj.cpp:393-517). */
+int cv2_compute_forces(cv2_ctx *ctx);
+
+/* nsteps of velocity Verlet.  Md_simulator::integrate_velocity_verlet
+ * (md_simulator.cpp:617-729) including re_calculate_acc (:422-437).  The
+ * state must be consistent (forces computed) on entry; it is on exit. */
+int cv2_run_vv(cv2_ctx *ctx, int64_t nsteps, double dt);
+
+/* nsteps of the Langevin integrator of Md_simulator::
+ * integrate_velocity_verlet_langevin (md_simulator.cpp:731-869) with
+ * a, b, c of md_simulator.cpp:398-405.  The Gaussian noise is Philox4x32-10
+ * keyed by (seed, atom id, step) instead of the reference's mt19937 streams. */
+int cv2_run_langevin(cv2_ctx *ctx, int64_t nsteps, double dt, double gamma, double kbt,
+                     uint64_t seed);
+
+/* Thermodynamics of the current state: kinetic energy (kinetic_energy.cpp:
+ * 26-70, n_r_df as set), potential energy and virial of the last force pass
+ * (Atom_data::virialForce, lj.cpp:487-490,516).  Any pointer may be NULL. */
+int cv2_thermo(cv2_ctx *ctx, int n_r_df, double *ke, double *pe, double *virial);
+
+/* Copy state back in upload order (AoS xyz; any pointer may be NULL). */
+int cv2_download(cv2_ctx *ctx, double *pos, double *vel, double *acc);
+
+/* The current Verlet list as pairs in the reference's half-list convention
+ * (Neighborlist::neighlist, neighborlist.hpp:102: owned j > i, plus every
+ * ghost image in range).  Each pair is three int64: upload index of i, upload
+ * index of j (or of the ghost's owner / host ghost index + n in host-halo
+ * mode), image code (vx+1)+3(vy+1)+9(vz+1) with image = pos_j - v*L.
+ * *npairs receives the total; at most cap pairs are written. */
+int cv2_get_pairs(cv2_ctx *ctx, int64_t cap, int64_t *npairs, int64_t *pairs);
+
+int cv2_get_stats(cv2_ctx *ctx, cv2_stats *out);
+
+/* Kernel timing with HIP events recorded on the context's stream around each
+ * kernel group (no reference counterpart; the reference only logs the wall
+ * time of Md_simulator::run, md_simulator.cpp:325-353). */
+typedef struct {
+  double force_ms;      /* summed duration of the pair-force kernel launches */
+  int64_t force_calls;
+  double build_ms;      /* rebuild-check + build sequence (also when it exits early) */
+  double integrate_ms;  /* integrator kernels */
+} cv2_timing;
+
+/* enable != 0 starts (and zeroes) timing; 0 stops it. */
+int cv2_set_timing(cv2_ctx *ctx, int enable);
+int cv2_get_timing(cv2_ctx *ctx, cv2_timing *out);
+
+/* Wait for all work queued on the context's stream. */
+int cv2_synchronize(cv2_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,49 @@
+"""Shared helpers for the parity tests: the oracle's view of a configuration
+(pair keys, accelerations, energy) on the same inputs the device sees."""
+import numpy as np
+
+import oracle as O
+
+
+def oracle_pairs(pos, s, cell=True):
+    """Reference pair set as sorted int64 rows (i, j, image_code); ghost
+    entries are keyed by their owner and image (the device stores no ghosts)."""
+    box = O.make_box(s["lo"], s["hi"], s.get("bc", (1, 1, 1)))
+    p, _ = O.wrap(pos, box)
+    gpos, owner, code = O.ghosts(p, box, s["ghost_cutoff"])
+    if cell:
+        offs, nbr = O.nlist_cell(p, gpos, box, s["nl_cutoff"], s["nl_coef"])
+    else:
+        offs, nbr = O.nlist_brute(p, gpos, s["nl_cutoff"], s["nl_coef"])
+    n = len(p)
+    i = np.repeat(np.arange(n), np.diff(offs))
+    j = nbr.astype(np.int64)
+    g = j >= n
+    jj = np.where(g, owner[np.minimum(j - n, len(owner) - 1)] if len(owner) else j, j)
+    cc = np.where(g, code[np.minimum(j - n, len(code) - 1)] if len(code) else 13, 13)
+    return sort_rows(np.stack([i, jj, cc], 1))
+
+
+def sort_rows(a):
+    a = np.asarray(a, np.int64).reshape(-1, 3)
+    return a[np.lexsort((a[:, 2], a[:, 1], a[:, 0]))]
+
+
+def oracle_forces(pos, s):
+    """Reference accelerations, virial and (added) potential energy."""
+    box = O.make_box(s["lo"], s["hi"], s.get("bc", (1, 1, 1)))
+    p, _ = O.wrap(pos, box)
+    gpos, owner, _ = O.ghosts(p, box, s["ghost_cutoff"])
+    offs, nbr = O.nlist_cell(p, gpos, box, s["nl_cutoff"], s["nl_coef"])
+    t = np.asarray(s["type"], np.int32)
+    acc, vir, pe = O.lj(p, t, np.arange(len(p)), gpos, t[owner], owner, offs, nbr, s["eps"],
+                        s["sigma"], s["lj_cutoff"], s["mass"], s.get("cutoff_list"))
+    return p, acc, vir, pe
+
+
+def force_error(a, ref):
+    """max_i |a_i - ref_i| / max(|ref_i|, rms|ref|), and the rms of |a - ref| / rms|ref|."""
+    d = np.linalg.norm(a - ref, axis=1)
+    m = np.linalg.norm(ref, axis=1)
+    rms = np.sqrt((m ** 2).mean())
+    return float((d / np.maximum(m, rms)).max()), float(np.sqrt((d ** 2).mean()) / rms)

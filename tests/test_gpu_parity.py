@@ -1,0 +1,181 @@
+"""Parity of the HIP path against the oracle, through the C ABI.
+
+Pair sets: bit-exact, as (i, j, image) keys in the reference's half-list
+convention (owned j > i plus every ghost image in range).
+Forces: FP64 path within 1e-12 (relative to max(|a_i|, rms|a|, 1)) --
+the only difference from the oracle is summation order; FP32+fixed-point
+path within the tolerance stated in FP32_TOL.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from cv2md import device as D
+from cv2md.systems import ka_mixture, lj_fluid
+from helpers import force_error, oracle_forces, oracle_pairs, sort_rows
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+# FP32 pair arithmetic: per-atom |da| / max(|a|, rms|a|) (precision)
+FP32_TOL_MAX = 2e-5
+FP32_TOL_RMS = 5e-6
+
+
+def perturbed(s, amp, seed):
+    s = dict(s)
+    rng = np.random.default_rng(seed)
+    s["pos"] = s["pos"] + rng.normal(0.0, amp, s["pos"].shape)
+    return s
+
+
+def t001_like():
+    # ghost cutoff (2.8) below the list cutoff (2.5 * 1.2 = 3.0): images whose
+    # owner is between 2.8 and 3.0 from a face are absent, as in the reference
+    s = perturbed(lj_fluid(6, seed=5), 0.08, 5)
+    s["nl_coef"] = 1.2
+    s["ghost_cutoff"] = 2.8
+    return s
+
+
+def open_z():
+    s = perturbed(lj_fluid(6, seed=6), 0.08, 6)
+    s["bc"] = (1, 1, 0)
+    return s
+
+
+CASES = {
+    "c1_fcc": lambda: lj_fluid(10),
+    "c1_perturbed": lambda: perturbed(lj_fluid(10), 0.1, 1),
+    "ka_500": lambda: ka_mixture(500, seed=3),
+    "ka_4000": lambda: ka_mixture(4000, seed=4),
+    "small_box_2cells": lambda: perturbed(lj_fluid(4), 0.05, 2),
+    "ghost_below_list_cutoff": t001_like,
+    "open_z": open_z,
+}
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request):
+    return request.param, CASES[request.param]()
+
+
+def test_pair_set_bit_exact(case):
+    name, s = case
+    print(f"[pairs] {name} n={len(s['pos'])}", flush=True)
+    with D.Context.from_sysdef(s) as c:
+        c.build(True)
+        got = sort_rows(c.pairs())
+        st = c.stats()
+    exp = oracle_pairs(s["pos"], s)
+    assert st["max_neighbors"] <= st["capacity"]
+    assert len(got) == len(exp)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("prec", [D.CV2_PREC_FP64, D.CV2_PREC_FP32_FIXED])
+def test_forces_virial_energy(case, prec):
+    name, s = case
+    print(f"[forces] {name} prec={prec}", flush=True)
+    with D.Context.from_sysdef(s, precision=prec) as c:
+        c.compute_forces()
+        pos, _, acc = c.download()
+        th = c.thermo()
+    p, a_ref, vir, pe = oracle_forces(s["pos"], s)
+    assert np.array_equal(pos, p)
+    scale = max(float(np.sqrt((np.linalg.norm(a_ref, axis=1) ** 2).mean())), 1.0)
+    d = np.linalg.norm(acc - a_ref, axis=1) / np.maximum(np.linalg.norm(a_ref, axis=1), scale)
+    emax, erms = float(d.max()), float(np.sqrt((d ** 2).mean()))
+    print(f"  force error max {emax:.3e} rms {erms:.3e}; virial {th['virial']:.9e} vs {vir:.9e}; "
+          f"pe {th['pe']:.9e} vs {pe:.9e}")
+    esc = max(abs(pe), float(len(p)))
+    vsc = max(abs(vir), float(len(p)))
+    if prec == D.CV2_PREC_FP64:
+        assert emax < FP64_TOL
+        # global sums over ~1e5 pairs in a different order: 1e-10 relative
+        assert abs(th["virial"] - vir) < 1e-10 * vsc
+        assert abs(th["pe"] - pe) < 1e-10 * esc
+    else:
+        assert emax < FP32_TOL_MAX and erms < FP32_TOL_RMS
+        assert abs(th["virial"] - vir) < 1e-5 * vsc
+        assert abs(th["pe"] - pe) < 1e-5 * esc
+
+
+def test_pairs_after_dynamics_bit_exact():
+    """Pair set at a later rebuild, as a function of the device's positions."""
+    s = perturbed(lj_fluid(8, temperature=1.44, seed=9), 0.05, 9)
+    with D.Context.from_sysdef(s) as c:
+        c.run_vv(200, 0.005)
+        c.build(True)
+        pos = c.download()[0]
+        got = sort_rows(c.pairs())
+        assert c.stats()["rebuilds"] >= 3
+    s2 = dict(s, pos=pos)
+    assert np.array_equal(got, oracle_pairs(pos, s2))
+
+
+def test_short_trajectory_matches_oracle():
+    s = perturbed(lj_fluid(10, temperature=1.44, seed=11), 0.05, 11)
+    s["use_cell_list"] = 1
+    with O.OracleSim(s) as o, D.Context.from_sysdef(s) as c:
+        o.setup()
+        for _ in range(10):
+            o.step()
+        c.run_vv(10, s["dt"])
+        po, vo, _ = o.state()
+        pg, vg, _ = c.download()
+    # the reference wraps every step, the device at list builds: compare modulo L
+    L = s["hi"] - s["lo"]
+    d = pg - po
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() < 1e-10
+    assert np.abs(vg - vo).max() < 1e-8
+
+
+@pytest.mark.parametrize("prec", [D.CV2_PREC_FP64, D.CV2_PREC_FP32_FIXED])
+def test_deterministic(prec):
+    s = perturbed(lj_fluid(8, seed=12), 0.05, 12)
+    out = []
+    for _ in range(2):
+        with D.Context.from_sysdef(s, precision=prec) as c:
+            c.run_vv(100, 0.005)
+            out.append(c.download())
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_energy_conservation_short():
+    s = lj_fluid(8, temperature=1.44, seed=13)
+    with D.Context.from_sysdef(s) as c:
+        c.compute_forces()
+        t0 = c.thermo()
+        c.run_vv(2000, 0.005)
+        t1 = c.thermo()
+    e0 = t0["ke"] + t0["pe"]
+    e1 = t1["ke"] + t1["pe"]
+    print(f"E0 {e0:.6f} E1 {e1:.6f} per atom drift {(e1 - e0) / len(s['pos']):.3e}")
+    assert abs(e1 - e0) / len(s["pos"]) < 2e-3
+
+
+def test_host_halo_matches_periodic():
+    """One domain's share with a host-supplied halo: the ghosts are the
+    reference's periodic images, so forces must equal the periodic answer."""
+    s = ka_mixture(4000, seed=21)
+    box = O.make_box(s["lo"], s["hi"], s["bc"])
+    p, _ = O.wrap(s["pos"], box)
+    gpos, owner, _ = O.ghosts(p, box, s["ghost_cutoff"])
+    t = np.asarray(s["type"], np.int32)
+    with D.Context.from_sysdef(dict(s, pos=p)) as c:
+        c.set_ghosts(gpos, t[owner], owner)
+        c.compute_forces()
+        acc = c.download()[2]
+        th = c.thermo()
+        pairs = sort_rows(c.pairs())
+    offs, nbr = O.nlist_brute(p, gpos, s["nl_cutoff"], s["nl_coef"])
+    n = len(p)
+    exp = sort_rows(np.stack([np.repeat(np.arange(n), np.diff(offs)), nbr.astype(np.int64),
+                              np.full(len(nbr), 13)], 1))
+    assert np.array_equal(pairs, exp)
+    _, a_ref, vir, pe = oracle_forces(s["pos"], s)
+    assert np.abs(acc - a_ref).max() < 1e-12 * max(np.abs(a_ref).max(), 1.0)
+    assert abs(th["virial"] - vir) < 1e-12 * abs(vir)
