@@ -126,7 +126,7 @@ hipError_t dalloc(T **p, size_t count) {
 void free_state(DevState &s) {
   void *ptrs[] = {s.pos, s.pos_alt, s.pos_ref, s.q, s.vel, s.vel_alt, s.acc, s.acc_alt, s.tag,
                   s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.cell_count, s.cell_start,
-                  s.gcell_count, s.gcell_start, s.gcell_items, s.nbr, s.nnbr, s.flags,
+                  s.gcell_count, s.gcell_start, s.gcell_items, s.tile_sum, s.nbr, s.nnbr, s.flags,
                   s.partials, s.fx_acc, s.reduce_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -168,6 +168,7 @@ int setup_box(cv2_ctx *ctx) {
     HIPCHK(dalloc(&ctx->s.cell_start, ncell + 1));
     HIPCHK(dalloc(&ctx->s.gcell_count, ncell + 1));
     HIPCHK(dalloc(&ctx->s.gcell_start, ncell + 1));
+    HIPCHK(dalloc(&ctx->s.tile_sum, ncell / 1024 + 2));
     HIPCHK(hipMemsetAsync(ctx->s.gcell_start, 0, sizeof(int) * (ncell + 1), ctx->stream));
   }
   return CV2_OK;
@@ -376,6 +377,7 @@ int cv2_set_precision(cv2_ctx *ctx, int precision) {
   if (precision != CV2_PREC_FP64 && precision != CV2_PREC_FP32_FIXED)
     return fail(ctx, CV2_ERR_BAD_ARG, "unknown precision");
   ctx->precision = precision;
+  ctx->s.write_q = precision != CV2_PREC_FP64;
   ctx->forces_valid = false;
   return CV2_OK;
 }
@@ -476,6 +478,8 @@ int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *t
   ctx->ng = ng;
   s.ng = (int)ng;
   if (count_changed) ctx->need_build = true;  // neighborlist.cpp:118-121
+  if (s.write_q) launch_pack_q(s, ctx->b, ctx->stream);
+  launch_check_disp(s, ctx->b, ctx->stream);  // ghosts compared by index, :139-144
   HIPCHK(hipStreamSynchronize(ctx->stream));  // p4/idv are stack buffers
   return CV2_OK;
 }
@@ -489,7 +493,7 @@ int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt) {
   if (rebuilt) {
     HIPCHK(hipMemcpy(&before, ctx->s.flags + F_REBUILDS, sizeof(int), hipMemcpyDeviceToHost));
   }
-  if (!force) launch_kick_drift(ctx->s, ctx->b, 0.0, ctx->stream);  // displacement check only
+  if (!force) launch_check_disp(ctx->s, ctx->b, ctx->stream);
   queue_build(ctx, force != 0);
   ctx->forces_valid = false;
   rc = check_flags(ctx);
@@ -503,6 +507,7 @@ int cv2_compute_forces(cv2_ctx *ctx) {
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
   if (ctx->need_build) queue_build(ctx, true);
+  if (ctx->precision != CV2_PREC_FP64) launch_pack_q(ctx->s, ctx->b, ctx->stream);
   const int t = span_begin(ctx);
   launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, true, ctx->stream);
   span_end(ctx, t, 0);
@@ -514,6 +519,46 @@ int cv2_compute_forces(cv2_ctx *ctx) {
   return CV2_OK;
 }
 
+// Shared step loop.  On entry the forces of the current positions are
+// known; each iteration closes the previous step and opens the next one in
+// one integrator pass, runs the (device-conditional) list build, and
+// computes forces (energy and virial on the last step only).
+int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
+  u.writeq = ctx->precision != CV2_PREC_FP64;
+  const uint64_t s0 = (uint64_t)ctx->steps;
+  for (int64_t k = 0; k < nsteps; ++k) {
+    u.finish = k > 0;
+    u.start = true;
+    u.step = s0 + (uint64_t)k - 1;
+    int t = span_begin(ctx);
+    launch_update(ctx->s, ctx->b, u, ctx->stream);
+    span_end(ctx, t, 2);
+    t = span_begin(ctx);
+    queue_build(ctx, false);
+    span_end(ctx, t, 1);
+    t = span_begin(ctx);
+    launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
+    span_end(ctx, t, 0);
+    if (ctx->spans.size() > 4000) {
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      spans_collect(ctx);
+    }
+  }
+  if (nsteps > 0) {
+    u.finish = true;
+    u.start = false;
+    u.step = s0 + (uint64_t)nsteps - 1;
+    const int t = span_begin(ctx);
+    launch_update(ctx->s, ctx->b, u, ctx->stream);
+    span_end(ctx, t, 2);
+  }
+  ctx->steps += nsteps;
+  int rc = check_flags(ctx);
+  if (rc) return rc;
+  if (nsteps > 0) return fetch_energy(ctx);
+  return CV2_OK;
+}
+
 int cv2_run_vv(cv2_ctx *ctx, int64_t nsteps, double dt) {
   if (!ctx || nsteps < 0 || dt < 0) return fail(ctx, CV2_ERR_BAD_ARG, "bad step arguments");
   int rc = ensure_ready(ctx);
@@ -522,29 +567,10 @@ int cv2_run_vv(cv2_ctx *ctx, int64_t nsteps, double dt) {
     rc = cv2_compute_forces(ctx);
     if (rc) return rc;
   }
-  for (int64_t k = 0; k < nsteps; ++k) {
-    int t = span_begin(ctx);
-    launch_kick_drift(ctx->s, ctx->b, dt, ctx->stream);
-    span_end(ctx, t, 2);
-    t = span_begin(ctx);
-    queue_build(ctx, false);
-    span_end(ctx, t, 1);
-    t = span_begin(ctx);
-    launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
-    span_end(ctx, t, 0);
-    t = span_begin(ctx);
-    launch_kick(ctx->s, dt, ctx->stream);
-    span_end(ctx, t, 2);
-    if (ctx->spans.size() > 4000) {
-      HIPCHK(hipStreamSynchronize(ctx->stream));
-      spans_collect(ctx);
-    }
-  }
-  ctx->steps += nsteps;
-  rc = check_flags(ctx);
-  if (rc) return rc;
-  if (nsteps > 0) return fetch_energy(ctx);
-  return CV2_OK;
+  UpdateArgs u{};
+  u.langevin = false;
+  u.dt = dt;
+  return run_steps(ctx, nsteps, u);
 }
 
 int cv2_run_langevin(cv2_ctx *ctx, int64_t nsteps, double dt, double gamma, double kbt,
@@ -557,34 +583,15 @@ int cv2_run_langevin(cv2_ctx *ctx, int64_t nsteps, double dt, double gamma, doub
     rc = cv2_compute_forces(ctx);
     if (rc) return rc;
   }
+  UpdateArgs u{};
+  u.langevin = true;
+  u.dt = dt;
   // md_simulator.cpp:398-405
-  const double la = (2.0 - gamma * dt) / (2.0 + gamma * dt);
-  const double lb = std::sqrt(kbt * gamma * 0.5 * dt);
-  const double lc = 2.0 * dt / (2.0 + gamma * dt);
-  for (int64_t k = 0; k < nsteps; ++k) {
-    const uint64_t step = (uint64_t)(ctx->steps + k);
-    int t = span_begin(ctx);
-    launch_langevin_1(ctx->s, ctx->b, ctx->lj_dev, dt, lb, lc, seed, step, ctx->stream);
-    span_end(ctx, t, 2);
-    t = span_begin(ctx);
-    queue_build(ctx, false);
-    span_end(ctx, t, 1);
-    t = span_begin(ctx);
-    launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
-    span_end(ctx, t, 0);
-    t = span_begin(ctx);
-    launch_langevin_2(ctx->s, ctx->lj_dev, dt, la, lb, seed, step, ctx->stream);
-    span_end(ctx, t, 2);
-    if (ctx->spans.size() > 4000) {
-      HIPCHK(hipStreamSynchronize(ctx->stream));
-      spans_collect(ctx);
-    }
-  }
-  ctx->steps += nsteps;
-  rc = check_flags(ctx);
-  if (rc) return rc;
-  if (nsteps > 0) return fetch_energy(ctx);
-  return CV2_OK;
+  u.la = (2.0 - gamma * dt) / (2.0 + gamma * dt);
+  u.lb = std::sqrt(kbt * gamma * 0.5 * dt);
+  u.lc = 2.0 * dt / (2.0 + gamma * dt);
+  u.seed = seed;
+  return run_steps(ctx, nsteps, u);
 }
 
 int cv2_thermo(cv2_ctx *ctx, int n_r_df, double *ke, double *pe, double *virial) {

@@ -82,6 +82,7 @@ struct DevState {
   int *tag, *tag_alt, *id, *id_alt;
   int *cell, *slot, *perm;     // per owned atom during a build
   int *cell_count, *cell_start; // ncell (+1)
+  int *tile_sum;                // per scan tile (ncell / 1024 + 1)
   int *gcell_count, *gcell_start, *gcell_items; // host-halo ghost cell list
   uint32_t *nbr;
   int *nnbr;
@@ -90,6 +91,7 @@ struct DevState {
   long long *fx_acc;           // fixed-point reductions (energy, virial)
   double *reduce_out;          // final reductions
   int ncell;
+  bool write_q;                // keep q current (FP32 pair kernel in use)
 };
 
 // ---- launchers (defined in the .hip files) ----
@@ -97,12 +99,16 @@ void launch_pack_q(const DevState &s, const BoxDev &b, hipStream_t st);
 void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo);
 void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                    int precision, bool energy, hipStream_t st);
-void launch_kick_drift(DevState &s, const BoxDev &b, double dt, hipStream_t st);
-void launch_kick(DevState &s, double dt, hipStream_t st);
-void launch_langevin_1(DevState &s, const BoxDev &b, const LJDev *lj_dev, double dt, double lb,
-                       double lc, uint64_t seed, uint64_t step, hipStream_t st);
-void launch_langevin_2(DevState &s, const LJDev *lj_dev, double dt, double la, double lb,
-                       uint64_t seed, uint64_t step, hipStream_t st);
+// One pass of the integrator: finish = close step `step` (velocity update
+// after its forces), start = open step `step + 1` (velocity and position
+// update, fixed-point coordinates when writeq, rebuild check).
+struct UpdateArgs {
+  bool langevin, finish, start, writeq;
+  double dt, la, lb, lc;
+  uint64_t seed, step;
+};
+void launch_update(DevState &s, const BoxDev &b, const UpdateArgs &u, hipStream_t st);
+void launch_check_disp(DevState &s, const BoxDev &b, hipStream_t st);
 void launch_thermo(DevState &s, const LJDev *lj_dev, hipStream_t st);
 void launch_set_flag(int *flags, int which, int value, hipStream_t st);
 

@@ -64,32 +64,55 @@ __global__ void k_wrap_bin(double4 *__restrict__ pos, int n, BoxDev b, int *__re
   slot[i] = atomicAdd(&cell_count[ci], 1);
 }
 
-// Exclusive scan of cell counts (one work-group; ncell up to a few million).
-__global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ count,
-                                               int *__restrict__ start, int ncell,
-                                               const int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
-  __shared__ int part[1024];
+// Exclusive scan of cell counts in two passes over tiles of kScanTile cells:
+// (1) each work-group scans its tile in LDS and stores the tile total;
+// (2) each work-group adds the sum of the totals before it.
+constexpr int kScanTile = 1024;
+
+__device__ inline int block_exclusive_scan(int v, int *sh) {
   const int t = threadIdx.x;
-  const int chunk = (ncell + 1023) / 1024;
-  const int b0 = t * chunk;
-  const int b1 = min(ncell, b0 + chunk);
-  int s = 0;
-  for (int c = b0; c < b1; ++c) s += count[c];
-  part[t] = s;
+  sh[t] = v;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = t >= off ? part[t - off] : 0;
+  for (int off = 1; off < kScanTile; off <<= 1) {
+    const int a = t >= off ? sh[t - off] : 0;
     __syncthreads();
-    part[t] += v;
+    sh[t] += a;
     __syncthreads();
   }
-  int run = part[t] - s;
-  for (int c = b0; c < b1; ++c) {
-    start[c] = run;
-    run += count[c];
+  return sh[t] - v;
+}
+
+__global__ void __launch_bounds__(kScanTile)
+    k_scan_tiles(const int *__restrict__ count, int *__restrict__ start,
+                 int *__restrict__ tile_sum, int ncell, const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  __shared__ int sh[kScanTile];
+  const int c = blockIdx.x * kScanTile + threadIdx.x;
+  const int v = c < ncell ? count[c] : 0;
+  const int ex = block_exclusive_scan(v, sh);
+  if (c < ncell) start[c] = ex;
+  if (threadIdx.x == kScanTile - 1) tile_sum[blockIdx.x] = ex + v;
+}
+
+__global__ void __launch_bounds__(kScanTile)
+    k_scan_add(int *__restrict__ start, const int *__restrict__ tile_sum, int ncell, int ntiles,
+               const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  __shared__ int sh[kScanTile];
+  // offset of this tile: sum of the totals of the tiles before it
+  int s = 0;
+  for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanTile) s += tile_sum[k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kScanTile / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
   }
-  if (t == 1023) start[ncell] = part[1023];
+  const int off = sh[0];
+  const int c = blockIdx.x * kScanTile + threadIdx.x;
+  if (c < ncell) start[c] += off;
+  if ((int)blockIdx.x == ntiles - 1 && threadIdx.x == 0)
+    start[ncell] = off + tile_sum[blockIdx.x];
 }
 
 __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
@@ -103,22 +126,36 @@ __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ 
 
 // Order atoms inside each cell by upload index so that the sorted order,
 // and with it every list and sum, is independent of atomic arrival order.
-__global__ void k_sort_cells(int *__restrict__ perm, const int *__restrict__ start,
-                             const int *__restrict__ tag, int ncell,
-                             const int *__restrict__ flags) {
+// One wavefront per cell: each lane holds one atom and finds its rank by
+// comparing its key with every other lane's (keys are unique); cells with
+// more than 64 atoms fall back to an insertion sort by one lane.
+__global__ void __launch_bounds__(256)
+    k_sort_cells(int *__restrict__ perm, const int *__restrict__ start,
+                 const int *__restrict__ tag, int ncell, const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (c >= ncell) return;
-  const int a = start[c], e = start[c + 1];
-  for (int k = a + 1; k < e; ++k) {
+  const int a = start[c], m = start[c + 1] - a;
+  if (m <= 1) return;
+  if (m <= 64) {
+    const int v = lane < m ? perm[a + lane] : 0;
+    const int key = lane < m ? tag[v] : 0x7fffffff;
+    int rank = 0;
+    for (int l = 0; l < m; ++l) rank += __shfl(key, l, 64) < key;
+    if (lane < m) perm[a + rank] = v;
+    return;
+  }
+  if (lane != 0) return;
+  for (int k = a + 1; k < a + m; ++k) {
     const int v = perm[k];
     const int tv = tag[v];
-    int m = k - 1;
-    while (m >= a && tag[perm[m]] > tv) {
-      perm[m + 1] = perm[m];
-      --m;
+    int q = k - 1;
+    while (q >= a && tag[perm[q]] > tv) {
+      perm[q + 1] = perm[q];
+      --q;
     }
-    perm[m + 1] = v;
+    perm[q + 1] = v;
   }
 }
 
@@ -146,16 +183,30 @@ __global__ void k_permute(const int *__restrict__ perm, int n, const double4 *__
   id_out[k] = id[o];
 }
 
+// Fixed-point copy of positions for the FP32 pair kernel: (x - lo) * 2^32 / L
+// reduced modulo 2^32, so that int32 differences are minimum-image
+// separations on periodic axes.
+__device__ inline int4 qpack(const double4 &p, const BoxDev &b) {
+  int4 r;
+  r.x = (int)(unsigned)(unsigned long long)llrint((p.x - b.lo[0]) * b.qscale[0]);
+  r.y = (int)(unsigned)(unsigned long long)llrint((p.y - b.lo[1]) * b.qscale[1]);
+  r.z = (int)(unsigned)(unsigned long long)llrint((p.z - b.lo[2]) * b.qscale[2]);
+  r.w = (int)p.w;
+  return r;
+}
+
 __global__ void k_copy_back(int n, const double4 *__restrict__ pos_alt, double4 *__restrict__ pos,
                             const double *__restrict__ vel_alt, double *__restrict__ vel,
                             const double *__restrict__ acc_alt, double *__restrict__ acc,
                             const int *__restrict__ tag_alt, int *__restrict__ tag,
                             const int *__restrict__ id_alt, int *__restrict__ id,
-                            const int *__restrict__ flags) {
+                            int4 *__restrict__ q, BoxDev b, const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  pos[k] = pos_alt[k];
+  const double4 p = pos_alt[k];
+  pos[k] = p;
+  if (q) q[k] = qpack(p, b);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     vel[d * n + k] = vel_alt[d * n + k];
@@ -248,6 +299,19 @@ __device__ inline bool axis_image(double pi, double pj, double lo, double hi, do
 }
 
 // Verlet list: one thread per owned atom, full list (both directions).
+//
+// Stencil cells whose box lies farther than the list cutoff from the atom
+// are skipped (with a relative margin of 1e-9 so that rounding in the cell
+// assignment can never drop a neighbour).  Atoms whose cell touches no
+// periodic face take a fast path without image logic: for them every
+// candidate's minimum image is the direct one.
+__device__ inline bool list_pair(double px, double py, double pz, double gx, double gy,
+                                 double gz, double ce_sq) {
+  const double dx = __dsub_rn(gx, px), dy = __dsub_rn(gy, py), dz = __dsub_rn(gz, pz);
+  const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+  return r2 < ce_sq;
+}
+
 __global__ void __launch_bounds__(256)
     k_nlist(const double4 *__restrict__ pos, int n, BoxDev b, const int *__restrict__ cstart,
             const int *__restrict__ gstart, const int *__restrict__ gitems,
@@ -257,34 +321,57 @@ __global__ void __launch_bounds__(256)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 pi = pos[i];
-  const int cx = axis_cell(pi.x, b.glo[0], b.cinv[0], b.nc[0]);
-  const int cy = axis_cell(pi.y, b.glo[1], b.cinv[1], b.nc[1]);
-  const int cz = axis_cell(pi.z, b.glo[2], b.cinv[2], b.nc[2]);
-  int sx[3], sy[3], sz[3];
-  const int nx = axis_stencil(cx, b.nc[0], b.bc[0], sx);
-  const int ny = axis_stencil(cy, b.nc[1], b.bc[1], sy);
-  const int nz = axis_stencil(cz, b.nc[2], b.bc[2], sz);
+  const double pc[3] = {pi.x, pi.y, pi.z};
+  int cc[3], sten[3][3], nst[3];
+  double gap[3][3];  // distance from the atom to stencil cell k along axis d
+  bool interior = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    cc[d] = axis_cell(pc[d], b.glo[d], b.cinv[d], b.nc[d]);
+    nst[d] = axis_stencil(cc[d], b.nc[d], b.bc[d], sten[d]);
+    const double w = 1.0 / b.cinv[d];
+    const double off = pc[d] - (b.glo[d] + cc[d] * w);  // position inside the cell
+    const bool wrapped = b.bc[d] && b.nc[d] <= 3;       // all cells, offsets ambiguous
+    for (int k = 0; k < nst[d]; ++k) {
+      const int dc = sten[d][k] - cc[d];
+      gap[d][k] = wrapped ? 0.0 : (dc < 0 ? off : (dc > 0 ? w - off : 0.0));
+      if (gap[d][k] < 0.0) gap[d][k] = 0.0;
+    }
+    if (b.bc[d] && (cc[d] == 0 || cc[d] == b.nc[d] - 1)) interior = false;
+  }
+  const double prune = b.ce_sq * (1.0 + 1e-9);
   int cnt = 0;
-  for (int a = 0; a < nz; ++a)
-    for (int bb = 0; bb < ny; ++bb)
-      for (int c = 0; c < nx; ++c) {
-        const int cell = sx[c] + b.nc[0] * (sy[bb] + b.nc[1] * sz[a]);
+  for (int a = 0; a < nst[2]; ++a)
+    for (int bb = 0; bb < nst[1]; ++bb) {
+      const double gyz = gap[2][a] * gap[2][a] + gap[1][bb] * gap[1][bb];
+      if (gyz > prune) continue;
+      for (int c = 0; c < nst[0]; ++c) {
+        if (gyz + gap[0][c] * gap[0][c] > prune) continue;
+        const int cell = sten[0][c] + b.nc[0] * (sten[1][bb] + b.nc[1] * sten[2][a]);
         const int j1 = cstart[cell + 1];
-        for (int j = cstart[cell]; j < j1; ++j) {
-          if (j == i) continue;
-          const double4 pj = pos[j];
-          int vx, vy, vz;
-          double gx, gy, gz;
-          if (!axis_image(pi.x, pj.x, b.lo[0], b.hi[0], b.L[0], b.halfL[0], b.ghost_cut, b.bc[0], vx, gx)) continue;
-          if (!axis_image(pi.y, pj.y, b.lo[1], b.hi[1], b.L[1], b.halfL[1], b.ghost_cut, b.bc[1], vy, gy)) continue;
-          if (!axis_image(pi.z, pj.z, b.lo[2], b.hi[2], b.L[2], b.halfL[2], b.ghost_cut, b.bc[2], vz, gz)) continue;
-          const double dx = __dsub_rn(gx, pi.x), dy = __dsub_rn(gy, pi.y), dz = __dsub_rn(gz, pi.z);
-          const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-          if (r2 < b.ce_sq) {
-            if (cnt < cap)
-              nbr[(size_t)cnt * stride + i] =
-                  (uint32_t)j | ((uint32_t)(vx + 1) << 26) | ((uint32_t)(vy + 1) << 28) | ((uint32_t)(vz + 1) << 30);
-            ++cnt;
+        if (interior) {
+          for (int j = cstart[cell]; j < j1; ++j) {
+            const double4 pj = pos[j];
+            if (j != i && list_pair(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, b.ce_sq)) {
+              if (cnt < cap) nbr[(size_t)cnt * stride + i] = (uint32_t)j | (kNoImage << 26);
+              ++cnt;
+            }
+          }
+        } else {
+          for (int j = cstart[cell]; j < j1; ++j) {
+            if (j == i) continue;
+            const double4 pj = pos[j];
+            int vx, vy, vz;
+            double gx, gy, gz;
+            if (!axis_image(pi.x, pj.x, b.lo[0], b.hi[0], b.L[0], b.halfL[0], b.ghost_cut, b.bc[0], vx, gx)) continue;
+            if (!axis_image(pi.y, pj.y, b.lo[1], b.hi[1], b.L[1], b.halfL[1], b.ghost_cut, b.bc[1], vy, gy)) continue;
+            if (!axis_image(pi.z, pj.z, b.lo[2], b.hi[2], b.L[2], b.halfL[2], b.ghost_cut, b.bc[2], vz, gz)) continue;
+            if (list_pair(pi.x, pi.y, pi.z, gx, gy, gz, b.ce_sq)) {
+              if (cnt < cap)
+                nbr[(size_t)cnt * stride + i] = (uint32_t)j | ((uint32_t)(vx + 1) << 26) |
+                                                ((uint32_t)(vy + 1) << 28) | ((uint32_t)(vz + 1) << 30);
+              ++cnt;
+            }
           }
         }
         if (host_halo) {
@@ -292,15 +379,14 @@ __global__ void __launch_bounds__(256)
           for (int t = gstart[cell]; t < g1; ++t) {
             const int j = n + gitems[t];
             const double4 pj = pos[j];
-            const double dx = __dsub_rn(pj.x, pi.x), dy = __dsub_rn(pj.y, pi.y), dz = __dsub_rn(pj.z, pi.z);
-            const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-            if (r2 < b.ce_sq) {
+            if (list_pair(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, b.ce_sq)) {
               if (cnt < cap) nbr[(size_t)cnt * stride + i] = (uint32_t)j | (kNoImage << 26);
               ++cnt;
             }
           }
         }
       }
+    }
   nnbr[i] = cnt < cap ? cnt : cap;
   if (cnt > cap) atomicOr(&flags[F_OVERFLOW], 1);
   atomicMax(&flags[F_MAXNBR], cnt);
@@ -327,24 +413,22 @@ __global__ void k_build_begin(int *__restrict__ flags, int *__restrict__ cell_co
   }
 }
 
-// Fixed-point copy of positions for the FP32 pair kernel: (x - lo) * 2^32 / L
-// reduced modulo 2^32, so that int32 differences are minimum-image
-// separations on periodic axes.
 __global__ void k_pack_q(const double4 *__restrict__ pos, int4 *__restrict__ q, int ntot, BoxDev b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ntot) return;
-  const double4 p = pos[i];
-  int4 r;
-  r.x = (int)(unsigned)(unsigned long long)llrint((p.x - b.lo[0]) * b.qscale[0]);
-  r.y = (int)(unsigned)(unsigned long long)llrint((p.y - b.lo[1]) * b.qscale[1]);
-  r.z = (int)(unsigned)(unsigned long long)llrint((p.z - b.lo[2]) * b.qscale[2]);
-  r.w = (int)p.w;
-  q[i] = r;
+  q[i] = qpack(pos[i], b);
 }
 
 __global__ void k_set_flag(int *flags, int which, int value) { flags[which] = value; }
 
 inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+void launch_scan(const int *count, int *start, int *tile_sum, int ncell, const int *flags,
+                 hipStream_t st) {
+  const int tiles = nblk(ncell, kScanTile);
+  k_scan_tiles<<<tiles, kScanTile, 0, st>>>(count, start, tile_sum, ncell, flags);
+  k_scan_add<<<tiles, kScanTile, 0, st>>>(start, tile_sum, ncell, tiles, flags);
+}
 
 }  // namespace
 
@@ -366,21 +450,22 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) 
                                                 ghosts ? s.gcell_count : nullptr);
   if (n > 0) {
     k_wrap_bin<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell, s.slot, s.cell_count, s.flags);
-    k_scan<<<1, 1024, 0, st>>>(s.cell_count, s.cell_start, nc, s.flags);
+    launch_scan(s.cell_count, s.cell_start, s.tile_sum, nc, s.flags, st);
     k_scatter<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.slot, s.cell_start, s.perm, n, s.flags);
-    k_sort_cells<<<nblk(nc, 256), 256, 0, st>>>(s.perm, s.cell_start, s.tag, nc, s.flags);
+    k_sort_cells<<<nblk(nc, 4), 256, 0, st>>>(s.perm, s.cell_start, s.tag, nc, s.flags);
     k_permute<<<nblk(n, 256), 256, 0, st>>>(s.perm, n, s.pos, s.pos_alt, s.pos_ref, s.vel,
                                             s.vel_alt, s.acc, s.acc_alt, s.tag, s.tag_alt, s.id,
                                             s.id_alt, s.flags);
     // The host cannot know whether the device ran the build, so the new
     // order is copied back instead of swapping buffer pointers.
     k_copy_back<<<nblk(n, 256), 256, 0, st>>>(n, s.pos_alt, s.pos, s.vel_alt, s.vel, s.acc_alt,
-                                              s.acc, s.tag_alt, s.tag, s.id_alt, s.id, s.flags);
+                                              s.acc, s.tag_alt, s.tag, s.id_alt, s.id,
+                                              s.write_q ? s.q : nullptr, b, s.flags);
   }
   if (ghosts) {
     k_ghost_bin<<<nblk(s.ng, 256), 256, 0, st>>>(s.pos, s.pos_ref, n, s.ng, b, s.cell + n,
                                                   s.slot + n, s.gcell_count, s.flags);
-    k_scan<<<1, 1024, 0, st>>>(s.gcell_count, s.gcell_start, nc, s.flags);
+    launch_scan(s.gcell_count, s.gcell_start, s.tile_sum, nc, s.flags, st);
     k_ghost_scatter<<<nblk(s.ng, 256), 256, 0, st>>>(s.cell + n, s.slot + n, s.gcell_start,
                                                       s.gcell_items, s.ng, s.flags);
     k_ghost_sort_cells<<<nblk(nc, 256), 256, 0, st>>>(s.gcell_items, s.gcell_start, nc, s.flags);

@@ -12,6 +12,9 @@
 // (Salmon, Moraes, Dror, Shaw, SC'11) keyed by (seed, atom id, step) and
 // Box-Muller, so it needs no state and is independent of atom order.
 //
+// One kernel, k_update, applies the end of step s (after its forces) and the
+// start of step s+1 (up to its positions) in a single pass over the atoms,
+// and also writes the fixed-point coordinates the FP32 force kernel reads.
 // The rebuild check of Neighborlist::update_is_needed (neighborlist.cpp:
 // 100-148) -- any atom displaced by more than half the skin since the last
 // build -- is fused into the position update and raises the device flag
@@ -25,43 +28,6 @@ namespace {
 constexpr int kBlock = 256;
 
 inline int nblk(int n, int b) { return (n + b - 1) / b; }
-
-__device__ inline void check_disp(const double4 &p, const double4 &r, double thresh_sq,
-                                  int *flags) {
-  const double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
-  if (dx * dx + dy * dy + dz * dz > thresh_sq) atomicOr(&flags[F_REBUILD], 1);
-}
-
-__global__ void __launch_bounds__(kBlock)
-    k_kick_drift(double4 *__restrict__ pos, const double4 *__restrict__ pos_ref,
-                 double *__restrict__ vel, const double *__restrict__ acc, int n, double dt,
-                 double thresh_sq, int *__restrict__ flags) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const double hdt = 0.5 * dt;
-  double4 p = pos[i];
-  const double vx = vel[i] + acc[i] * hdt;
-  const double vy = vel[n + i] + acc[n + i] * hdt;
-  const double vz = vel[2 * n + i] + acc[2 * n + i] * hdt;
-  p.x += vx * dt;
-  p.y += vy * dt;
-  p.z += vz * dt;
-  pos[i] = p;
-  vel[i] = vx;
-  vel[n + i] = vy;
-  vel[2 * n + i] = vz;
-  check_disp(p, pos_ref[i], thresh_sq, flags);
-}
-
-__global__ void __launch_bounds__(kBlock)
-    k_kick(double *__restrict__ vel, const double *__restrict__ acc, int n, double dt) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const double hdt = 0.5 * dt;
-  vel[i] += acc[i] * hdt;
-  vel[n + i] += acc[n + i] * hdt;
-  vel[2 * n + i] += acc[2 * n + i] * hdt;
-}
 
 // ---- Philox4x32-10 ----
 struct U4 {
@@ -100,43 +66,83 @@ __device__ inline void gauss3(uint64_t seed, int id, uint64_t step, double &g0, 
   (void)sb;
 }
 
-__global__ void __launch_bounds__(kBlock)
-    k_langevin_1(double4 *__restrict__ pos, const double4 *__restrict__ pos_ref,
-                 double *__restrict__ vel, const double *__restrict__ acc,
-                 const int *__restrict__ id, int n, double dt, double lb, double lc,
-                 uint64_t seed, uint64_t step, double thresh_sq, int *__restrict__ flags) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  double ex, ey, ez;
-  gauss3(seed, id[i], step, ex, ey, ez);
-  // md_simulator.cpp:778: v += 0.5*a*dt + b*eta
-  const double vx = vel[i] + (acc[i] * 0.5 * dt + ex * lb);
-  const double vy = vel[n + i] + (acc[n + i] * 0.5 * dt + ey * lb);
-  const double vz = vel[2 * n + i] + (acc[2 * n + i] * 0.5 * dt + ez * lb);
-  double4 p = pos[i];
-  // :835: x += v*c
-  p.x += vx * lc;
-  p.y += vy * lc;
-  p.z += vz * lc;
-  pos[i] = p;
-  vel[i] = vx;
-  vel[n + i] = vy;
-  vel[2 * n + i] = vz;
-  check_disp(p, pos_ref[i], thresh_sq, flags);
+__device__ inline int qcoord(double x, double lo, double scale) {
+  return (int)(unsigned)(unsigned long long)llrint((x - lo) * scale);
 }
 
+struct StepParams {
+  double dt;
+  double la, lb, lc;  // Langevin a, b, c (md_simulator.cpp:398-405)
+  uint64_t seed;
+  uint64_t step;      // step index s of the finishing half; s+1 starts
+  double thresh_sq;
+  double lo[3], qscale[3];
+};
+
+// LANGEVIN selects the integrator; FINISH applies the velocity update that
+// closes step s, START the updates that open step s+1; WRITEQ refreshes the
+// fixed-point coordinates.
+template <bool LANGEVIN, bool FINISH, bool START, bool WRITEQ>
 __global__ void __launch_bounds__(kBlock)
-    k_langevin_2(double *__restrict__ vel, const double *__restrict__ acc,
-                 const int *__restrict__ id, int n, double dt, double la, double lb,
-                 uint64_t seed, uint64_t step) {
+    k_update(double4 *__restrict__ pos, const double4 *__restrict__ pos_ref,
+             double *__restrict__ vel, const double *__restrict__ acc, const int *__restrict__ id,
+             int4 *__restrict__ q, int n, StepParams p, int *__restrict__ flags) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  double ex, ey, ez;
-  gauss3(seed, id[i], step, ex, ey, ez);
-  // :860: v = a*v + b*eta + 0.5*acc*dt
-  vel[i] = vel[i] * la + ex * lb + acc[i] * 0.5 * dt;
-  vel[n + i] = vel[n + i] * la + ey * lb + acc[n + i] * 0.5 * dt;
-  vel[2 * n + i] = vel[2 * n + i] * la + ez * lb + acc[2 * n + i] * 0.5 * dt;
+  double v[3] = {vel[i], vel[n + i], vel[2 * n + i]};
+  const double a[3] = {acc[i], acc[n + i], acc[2 * n + i]};
+  const double hdt = 0.5 * p.dt;
+  if (FINISH) {
+    if (LANGEVIN) {
+      double e[3];
+      gauss3(p.seed, id[i], p.step, e[0], e[1], e[2]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = v[d] * p.la + e[d] * p.lb + a[d] * 0.5 * p.dt;  // :860
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] += a[d] * hdt;
+    }
+  }
+  if (START) {
+    double4 x = pos[i];
+    if (LANGEVIN) {
+      double e[3];
+      gauss3(p.seed, id[i], p.step + 1, e[0], e[1], e[2]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] += a[d] * 0.5 * p.dt + e[d] * p.lb;  // :778
+      x.x += v[0] * p.lc;                                                   // :835
+      x.y += v[1] * p.lc;
+      x.z += v[2] * p.lc;
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] += a[d] * hdt;
+      x.x += v[0] * p.dt;
+      x.y += v[1] * p.dt;
+      x.z += v[2] * p.dt;
+    }
+    pos[i] = x;
+    if (WRITEQ)
+      q[i] = make_int4(qcoord(x.x, p.lo[0], p.qscale[0]), qcoord(x.y, p.lo[1], p.qscale[1]),
+                       qcoord(x.z, p.lo[2], p.qscale[2]), (int)x.w);
+    const double4 r = pos_ref[i];
+    const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
+    if (dx * dx + dy * dy + dz * dz > p.thresh_sq) atomicOr(&flags[F_REBUILD], 1);
+  }
+  vel[i] = v[0];
+  vel[n + i] = v[1];
+  vel[2 * n + i] = v[2];
+}
+
+// Displacement check alone (Neighborlist::update_is_needed), owned atoms
+// and host-halo ghosts.
+__global__ void __launch_bounds__(kBlock)
+    k_check_disp(const double4 *__restrict__ pos, const double4 *__restrict__ pos_ref, int ntot,
+                 double thresh_sq, int *__restrict__ flags) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ntot) return;
+  const double4 x = pos[i], r = pos_ref[i];
+  const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
+  if (dx * dx + dy * dy + dz * dz > thresh_sq) atomicOr(&flags[F_REBUILD], 1);
 }
 
 // Per-block sums of (m v.v, m vx, m vy, m vz, m).
@@ -187,32 +193,49 @@ __global__ void __launch_bounds__(1024) k_reduce5(const double *__restrict__ par
   }
 }
 
+template <bool L, bool F, bool S>
+void update_q(DevState &s, const StepParams &p, bool writeq, hipStream_t st) {
+  const int g = nblk(s.n, kBlock);
+  if (writeq)
+    k_update<L, F, S, true><<<g, kBlock, 0, st>>>(s.pos, s.pos_ref, s.vel, s.acc, s.id, s.q, s.n, p, s.flags);
+  else
+    k_update<L, F, S, false><<<g, kBlock, 0, st>>>(s.pos, s.pos_ref, s.vel, s.acc, s.id, s.q, s.n, p, s.flags);
+}
+
 }  // namespace
 
-void launch_kick_drift(DevState &s, const BoxDev &b, double dt, hipStream_t st) {
-  if (s.n == 0) return;
-  k_kick_drift<<<nblk(s.n, kBlock), kBlock, 0, st>>>(s.pos, s.pos_ref, s.vel, s.acc, s.n, dt,
-                                                      b.thresh_sq, s.flags);
+void launch_update(DevState &s, const BoxDev &b, const UpdateArgs &u, hipStream_t st) {
+  if (s.n == 0 || (!u.finish && !u.start)) return;
+  StepParams p{};
+  p.dt = u.dt;
+  p.la = u.la;
+  p.lb = u.lb;
+  p.lc = u.lc;
+  p.seed = u.seed;
+  p.step = u.step;
+  p.thresh_sq = b.thresh_sq;
+  for (int d = 0; d < 3; ++d) {
+    p.lo[d] = b.lo[d];
+    p.qscale[d] = b.qscale[d];
+  }
+#define CV2_UPD(L, F, S)                             \
+  if (u.langevin == L && u.finish == F && u.start == S) { \
+    update_q<L, F, S>(s, p, u.writeq, st);           \
+    return;                                          \
+  }
+  CV2_UPD(false, true, true)
+  CV2_UPD(false, true, false)
+  CV2_UPD(false, false, true)
+  CV2_UPD(true, true, true)
+  CV2_UPD(true, true, false)
+  CV2_UPD(true, false, true)
+#undef CV2_UPD
 }
 
-void launch_kick(DevState &s, double dt, hipStream_t st) {
-  if (s.n == 0) return;
-  k_kick<<<nblk(s.n, kBlock), kBlock, 0, st>>>(s.vel, s.acc, s.n, dt);
-}
-
-void launch_langevin_1(DevState &s, const BoxDev &b, const LJDev *, double dt, double lb,
-                       double lc, uint64_t seed, uint64_t step, hipStream_t st) {
-  if (s.n == 0) return;
-  k_langevin_1<<<nblk(s.n, kBlock), kBlock, 0, st>>>(s.pos, s.pos_ref, s.vel, s.acc, s.id, s.n,
-                                                      dt, lb, lc, seed, step, b.thresh_sq,
-                                                      s.flags);
-}
-
-void launch_langevin_2(DevState &s, const LJDev *, double dt, double la, double lb,
-                       uint64_t seed, uint64_t step, hipStream_t st) {
-  if (s.n == 0) return;
-  k_langevin_2<<<nblk(s.n, kBlock), kBlock, 0, st>>>(s.vel, s.acc, s.id, s.n, dt, la, lb, seed,
-                                                      step);
+void launch_check_disp(DevState &s, const BoxDev &b, hipStream_t st) {
+  const int ntot = s.n + s.ng;
+  if (ntot == 0) return;
+  k_check_disp<<<nblk(ntot, kBlock), kBlock, 0, st>>>(s.pos, s.pos_ref, ntot, b.thresh_sq, s.flags);
 }
 
 // Writes (sum m v^2, sum m v (3), sum m) to reduce_out[2..6].

@@ -28,6 +28,7 @@ namespace cv2 {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kUnroll = 4;
 // 1.5 * 2^52: adding it to a double of magnitude < 2^51 rounds to an integer
 // held in the low mantissa bits, so bits(t) - bits(kMagic) is that integer.
 constexpr double kMagic = 6755399441055744.0;
@@ -83,34 +84,50 @@ __global__ void __launch_bounds__(kBlock)
     const int cnt = nnbr[i];
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int id_i = (ENERGY && HALO) ? id[i] : 0;
-    for (int k = 0; k < cnt; ++k) {
-      const uint32_t e = nbr[(size_t)k * stride + i];
-      const int j = (int)(e & kIndexMask);
-      const double4 pj = pos[j];
-      double gx = pj.x, gy = pj.y, gz = pj.z;
-      const uint32_t img = e >> 26;
-      if (img != kNoImage) {
-        gx -= (double)((int)(img & 3u) - 1) * b.L[0];
-        gy -= (double)((int)((img >> 2) & 3u) - 1) * b.L[1];
-        gz -= (double)((int)(img >> 4) - 1) * b.L[2];
-      }
-      const double dx = gx - pi.x, dy = gy - pi.y, dz = gz - pi.z;
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      const int tp = MULTI ? ti * nt + (int)pj.w : 0;
-      if (r2 > lj->rc2[tp]) continue;
-      const double r2i = 1.0 / r2;
-      const double s2 = lj->sig2[tp] * r2i;
-      const double s6 = s2 * s2 * s2;
-      const double eps4 = lj->eps4[tp];
-      const double fc = eps4 * (r2i * (s6 * (-12.0 * s6 + 6.0)) + lj->fshift[tp]);
-      fx += fc * dx;
-      fy += fc * dy;
-      fz += fc * dz;
-      if (ENERGY) {
-        const bool once = HALO && j >= n ? (id_i < id[j]) : (i < j);
-        if (once) {
-          vir -= fc * r2;
-          pe += eps4 * ((s6 * s6 - s6) + lj->ecut[tp] * r2 - lj->ecut2[tp]);
+    // Batches of kUnroll neighbours: list entries (streamed past L2), then
+    // the gathers, then the arithmetic, so each wave keeps several loads in
+    // flight instead of two dependent round trips per neighbour.
+    for (int k = 0; k < cnt; k += kUnroll) {
+      uint32_t e[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u)
+        e[u] = (k + u < cnt) ? __builtin_nontemporal_load(&nbr[(size_t)(k + u) * stride + i])
+                             : ((uint32_t)i | (kNoImage << 26));
+      double4 pj[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) pj[u] = pos[e[u] & kIndexMask];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int j = (int)(e[u] & kIndexMask);
+        double gx = pj[u].x, gy = pj[u].y, gz = pj[u].z;
+        const uint32_t img = e[u] >> 26;
+        if (img != kNoImage) {
+          gx -= (double)((int)(img & 3u) - 1) * b.L[0];
+          gy -= (double)((int)((img >> 2) & 3u) - 1) * b.L[1];
+          gz -= (double)((int)(img >> 4) - 1) * b.L[2];
+        }
+        const double dx = gx - pi.x, dy = gy - pi.y, dz = gz - pi.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const int tp = MULTI ? ti * nt + (int)pj[u].w : 0;
+        if (k + u < cnt && r2 <= lj->rc2[tp]) {
+          // 1/r^2: hardware reciprocal and two Newton steps (about 1 ulp)
+          double r2i = __builtin_amdgcn_rcp(r2);
+          r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
+          r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
+          const double s2 = lj->sig2[tp] * r2i;
+          const double s6 = s2 * s2 * s2;
+          const double eps4 = lj->eps4[tp];
+          const double fc = eps4 * (r2i * (s6 * (-12.0 * s6 + 6.0)) + lj->fshift[tp]);
+          fx += fc * dx;
+          fy += fc * dy;
+          fz += fc * dz;
+          if (ENERGY) {
+            const bool once = HALO && j >= n ? (id_i < id[j]) : (i < j);
+            if (once) {
+              vir -= fc * r2;
+              pe += eps4 * ((s6 * s6 - s6) + lj->ecut[tp] * r2 - lj->ecut2[tp]);
+            }
+          }
         }
       }
     }
@@ -142,9 +159,8 @@ __global__ void __launch_bounds__(kBlock)
   const int blk = xcd_block(nblk_real);
   if (blk < 0) return;
   const int i = blk * kBlock + threadIdx.x;
-  // separation in units of 2^-32 length, then r^2 scaled back by 2^-64
-  const double qs0 = b.qinv[0] * kForceScale, qs1 = b.qinv[1] * kForceScale,
-               qs2 = b.qinv[2] * kForceScale;
+  // length of one fixed-point unit per axis
+  const double qu0 = b.qinv[0], qu1 = b.qinv[1], qu2 = b.qinv[2];
   long long vir = 0, pe = 0;
   if (i < n) {
     const int4 qi = q[i];
@@ -155,35 +171,51 @@ __global__ void __launch_bounds__(kBlock)
     int terms = 0;
     bool ovf = false;
     const int id_i = (ENERGY && HALO) ? id[i] : 0;
-    for (int k = 0; k < cnt; ++k) {
-      const uint32_t e = nbr[(size_t)k * stride + i];
-      const int j = (int)(e & kIndexMask);
-      const int4 qj = q[j];
-      const double dxs = (double)(qj.x - qi.x) * qs0;
-      const double dys = (double)(qj.y - qi.y) * qs1;
-      const double dzs = (double)(qj.z - qi.z) * qs2;
-      const float r2 = (float)((dxs * dxs + dys * dys + dzs * dzs) * 5.421010862427522e-20);
-      const int tp = MULTI ? ti * nt + qj.w : 0;
-      if (r2 > lj->rc2f[tp]) continue;
-      const float sig2 = lj->sig2f[tp];
-      ovf |= r2 < 0.25f * sig2;
-      const float r2i = __builtin_amdgcn_rcpf(r2);
-      const float s2 = sig2 * r2i;
-      const float s6 = s2 * s2 * s2;
-      const float eps4 = lj->eps4f[tp];
-      const float fc = eps4 * (r2i * (s6 * (-12.0f * s6 + 6.0f)) + lj->fshiftf[tp]);
-      const double fcd = (double)fc;
-      sx += (unsigned long long)__double_as_longlong(fma(fcd, dxs, kMagic));
-      sy += (unsigned long long)__double_as_longlong(fma(fcd, dys, kMagic));
-      sz += (unsigned long long)__double_as_longlong(fma(fcd, dzs, kMagic));
-      ++terms;
-      if (ENERGY) {
-        const bool once = HALO && j >= n ? (id_i < id[j]) : (i < j);
-        if (once) {
-          const double r2d = (double)r2;
-          vir += __double_as_longlong(fma(-fcd * r2d, kEnergyScale, kMagic)) - kMagicBits;
-          const float u = eps4 * ((s6 * s6 - s6) + (float)lj->ecut[tp] * r2 - (float)lj->ecut2[tp]);
-          pe += __double_as_longlong(fma((double)u, kEnergyScale, kMagic)) - kMagicBits;
+    for (int k = 0; k < cnt; k += kUnroll) {
+      uint32_t e[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u)
+        e[u] = (k + u < cnt) ? __builtin_nontemporal_load(&nbr[(size_t)(k + u) * stride + i])
+                             : (uint32_t)i;
+      int4 qj[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) qj[u] = q[e[u] & kIndexMask];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int j = (int)(e[u] & kIndexMask);
+        // exact integer separation -> FP64 (one rounding by the unit scale)
+        const double dx = (double)(qj[u].x - qi.x) * qu0;
+        const double dy = (double)(qj[u].y - qi.y) * qu1;
+        const double dz = (double)(qj[u].z - qi.z) * qu2;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const int tp = MULTI ? ti * nt + qj[u].w : 0;
+        if (k + u < cnt && r2 <= lj->rc2[tp]) {
+          // r^-2 and (sigma^2/r^2)^3 in FP64, each rounded once to FP32;
+          // the force polynomial in FP32
+          double r2i = __builtin_amdgcn_rcp(r2);
+          r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
+          const double s2 = lj->sig2[tp] * r2i;
+          const float s6 = (float)(s2 * s2 * s2);
+          const float r2if = (float)r2i;
+          const float eps4 = lj->eps4f[tp];
+          const float fc = eps4 * (r2if * (s6 * (-12.0f * s6 + 6.0f)) + lj->fshiftf[tp]);
+          // F * 2^32 rounded to an integer inside one FP64 FMA; valid while
+          // |F| < 2^19 on every component (|fc| r < 2^19)
+          const double fcd = (double)fc;
+          ovf |= fcd * fcd * r2 > 274877906944.0;
+          const double fcs = fcd * kForceScale;
+          sx += (unsigned long long)__double_as_longlong(fma(fcs, dx, kMagic));
+          sy += (unsigned long long)__double_as_longlong(fma(fcs, dy, kMagic));
+          sz += (unsigned long long)__double_as_longlong(fma(fcs, dz, kMagic));
+          ++terms;
+          if (ENERGY) {
+            const bool once = HALO && j >= n ? (id_i < id[j]) : (i < j);
+            if (once) {
+              vir +=__double_as_longlong(fma(-fcd * r2, kEnergyScale, kMagic)) - kMagicBits;
+              const float u = eps4 * ((s6 * s6 - s6) + (float)(lj->ecut[tp] * r2) - (float)lj->ecut2[tp]);
+              pe += __double_as_longlong(fma((double)u, kEnergyScale, kMagic)) - kMagicBits;
+            }
+          }
         }
       }
     }
@@ -252,7 +284,6 @@ void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDe
   if (s.n == 0) return;
   const bool multi = lj.nt > 1;
   const bool halo = s.ng > 0;
-  if (precision != 0) launch_pack_q(s, b, st);
 #define CV2_LJ(M, E, H)                                      \
   if (multi == M && energy == E && halo == H) {              \
     lj_dispatch<M, E, H>(s, b, lj_dev, precision, st);       \
