@@ -151,7 +151,8 @@ int setup_box(cv2_ctx *ctx) {
     const double margin = b.bc[d] ? 0.0 : ctx->ghost_cut;
     b.glo[d] = b.lo[d] - margin;
     const double ext = b.L[d] + 2.0 * margin;
-    int nc = (int)std::floor(ext / (ce * (1.0 + 1e-12)));
+    // cells at least half the list cutoff wide (the build scans +-2 cells)
+    int nc = (int)std::floor(ext / (0.5 * ce * (1.0 + 1e-12)));
     nc = std::max(1, std::min(nc, 1024));
     b.nc[d] = nc;
     b.cinv[d] = nc / ext;
@@ -348,6 +349,21 @@ int cv2_set_lj(cv2_ctx *ctx, int ntypes, const double *eps, const double *sigma,
     L.sig2f[k] = (float)L.sig2[k];
     L.rc2f[k] = (float)L.rc2[k];
     L.fshiftf[k] = (float)L.fshift[k];
+    // FP32 path: largest r^2 at which |F(r)| = |forceCoef| r reaches 0.9 * 2^19
+    // (bisection on the repulsive branch, where |F| falls with r)
+    L.r2ovf[k] = 0.0;
+    if (eps[k] != 0.0 && sigma[k] > 0.0) {
+      auto force = [&](double r) {
+        const double r2 = r * r, r2i = 1.0 / r2, s6 = std::pow(s2 * r2i, 3);
+        return std::fabs(L.eps4[k] * (r2i * (s6 * (-12.0 * s6 + 6.0)) + L.fshift[k])) * r;
+      };
+      double a = 1e-3 * sigma[k], z = sigma[k];
+      for (int it = 0; it < 200; ++it) {
+        const double m = 0.5 * (a + z);
+        (force(m) > 0.9 * 524288.0 ? a : z) = m;
+      }
+      L.r2ovf[k] = z * z;
+    }
   }
   ctx->lj_set = true;
   ctx->need_build = true;
@@ -428,7 +444,7 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
   int cap = (int)std::ceil(1.6 * expect + 64.0);
   cap = (cap + 15) / 16 * 16;
   s.cap = cap;
-  s.stride = (int)((n + 63) / 64 * 64);
+  s.stride = (int)((n + 63) / 64 * 64);  // atoms per 4-entry group row
   HIPCHK(dalloc(&s.nbr, (size_t)s.cap * s.stride));
   const int nb = (int)((n + 255) / 256) + 8;
   if (nb > ctx->partial_blocks) {
@@ -659,7 +675,7 @@ int cv2_get_pairs(cv2_ctx *ctx, int64_t cap, int64_t *npairs, int64_t *pairs) {
   int64_t k = 0;
   for (int64_t i = 0; i < n; ++i) {
     for (int m = 0; m < cnt[i]; ++m) {
-      const uint32_t e = nb[(size_t)m * s.stride + i];
+      const uint32_t e = nb[((size_t)(m >> 2) * s.stride + i) * 4 + (m & 3)];
       const int64_t j = e & kIndexMask;
       const uint32_t img = e >> 26;
       const int code = (int)(img & 3u) + 3 * (int)((img >> 2) & 3u) + 9 * (int)(img >> 4);

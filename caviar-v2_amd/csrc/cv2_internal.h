@@ -63,12 +63,44 @@ struct LJDev {
   // FP32 copies for the mixed-precision kernel
   float eps4f[kMaxTypes * kMaxTypes], sig2f[kMaxTypes * kMaxTypes];
   float rc2f[kMaxTypes * kMaxTypes], fshiftf[kMaxTypes * kMaxTypes];
+  double r2ovf[kMaxTypes * kMaxTypes];  // r^2 below which |F| may exceed 2^19
 };
 
 // Fixed-point scales of the FP32 path: forces (and accelerations) in units of
 // 2^-32, energies and virials in units of 2^-24.
 constexpr double kForceScale = 4294967296.0;
 constexpr double kEnergyScale = 16777216.0;
+
+// Fixed-point coordinates of the FP32 pair kernel.  Each axis is
+// (x - lo) * 2^32 / L rounded to an integer and kept modulo 2^32 (int32
+// differences are then minimum-image separations on periodic axes), plus a
+// 9-bit signed residual in units of 2^-9 of that quantum: the separation is
+// ((dq * 512) + dres) * L / 2^41, i.e. positions resolved to L / 2^41
+// (6e-11 at L = 134) with 16-byte gathers.  w = type (bits 0-2) |
+// res_x (bits 5-13) | res_y (bits 14-22) | res_z (bits 23-31).
+constexpr double kResidScale = 512.0;
+
+// Signed 9-bit residual of axis d from the packed word (one v_bfe_i32).
+__device__ inline int q_resid(int w, int d) {
+  return ((int)((unsigned)w << (18 - 9 * d))) >> 23;
+}
+
+__device__ inline int4 pack_q(double x, double y, double z, int type, const double lo[3],
+                              const double scale[3]) {
+  const double c[3] = {x, y, z};
+  int q[3];
+  unsigned w = (unsigned)type & 7u;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double f = (c[d] - lo[d]) * scale[d];
+    const long long qi = llrint(f);
+    int r = (int)llrint((f - (double)qi) * kResidScale);  // in [-256, 256]
+    r = r > 255 ? 255 : r;
+    q[d] = (int)(unsigned)(unsigned long long)qi;
+    w |= ((unsigned)r & 511u) << (5 + 9 * d);
+  }
+  return make_int4(q[0], q[1], q[2], (int)w);
+}
 
 struct DevState {
   int n;        // owned atoms

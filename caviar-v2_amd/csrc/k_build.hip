@@ -187,12 +187,7 @@ __global__ void k_permute(const int *__restrict__ perm, int n, const double4 *__
 // reduced modulo 2^32, so that int32 differences are minimum-image
 // separations on periodic axes.
 __device__ inline int4 qpack(const double4 &p, const BoxDev &b) {
-  int4 r;
-  r.x = (int)(unsigned)(unsigned long long)llrint((p.x - b.lo[0]) * b.qscale[0]);
-  r.y = (int)(unsigned)(unsigned long long)llrint((p.y - b.lo[1]) * b.qscale[1]);
-  r.z = (int)(unsigned)(unsigned long long)llrint((p.z - b.lo[2]) * b.qscale[2]);
-  r.w = (int)p.w;
-  return r;
+  return pack_q(p.x, p.y, p.z, (int)p.w, b.lo, b.qscale);
 }
 
 __global__ void k_copy_back(int n, const double4 *__restrict__ pos_alt, double4 *__restrict__ pos,
@@ -260,19 +255,30 @@ __global__ void k_ghost_sort_cells(int *__restrict__ items, const int *__restric
 
 // Candidate cells along one axis: c-1, c, c+1 (periodic wrap, or clamped
 // for open axes), duplicates removed when the grid has fewer than 3 cells.
-__device__ inline int axis_stencil(int c, int nc, int periodic, int out[3]) {
-  if (nc <= 3 && periodic) {
-    for (int k = 0; k < nc; ++k) out[k] = k;
+// Candidate cells along one axis: c-2 .. c+2 (cells are at least half the
+// list cutoff wide), wrapped on periodic axes and clipped on open ones;
+// every cell once, with offset 0 (no pruning), when a periodic axis has
+// fewer than 5 cells.
+constexpr int kReach = 2;
+constexpr int kSpan = 2 * kReach + 1;
+
+__device__ inline int axis_stencil(int c, int nc, int periodic, int out[kSpan], int off[kSpan]) {
+  if (periodic && nc < kSpan) {
+    for (int k = 0; k < nc; ++k) {
+      out[k] = k;
+      off[k] = 0;
+    }
     return nc;
   }
   int m = 0;
-  for (int d = -1; d <= 1; ++d) {
+  for (int d = -kReach; d <= kReach; ++d) {
     int v = c + d;
     if (periodic) {
       v = v < 0 ? v + nc : (v >= nc ? v - nc : v);
     } else if (v < 0 || v >= nc) {
       continue;
     }
+    off[m] = d;
     out[m++] = v;
   }
   return m;
@@ -300,16 +306,86 @@ __device__ inline bool axis_image(double pi, double pj, double lo, double hi, do
 
 // Verlet list: one thread per owned atom, full list (both directions).
 //
+// List layout (shared with the force kernels): entries in groups of four,
+// nbr[((k / 4) * stride + i) * 4 + k % 4], so that a lane reads its next four
+// entries as one 16-byte load and a wave's loads are contiguous.  Each lane
+// collects four entries in registers before storing a group; the last group
+// is padded with the atom itself (r = 0, skipped by the force kernels).
+//
 // Stencil cells whose box lies farther than the list cutoff from the atom
 // are skipped (with a relative margin of 1e-9 so that rounding in the cell
-// assignment can never drop a neighbour).  Atoms whose cell touches no
-// periodic face take a fast path without image logic: for them every
-// candidate's minimum image is the direct one.
+// assignment can never drop a neighbour).  Cells are numbered x-fastest, so
+// the surviving cells of one (y, z) stencil row that are adjacent in x form
+// one contiguous range of sorted atoms, read kBatch candidates at a time.
+// Atoms whose stencil does not wrap a periodic face take a fast path without
+// image logic: for them every candidate's minimum image is the direct one.
 __device__ inline bool list_pair(double px, double py, double pz, double gx, double gy,
                                  double gz, double ce_sq) {
   const double dx = __dsub_rn(gx, px), dy = __dsub_rn(gy, py), dz = __dsub_rn(gz, pz);
   const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
   return r2 < ce_sq;
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+struct ListWriter {
+  uint32_t *nbr;
+  size_t stride;  // atoms per group row
+  int i, cap, cnt;
+  u32x4 g;
+  __device__ inline void add(uint32_t e) {
+    if (cnt < cap) {
+      switch (cnt & 3) {
+        case 0: g.x = e; break;
+        case 1: g.y = e; break;
+        case 2: g.z = e; break;
+        default:
+          g.w = e;
+          reinterpret_cast<u32x4 *>(nbr)[(size_t)(cnt >> 2) * stride + i] = g;
+      }
+    }
+    ++cnt;
+  }
+  __device__ inline void finish() {
+    const int kept = cnt < cap ? cnt : cap;
+    if (kept & 3) {
+      const uint32_t self = (uint32_t)i | (kNoImage << 26);
+      if ((kept & 3) < 2) g.y = self;
+      if ((kept & 3) < 3) g.z = self;
+      g.w = self;
+      reinterpret_cast<u32x4 *>(nbr)[(size_t)(kept >> 2) * stride + i] = g;
+    }
+  }
+};
+
+constexpr int kBatch = 8;
+
+template <bool IMAGES>
+__device__ inline void scan_range(const double4 *__restrict__ pos, int j0, int j1, int i,
+                                  const double4 &pi, const BoxDev &b, ListWriter &out) {
+  for (int j = j0; j < j1; j += kBatch) {
+    double4 pj[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) pj[u] = pos[min(j + u, j1 - 1)];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int jj = j + u;
+      if (jj >= j1 || jj == i) continue;
+      if (IMAGES) {
+        int vx, vy, vz;
+        double gx, gy, gz;
+        if (!axis_image(pi.x, pj[u].x, b.lo[0], b.hi[0], b.L[0], b.halfL[0], b.ghost_cut, b.bc[0], vx, gx)) continue;
+        if (!axis_image(pi.y, pj[u].y, b.lo[1], b.hi[1], b.L[1], b.halfL[1], b.ghost_cut, b.bc[1], vy, gy)) continue;
+        if (!axis_image(pi.z, pj[u].z, b.lo[2], b.hi[2], b.L[2], b.halfL[2], b.ghost_cut, b.bc[2], vz, gz)) continue;
+        if (!list_pair(pi.x, pi.y, pi.z, gx, gy, gz, b.ce_sq)) continue;
+        out.add((uint32_t)jj | ((uint32_t)(vx + 1) << 26) | ((uint32_t)(vy + 1) << 28) |
+                ((uint32_t)(vz + 1) << 30));
+      } else {
+        if (!list_pair(pi.x, pi.y, pi.z, pj[u].x, pj[u].y, pj[u].z, b.ce_sq)) continue;
+        out.add((uint32_t)jj | (kNoImage << 26));
+      }
+    }
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -322,75 +398,68 @@ __global__ void __launch_bounds__(256)
   if (i >= n) return;
   const double4 pi = pos[i];
   const double pc[3] = {pi.x, pi.y, pi.z};
-  int cc[3], sten[3][3], nst[3];
-  double gap[3][3];  // distance from the atom to stencil cell k along axis d
+  int cc[3], sten[3][kSpan], nst[3];
+  double gap[3][kSpan];  // distance from the atom to stencil cell k along axis d
   bool interior = true;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     cc[d] = axis_cell(pc[d], b.glo[d], b.cinv[d], b.nc[d]);
-    nst[d] = axis_stencil(cc[d], b.nc[d], b.bc[d], sten[d]);
+    int soff[kSpan];
+    nst[d] = axis_stencil(cc[d], b.nc[d], b.bc[d], sten[d], soff);
     const double w = 1.0 / b.cinv[d];
     const double off = pc[d] - (b.glo[d] + cc[d] * w);  // position inside the cell
-    const bool wrapped = b.bc[d] && b.nc[d] <= 3;       // all cells, offsets ambiguous
     for (int k = 0; k < nst[d]; ++k) {
-      const int dc = sten[d][k] - cc[d];
-      gap[d][k] = wrapped ? 0.0 : (dc < 0 ? off : (dc > 0 ? w - off : 0.0));
-      if (gap[d][k] < 0.0) gap[d][k] = 0.0;
+      const int o = soff[k];
+      double g = o < 0 ? (-o - 1) * w + off : (o > 0 ? (o - 1) * w + (w - off) : 0.0);
+      gap[d][k] = g < 0.0 ? 0.0 : g;
     }
-    if (b.bc[d] && (cc[d] == 0 || cc[d] == b.nc[d] - 1)) interior = false;
+    if (b.bc[d] && (cc[d] < kReach || cc[d] > b.nc[d] - 1 - kReach)) interior = false;
   }
   const double prune = b.ce_sq * (1.0 + 1e-9);
-  int cnt = 0;
+  ListWriter out{nbr, (size_t)stride, i, cap, 0, u32x4{0u, 0u, 0u, 0u}};
   for (int a = 0; a < nst[2]; ++a)
     for (int bb = 0; bb < nst[1]; ++bb) {
       const double gyz = gap[2][a] * gap[2][a] + gap[1][bb] * gap[1][bb];
       if (gyz > prune) continue;
-      for (int c = 0; c < nst[0]; ++c) {
-        if (gyz + gap[0][c] * gap[0][c] > prune) continue;
-        const int cell = sten[0][c] + b.nc[0] * (sten[1][bb] + b.nc[1] * sten[2][a]);
-        const int j1 = cstart[cell + 1];
-        if (interior) {
-          for (int j = cstart[cell]; j < j1; ++j) {
-            const double4 pj = pos[j];
-            if (j != i && list_pair(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, b.ce_sq)) {
-              if (cnt < cap) nbr[(size_t)cnt * stride + i] = (uint32_t)j | (kNoImage << 26);
-              ++cnt;
-            }
-          }
-        } else {
-          for (int j = cstart[cell]; j < j1; ++j) {
-            if (j == i) continue;
-            const double4 pj = pos[j];
-            int vx, vy, vz;
-            double gx, gy, gz;
-            if (!axis_image(pi.x, pj.x, b.lo[0], b.hi[0], b.L[0], b.halfL[0], b.ghost_cut, b.bc[0], vx, gx)) continue;
-            if (!axis_image(pi.y, pj.y, b.lo[1], b.hi[1], b.L[1], b.halfL[1], b.ghost_cut, b.bc[1], vy, gy)) continue;
-            if (!axis_image(pi.z, pj.z, b.lo[2], b.hi[2], b.L[2], b.halfL[2], b.ghost_cut, b.bc[2], vz, gz)) continue;
-            if (list_pair(pi.x, pi.y, pi.z, gx, gy, gz, b.ce_sq)) {
-              if (cnt < cap)
-                nbr[(size_t)cnt * stride + i] = (uint32_t)j | ((uint32_t)(vx + 1) << 26) |
-                                                ((uint32_t)(vy + 1) << 28) | ((uint32_t)(vz + 1) << 30);
-              ++cnt;
-            }
-          }
+      const int row = b.nc[0] * (sten[1][bb] + b.nc[1] * sten[2][a]);
+      // merge the surviving x cells of this row into contiguous ranges
+      int c = 0;
+      while (c < nst[0]) {
+        if (gyz + gap[0][c] * gap[0][c] > prune) {
+          ++c;
+          continue;
         }
+        int x0 = sten[0][c], x1 = x0;
+        int c2 = c + 1;
+        while (c2 < nst[0] && sten[0][c2] == x1 + 1 && gyz + gap[0][c2] * gap[0][c2] <= prune) {
+          ++x1;
+          ++c2;
+        }
+        const int j0 = cstart[row + x0], j1 = cstart[row + x1 + 1];
+        if (interior)
+          scan_range<false>(pos, j0, j1, i, pi, b, out);
+        else
+          scan_range<true>(pos, j0, j1, i, pi, b, out);
         if (host_halo) {
-          const int g1 = gstart[cell + 1];
-          for (int t = gstart[cell]; t < g1; ++t) {
-            const int j = n + gitems[t];
-            const double4 pj = pos[j];
-            if (list_pair(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, b.ce_sq)) {
-              if (cnt < cap) nbr[(size_t)cnt * stride + i] = (uint32_t)j | (kNoImage << 26);
-              ++cnt;
+          for (int x = x0; x <= x1; ++x) {
+            const int cell = row + x;
+            const int g1 = gstart[cell + 1];
+            for (int t = gstart[cell]; t < g1; ++t) {
+              const int j = n + gitems[t];
+              const double4 pj = pos[j];
+              if (list_pair(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, b.ce_sq)) out.add((uint32_t)j | (kNoImage << 26));
             }
           }
         }
+        c = c2;
       }
     }
-  nnbr[i] = cnt < cap ? cnt : cap;
-  if (cnt > cap) atomicOr(&flags[F_OVERFLOW], 1);
-  atomicMax(&flags[F_MAXNBR], cnt);
-  atomicAdd(&flags[F_ENTRIES], cnt < cap ? cnt : cap);
+  out.finish();
+  const int kept = out.cnt < cap ? out.cnt : cap;
+  nnbr[i] = kept;
+  if (out.cnt > cap) atomicOr(&flags[F_OVERFLOW], 1);
+  atomicMax(&flags[F_MAXNBR], out.cnt);
+  atomicAdd(&flags[F_ENTRIES], kept);
 }
 
 __global__ void k_build_done(int *__restrict__ flags) {

@@ -66,10 +66,6 @@ __device__ inline void gauss3(uint64_t seed, int id, uint64_t step, double &g0, 
   (void)sb;
 }
 
-__device__ inline int qcoord(double x, double lo, double scale) {
-  return (int)(unsigned)(unsigned long long)llrint((x - lo) * scale);
-}
-
 struct StepParams {
   double dt;
   double la, lb, lc;  // Langevin a, b, c (md_simulator.cpp:398-405)
@@ -121,9 +117,7 @@ __global__ void __launch_bounds__(kBlock)
       x.z += v[2] * p.dt;
     }
     pos[i] = x;
-    if (WRITEQ)
-      q[i] = make_int4(qcoord(x.x, p.lo[0], p.qscale[0]), qcoord(x.y, p.lo[1], p.qscale[1]),
-                       qcoord(x.z, p.lo[2], p.qscale[2]), (int)x.w);
+    if (WRITEQ) q[i] = pack_q(x.x, x.y, x.z, (int)x.w, p.lo, p.qscale);
     const double4 r = pos_ref[i];
     const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
     if (dx * dx + dy * dy + dz * dz > p.thresh_sq) atomicOr(&flags[F_REBUILD], 1);

@@ -28,7 +28,6 @@ namespace cv2 {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kUnroll = 4;
 // 1.5 * 2^52: adding it to a double of magnitude < 2^51 rounds to an integer
 // held in the low mantissa bits, so bits(t) - bits(kMagic) is that integer.
 constexpr double kMagic = 6755399441055744.0;
@@ -66,6 +65,16 @@ __device__ inline long long block_sum_i64(long long v, long long *sh) {
   return r;
 }
 
+// The list is stored in groups of four entries per atom,
+// nbr[((k / 4) * stride + i) * 4 + k % 4], padded with the atom itself (r = 0,
+// skipped by the r^2 > 0 test): a lane reads its next four entries as one
+// 16-byte load and the loads of a wave are contiguous.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline u32x4 load_group(const uint32_t *__restrict__ nbr, size_t stride, int i, int k) {
+  return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(nbr) + (size_t)(k >> 2) * stride + i);
+}
+
 template <bool MULTI, bool ENERGY, bool HALO>
 __global__ void __launch_bounds__(kBlock)
     k_lj_f64(const double4 *__restrict__ pos, const uint32_t *__restrict__ nbr,
@@ -81,23 +90,19 @@ __global__ void __launch_bounds__(kBlock)
     const double4 pi = pos[i];
     const int ti = MULTI ? (int)pi.w : 0;
     const int nt = MULTI ? lj->nt : 1;
-    const int cnt = nnbr[i];
+    // single-type parameters stay in registers for the whole loop
+    const double rc2_0 = lj->rc2[0], sig2_0 = lj->sig2[0], eps4_0 = lj->eps4[0], fsh_0 = lj->fshift[0];
+    const int cp = (nnbr[i] + 3) & ~3;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int id_i = (ENERGY && HALO) ? id[i] : 0;
-    // Batches of kUnroll neighbours: list entries (streamed past L2), then
-    // the gathers, then the arithmetic, so each wave keeps several loads in
-    // flight instead of two dependent round trips per neighbour.
-    for (int k = 0; k < cnt; k += kUnroll) {
-      uint32_t e[kUnroll];
+    for (int k = 0; k < cp; k += 4) {
+      const u32x4 g = load_group(nbr, (size_t)stride, i, k);
+      const uint32_t e[4] = {g.x, g.y, g.z, g.w};
+      double4 pj[4];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u)
-        e[u] = (k + u < cnt) ? __builtin_nontemporal_load(&nbr[(size_t)(k + u) * stride + i])
-                             : ((uint32_t)i | (kNoImage << 26));
-      double4 pj[kUnroll];
+      for (int u = 0; u < 4; ++u) pj[u] = pos[e[u] & kIndexMask];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) pj[u] = pos[e[u] & kIndexMask];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
+      for (int u = 0; u < 4; ++u) {
         const int j = (int)(e[u] & kIndexMask);
         double gx = pj[u].x, gy = pj[u].y, gz = pj[u].z;
         const uint32_t img = e[u] >> 26;
@@ -109,15 +114,16 @@ __global__ void __launch_bounds__(kBlock)
         const double dx = gx - pi.x, dy = gy - pi.y, dz = gz - pi.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         const int tp = MULTI ? ti * nt + (int)pj[u].w : 0;
-        if (k + u < cnt && r2 <= lj->rc2[tp]) {
+        const double rc2 = MULTI ? lj->rc2[tp] : rc2_0;
+        if (r2 <= rc2 && r2 > 0.0) {
           // 1/r^2: hardware reciprocal and two Newton steps (about 1 ulp)
           double r2i = __builtin_amdgcn_rcp(r2);
           r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
           r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
-          const double s2 = lj->sig2[tp] * r2i;
+          const double s2 = (MULTI ? lj->sig2[tp] : sig2_0) * r2i;
           const double s6 = s2 * s2 * s2;
-          const double eps4 = lj->eps4[tp];
-          const double fc = eps4 * (r2i * (s6 * (-12.0 * s6 + 6.0)) + lj->fshift[tp]);
+          const double eps4 = MULTI ? lj->eps4[tp] : eps4_0;
+          const double fc = eps4 * (r2i * (s6 * (-12.0 * s6 + 6.0)) + (MULTI ? lj->fshift[tp] : fsh_0));
           fx += fc * dx;
           fy += fc * dy;
           fz += fc * dz;
@@ -159,50 +165,54 @@ __global__ void __launch_bounds__(kBlock)
   const int blk = xcd_block(nblk_real);
   if (blk < 0) return;
   const int i = blk * kBlock + threadIdx.x;
-  // length of one fixed-point unit per axis
-  const double qu0 = b.qinv[0], qu1 = b.qinv[1], qu2 = b.qinv[2];
+  // length of one residual unit (2^-9 of the fixed-point quantum) per axis
+  const double qu0 = b.qinv[0] / kResidScale, qu1 = b.qinv[1] / kResidScale,
+               qu2 = b.qinv[2] / kResidScale;
   long long vir = 0, pe = 0;
   if (i < n) {
     const int4 qi = q[i];
-    const int ti = MULTI ? qi.w : 0;
+    const int ti = MULTI ? (qi.w & 7) : 0;
     const int nt = MULTI ? lj->nt : 1;
-    const int cnt = nnbr[i];
+    const double rc2_0 = lj->rc2[0], sig2_0 = lj->sig2[0], r2ovf_0 = lj->r2ovf[0];
+    const float eps4_0 = lj->eps4f[0], fsh_0 = lj->fshiftf[0];
+    const int rix = q_resid(qi.w, 0), riy = q_resid(qi.w, 1), riz = q_resid(qi.w, 2);
+    const int cp = (nnbr[i] + 3) & ~3;
     unsigned long long sx = 0, sy = 0, sz = 0;
     int terms = 0;
     bool ovf = false;
     const int id_i = (ENERGY && HALO) ? id[i] : 0;
-    for (int k = 0; k < cnt; k += kUnroll) {
-      uint32_t e[kUnroll];
+    for (int k = 0; k < cp; k += 4) {
+      const u32x4 g = load_group(nbr, (size_t)stride, i, k);
+      const uint32_t e[4] = {g.x, g.y, g.z, g.w};
+      int4 qj[4];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u)
-        e[u] = (k + u < cnt) ? __builtin_nontemporal_load(&nbr[(size_t)(k + u) * stride + i])
-                             : (uint32_t)i;
-      int4 qj[kUnroll];
+      for (int u = 0; u < 4; ++u) qj[u] = q[e[u] & kIndexMask];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) qj[u] = q[e[u] & kIndexMask];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
+      for (int u = 0; u < 4; ++u) {
         const int j = (int)(e[u] & kIndexMask);
-        // exact integer separation -> FP64 (one rounding by the unit scale)
-        const double dx = (double)(qj[u].x - qi.x) * qu0;
-        const double dy = (double)(qj[u].y - qi.y) * qu1;
-        const double dz = (double)(qj[u].z - qi.z) * qu2;
+        // exact separation in residual units: quanta * 512 + residual difference
+        const double dx = fma((double)(qj[u].x - qi.x), kResidScale,
+                              (double)(q_resid(qj[u].w, 0) - rix)) * qu0;
+        const double dy = fma((double)(qj[u].y - qi.y), kResidScale,
+                              (double)(q_resid(qj[u].w, 1) - riy)) * qu1;
+        const double dz = fma((double)(qj[u].z - qi.z), kResidScale,
+                              (double)(q_resid(qj[u].w, 2) - riz)) * qu2;
         const double r2 = dx * dx + dy * dy + dz * dz;
-        const int tp = MULTI ? ti * nt + qj[u].w : 0;
-        if (k + u < cnt && r2 <= lj->rc2[tp]) {
+        const int tp = MULTI ? ti * nt + (qj[u].w & 7) : 0;
+        if (r2 <= (MULTI ? lj->rc2[tp] : rc2_0) && r2 > 0.0) {
           // r^-2 and (sigma^2/r^2)^3 in FP64, each rounded once to FP32;
           // the force polynomial in FP32
           double r2i = __builtin_amdgcn_rcp(r2);
           r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
-          const double s2 = lj->sig2[tp] * r2i;
+          const double s2 = (MULTI ? lj->sig2[tp] : sig2_0) * r2i;
           const float s6 = (float)(s2 * s2 * s2);
           const float r2if = (float)r2i;
-          const float eps4 = lj->eps4f[tp];
-          const float fc = eps4 * (r2if * (s6 * (-12.0f * s6 + 6.0f)) + lj->fshiftf[tp]);
-          // F * 2^32 rounded to an integer inside one FP64 FMA; valid while
-          // |F| < 2^19 on every component (|fc| r < 2^19)
+          const float eps4 = MULTI ? lj->eps4f[tp] : eps4_0;
+          const float fc = eps4 * (r2if * (s6 * (-12.0f * s6 + 6.0f)) + (MULTI ? lj->fshiftf[tp] : fsh_0));
+          // |F| = |fc| r stays below 2^19 for r^2 >= r2ovf (fixed-point range)
+          ovf |= r2 < (MULTI ? lj->r2ovf[tp] : r2ovf_0);
+          // F * 2^32 rounded to an integer inside one FP64 FMA
           const double fcd = (double)fc;
-          ovf |= fcd * fcd * r2 > 274877906944.0;
           const double fcs = fcd * kForceScale;
           sx += (unsigned long long)__double_as_longlong(fma(fcs, dx, kMagic));
           sy += (unsigned long long)__double_as_longlong(fma(fcs, dy, kMagic));
@@ -211,7 +221,7 @@ __global__ void __launch_bounds__(kBlock)
           if (ENERGY) {
             const bool once = HALO && j >= n ? (id_i < id[j]) : (i < j);
             if (once) {
-              vir +=__double_as_longlong(fma(-fcd * r2, kEnergyScale, kMagic)) - kMagicBits;
+              vir += __double_as_longlong(fma(-fcd * r2, kEnergyScale, kMagic)) - kMagicBits;
               const float u = eps4 * ((s6 * s6 - s6) + (float)(lj->ecut[tp] * r2) - (float)lj->ecut2[tp]);
               pe += __double_as_longlong(fma((double)u, kEnergyScale, kMagic)) - kMagicBits;
             }
