@@ -125,12 +125,24 @@ hipError_t dalloc(T **p, size_t count) {
 
 void free_state(DevState &s) {
   void *ptrs[] = {s.pos, s.pos_alt, s.pos_ref, s.q, s.vel, s.vel_alt, s.acc, s.acc_alt, s.tag,
-                  s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.cell_count, s.cell_start,
+                  s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.order, s.cpos, s.box_dev, s.wstat, s.cell_count, s.cell_start,
                   s.gcell_count, s.gcell_start, s.gcell_items, s.tile_sum, s.nbr, s.nnbr, s.flags,
                   s.partials, s.fx_acc, s.reduce_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   s = DevState{};
+}
+
+// Per-wavefront statistics of the list build: one entry per cell (wave-
+// per-cell build) or per wavefront of owned atoms (per-atom build).
+int ensure_wstat(cv2_ctx *ctx) {
+  DevState &s = ctx->s;
+  const int64_t need = std::max<int64_t>((int64_t)s.ncell, (ctx->n + 255) / 256 * 4) + 8;
+  if (need > s.wstat_cap) {
+    HIPCHK(dalloc(&s.wstat, need));
+    s.wstat_cap = need;
+  }
+  return CV2_OK;
 }
 
 // Cell grid and box constants; requires box, neighbour settings.
@@ -141,28 +153,67 @@ int setup_box(cv2_ctx *ctx) {
   b.ce_sq = ce * ce;
   b.thresh_sq = 0.25 * (ce - ctx->nl_cutoff) * (ce - ctx->nl_cutoff);  // :318
   b.ghost_cut = ctx->ghost_cut;
-  int64_t ncell = 1;
   for (int d = 0; d < 3; ++d) {
     b.lo[d] = ctx->lo[d];
     b.hi[d] = ctx->hi[d];
     b.L[d] = ctx->hi[d] - ctx->lo[d];
     b.halfL[d] = 0.5 * b.L[d];
     b.bc[d] = ctx->host_halo ? 0 : ctx->bc[d];
-    const double margin = b.bc[d] ? 0.0 : ctx->ghost_cut;
-    b.glo[d] = b.lo[d] - margin;
-    const double ext = b.L[d] + 2.0 * margin;
-    // cells at least half the list cutoff wide (the build scans +-2 cells)
-    int nc = (int)std::floor(ext / (0.5 * ce * (1.0 + 1e-12)));
-    nc = std::max(1, std::min(nc, 1024));
-    b.nc[d] = nc;
-    b.cinv[d] = nc / ext;
-    ncell *= nc;
     b.qscale[d] = 4294967296.0 / b.L[d];
     b.qinv[d] = b.L[d] / 4294967296.0;
     if (ctx->bc[d] && !ctx->host_halo && b.L[d] <= 2.0 * ce)
       return fail(ctx, CV2_ERR_BAD_ARG, "periodic box must be wider than twice the list cutoff");
   }
+  // Fully periodic boxes with at least 5 cells of width >= ce per axis use
+  // the wave-per-cell build with the coarse filter (k_nlist_cells); every
+  // other box uses the per-atom build over cells at least ce/2 wide.
+  bool fast = !ctx->host_halo;
+  for (int d = 0; d < 3; ++d)
+    fast = fast && b.bc[d] == 1 && std::floor(b.L[d] / (ce * (1.0 + 1e-12))) >= 5.0;
+  int64_t ncell = 1;
+  double wmax = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double margin = b.bc[d] ? 0.0 : ctx->ghost_cut;
+    b.glo[d] = b.lo[d] - margin;
+    const double ext = b.L[d] + 2.0 * margin;
+    const double wmin = fast ? ce : 0.5 * ce;
+    int nc = (int)std::floor(ext / (wmin * (1.0 + 1e-12)));
+    nc = std::max(1, std::min(nc, 1024));
+    b.nc[d] = nc;
+    b.cinv[d] = nc / ext;
+    wmax = std::max(wmax, ext / nc);
+    ncell *= nc;
+  }
+  // Coarse filter bounds.  Flooring to 2^21 units per box length moves a
+  // distance by less than sqrt(sum u_d^2); the float arithmetic on
+  // coordinates relative to a home atom (|x| < 2 w) adds at most
+  // e_f = 8 * 2^-24 * (2 w + ce) per axis.
+  {
+    double u2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      b.cscale[d] = 2097152.0 / b.L[d];
+      const double u = b.L[d] / 2097152.0;
+      b.cunit[d] = (float)u;
+      u2 += u * u;
+    }
+    const double ef = 8.0 * 5.9604644775390625e-08 * (2.0 * wmax + ce) * std::sqrt(3.0);
+    const double e = 1.01 * std::sqrt(u2) + ef + 1e-6 * ce;
+    b.cthr_in = std::nextafter((float)((ce - e) * (ce - e) * (1.0 - 4e-6)), 0.0f);
+    b.cthr_out = std::nextafter((float)((ce + e) * (ce + e) * (1.0 + 4e-6)), 1e30f);
+    bool ok = fast && ce > 2.0 * e;
+    for (int d = 0; d < 3; ++d) ok = ok && 0.5 * b.L[d] > ce + 2.0 * e + 2.0 * b.L[d] / 2097152.0;
+    if (fast && !ok) return fail(ctx, CV2_ERR_BAD_ARG, "box too large for the coarse list filter");
+    b.coarse = fast ? 1 : 0;
+    b.ghost_ok = ctx->ghost_cut >= ce ? 1 : 0;
+  }
   if (ncell > (1 << 26)) return fail(ctx, CV2_ERR_BAD_ARG, "cell grid too large");
+  {
+    int rc = ensure_wstat(ctx);
+    if (rc) return rc;
+  }
+  if (!ctx->s.box_dev) HIPCHK(dalloc(&ctx->s.box_dev, 1));
+  HIPCHK(hipMemcpyAsync(ctx->s.box_dev, &b, sizeof(BoxDev), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   if (ncell != ctx->s.ncell) {
     ctx->s.ncell = (int)ncell;
     HIPCHK(dalloc(&ctx->s.cell_count, ncell + 1));
@@ -436,6 +487,12 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
   HIPCHK(dalloc(&s.tag_alt, n));
   HIPCHK(dalloc(&s.id_alt, n));
   HIPCHK(dalloc(&s.perm, n));
+  HIPCHK(dalloc(&s.order, n));
+  {
+    int rc = ensure_wstat(ctx);
+    if (rc) return rc;
+  }
+  HIPCHK(dalloc(&s.cpos, n));
   HIPCHK(dalloc(&s.nnbr, n));
   // list capacity from the mean density: 1.6x the expected count + 64
   const double vol = (ctx->hi[0] - ctx->lo[0]) * (ctx->hi[1] - ctx->lo[1]) * (ctx->hi[2] - ctx->lo[2]);
@@ -713,6 +770,7 @@ int cv2_get_stats(cv2_ctx *ctx, cv2_stats *out) {
   out->capacity = ctx->s.cap;
   for (int d = 0; d < 3; ++d) out->ncell[d] = ctx->b.nc[d];
   out->precision = ctx->precision;
+  out->build_kind = ctx->b.coarse ? CV2_BUILD_CELL_WAVE : CV2_BUILD_PER_ATOM;
   return CV2_OK;
 }
 

@@ -41,27 +41,47 @@ __device__ inline int cell_of(const double4 &p, const BoxDev &b) {
   return cx + b.nc[0] * (cy + b.nc[1] * cz);
 }
 
-// Wrap, bin, count.  One thread per owned atom.
+// Wrap, bin, count.  One thread per owned atom.  The atoms are still in the
+// previous build's cell order, so neighbouring lanes mostly share a cell:
+// each run of equal cells in consecutive lanes takes its slots with one
+// atomic add (the in-cell order is fixed afterwards by k_rank).
 __global__ void k_wrap_bin(double4 *__restrict__ pos, int n, BoxDev b, int *__restrict__ cell,
                            int *__restrict__ slot, int *__restrict__ cell_count,
                            const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double4 p = pos[i];
-  double *c[3] = {&p.x, &p.y, &p.z};
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < n;
+  int ci = -1;
+  if (valid) {
+    double4 p = pos[i];
+    double *c[3] = {&p.x, &p.y, &p.z};
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    if (b.bc[d] != 1) continue;
-    if (*c[d] < b.lo[d])
-      *c[d] += b.L[d];
-    else if (*c[d] > b.hi[d])
-      *c[d] -= b.L[d];
+    for (int d = 0; d < 3; ++d) {
+      if (b.bc[d] != 1) continue;
+      if (*c[d] < b.lo[d])
+        *c[d] += b.L[d];
+      else if (*c[d] > b.hi[d])
+        *c[d] -= b.L[d];
+    }
+    pos[i] = p;
+    ci = cell_of(p, b);
+    cell[i] = ci;
   }
-  pos[i] = p;
-  const int ci = cell_of(p, b);
-  cell[i] = ci;
-  slot[i] = atomicAdd(&cell_count[ci], 1);
+  const int prev = __shfl_up(ci, 1, 64);
+  const bool lead = valid && (lane == 0 || prev != ci);
+  const unsigned long long lm = __ballot(lead);
+  const int nvalid = __popcll(__ballot(valid));  // valid lanes are a prefix
+  const unsigned long long upto = lane == 63 ? lm : lm & ((2ull << lane) - 1ull);
+  const int leader = upto ? 63 - __clzll(upto) : 0;
+  int base = 0;
+  if (lead) {
+    const unsigned long long above = lane == 63 ? 0ull : lm & ~((2ull << lane) - 1ull);
+    const int next = above ? __ffsll((long long)above) - 1 : nvalid;
+    base = atomicAdd(&cell_count[ci], next - lane);
+  }
+  base = __shfl(base, leader, 64);
+  if (valid) slot[i] = base + lane - leader;
 }
 
 // Exclusive scan of cell counts in two passes over tiles of kScanTile cells:
@@ -126,37 +146,20 @@ __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ 
 
 // Order atoms inside each cell by upload index so that the sorted order,
 // and with it every list and sum, is independent of atomic arrival order.
-// One wavefront per cell: each lane holds one atom and finds its rank by
-// comparing its key with every other lane's (keys are unique); cells with
-// more than 64 atoms fall back to an insertion sort by one lane.
-__global__ void __launch_bounds__(256)
-    k_sort_cells(int *__restrict__ perm, const int *__restrict__ start,
-                 const int *__restrict__ tag, int ncell, const int *__restrict__ flags) {
+// One thread per atom: its rank in its cell is the number of cell members
+// with a smaller upload index (indices are unique).
+__global__ void k_rank(const int *__restrict__ cell, const int *__restrict__ start,
+                       const int *__restrict__ perm, const int *__restrict__ tag,
+                       int *__restrict__ order, int n, const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (c >= ncell) return;
-  const int a = start[c], m = start[c + 1] - a;
-  if (m <= 1) return;
-  if (m <= 64) {
-    const int v = lane < m ? perm[a + lane] : 0;
-    const int key = lane < m ? tag[v] : 0x7fffffff;
-    int rank = 0;
-    for (int l = 0; l < m; ++l) rank += __shfl(key, l, 64) < key;
-    if (lane < m) perm[a + rank] = v;
-    return;
-  }
-  if (lane != 0) return;
-  for (int k = a + 1; k < a + m; ++k) {
-    const int v = perm[k];
-    const int tv = tag[v];
-    int q = k - 1;
-    while (q >= a && tag[perm[q]] > tv) {
-      perm[q + 1] = perm[q];
-      --q;
-    }
-    perm[q + 1] = v;
-  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell[i];
+  const int a = start[c], e = start[c + 1];
+  const int t = tag[i];
+  int rank = 0;
+  for (int k = a; k < e; ++k) rank += tag[perm[k]] < t;
+  order[a + rank] = i;
 }
 
 // Gather every per-atom array into the new order; remember build positions.
@@ -195,13 +198,15 @@ __global__ void k_copy_back(int n, const double4 *__restrict__ pos_alt, double4 
                             const double *__restrict__ acc_alt, double *__restrict__ acc,
                             const int *__restrict__ tag_alt, int *__restrict__ tag,
                             const int *__restrict__ id_alt, int *__restrict__ id,
-                            int4 *__restrict__ q, BoxDev b, const int *__restrict__ flags) {
+                            int4 *__restrict__ q, uint2 *__restrict__ cpos, BoxDev b,
+                            const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const double4 p = pos_alt[k];
   pos[k] = p;
   if (q) q[k] = qpack(p, b);
+  if (b.coarse) cpos[k] = pack_coarse(p.x, p.y, p.z, b);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     vel[d * n + k] = vel_alt[d * n + k];
@@ -253,8 +258,6 @@ __global__ void k_ghost_sort_cells(int *__restrict__ items, const int *__restric
   }
 }
 
-// Candidate cells along one axis: c-1, c, c+1 (periodic wrap, or clamped
-// for open axes), duplicates removed when the grid has fewer than 3 cells.
 // Candidate cells along one axis: c-2 .. c+2 (cells are at least half the
 // list cutoff wide), wrapped on periodic axes and clipped on open ones;
 // every cell once, with offset 0 (no pruning), when a periodic axis has
@@ -388,14 +391,40 @@ __device__ inline void scan_range(const double4 *__restrict__ pos, int j0, int j
   }
 }
 
+__device__ inline void wave_stat(int2 *wstat, int w, int cnt, int kept);
+__device__ void nlist_atom(const double4 *__restrict__ pos, int n, const BoxDev &b,
+                           const int *__restrict__ cstart, const int *__restrict__ gstart,
+                           const int *__restrict__ gitems, uint32_t *__restrict__ nbr,
+                           int *__restrict__ nnbr, int stride, int cap, int host_halo, int i,
+                           int &cnt_out, int &kept_out);
+
 __global__ void __launch_bounds__(256)
     k_nlist(const double4 *__restrict__ pos, int n, BoxDev b, const int *__restrict__ cstart,
             const int *__restrict__ gstart, const int *__restrict__ gitems,
             uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
-            int *__restrict__ flags, int host_halo) {
+            const int *__restrict__ flags, int host_halo, int2 *__restrict__ wstat) {
   if (flags[F_REBUILD] == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  int cnt = 0, kept = 0;
+  if (i < n) nlist_atom(pos, n, b, cstart, gstart, gitems, nbr, nnbr, stride, cap, host_halo, i, cnt, kept);
+  wave_stat(wstat, i >> 6, cnt, kept);
+}
+
+// Per-wavefront build statistics (longest list, entries kept), reduced by
+// k_build_done: no same-address atomics from thousands of wavefronts.
+__device__ inline void wave_stat(int2 *wstat, int w, int cnt, int kept) {
+  for (int o = 32; o > 0; o >>= 1) {
+    cnt = max(cnt, __shfl_xor(cnt, o, 64));
+    kept += __shfl_xor(kept, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) wstat[w] = make_int2(cnt, kept);
+}
+
+__device__ void nlist_atom(const double4 *__restrict__ pos, int n, const BoxDev &b,
+                           const int *__restrict__ cstart, const int *__restrict__ gstart,
+                           const int *__restrict__ gitems, uint32_t *__restrict__ nbr,
+                           int *__restrict__ nnbr, int stride, int cap, int host_halo, int i,
+                           int &cnt_out, int &kept_out) {
   const double4 pi = pos[i];
   const double pc[3] = {pi.x, pi.y, pi.z};
   int cc[3], sten[3][kSpan], nst[3];
@@ -457,15 +486,337 @@ __global__ void __launch_bounds__(256)
   out.finish();
   const int kept = out.cnt < cap ? out.cnt : cap;
   nnbr[i] = kept;
-  if (out.cnt > cap) atomicOr(&flags[F_OVERFLOW], 1);
-  atomicMax(&flags[F_MAXNBR], out.cnt);
-  atomicAdd(&flags[F_ENTRIES], kept);
+  cnt_out = out.cnt;
+  kept_out = kept;
 }
 
-__global__ void k_build_done(int *__restrict__ flags) {
+// Verlet list for fully periodic boxes with at least 5 cells per axis, each
+// at least the list cutoff wide (BoxDev::coarse): one wavefront per home
+// cell.
+//
+// The 27 stencil cells form 9 (dy, dz) rows; the x-adjacent cells
+// cx-1 .. cx+1 of a row are one contiguous range of sorted atoms (two when
+// the row wraps in x).  Slot k of the wave holds up to 64 atoms of range k,
+// one per lane, as coordinates relative to the first home atom (from the
+// 21-bit coarse positions, so the periodic image is implicit).  The wave
+// then takes the home atoms one at a time: every lane tests its candidates
+// against the home atom, and the accepted entries are appended to the home
+// atom's list in candidate order (ballot + prefix count), coalesced within
+// each 4-entry group.  Ranges longer than 64 atoms continue in extra passes.
+//
+// A candidate is accepted without further work when its coarse squared
+// distance is below cthr_in and rejected when it is at or above cthr_out;
+// in between, the FP64 predicate of the general build decides (with the
+// reference's ghost-existence rule for images when the ghost cutoff is
+// below the list cutoff).
+__device__ inline int sext21(unsigned v) { return (int)(v << 11) >> 11; }
+
+__device__ inline float coarse_rel(unsigned q, unsigned o, float unit) {
+  return (float)sext21(q - o) * unit;
+}
+
+__device__ inline uint32_t img_bits(int vx, int vy, int vz) {
+  return ((uint32_t)(vx + 1) << 26) | ((uint32_t)(vy + 1) << 28) | ((uint32_t)(vz + 1) << 30);
+}
+
+constexpr int kWaveRows = 9;
+constexpr int kMaxHomeCell = 64;  // home atoms per pass (LDS counts)
+
+// What the main path of k_nlist_cells needs of BoxDev (kernel arguments);
+// the rare FP64 path reads the device copy of BoxDev instead, so that its
+// constants do not occupy registers for the whole kernel.
+struct CellWaveBox {
+  int nc[3];
+  float cunit[3];
+  float cthr_in, cthr_out;
+  int ghost_ok;
+};
+
+// Range of sorted atoms of stencil row r (dy = r % 3 - 1, dz = r / 3 - 1) of
+// home cell (cx, cy, cz).  part 0: the x-cells that do not wrap; part 1: the
+// wrapped one (cx == 0: cell nx-1 seen with image +1; cx == nx-1: cell 0
+// with image -1), empty otherwise.
+__device__ inline void row_range(const int *__restrict__ cstart, const CellWaveBox &b, int cx,
+                                 int cy, int cz, int r, int part, int &j0, int &len,
+                                 uint32_t &img) {
+  const int nx = b.nc[0], ny = b.nc[1], nz = b.nc[2];
+  int y = cy + r % 3 - 1, z = cz + r / 3 - 1, vy = 0, vz = 0;
+  if (y < 0) { y += ny; vy = 1; } else if (y >= ny) { y -= ny; vy = -1; }
+  if (z < 0) { z += nz; vz = 1; } else if (z >= nz) { z -= nz; vz = -1; }
+  const int row = nx * (y + ny * z);
+  const bool xlo = cx == 0, xhi = cx == nx - 1;
+  if (part == 0) {
+    const int a0 = row + (xlo ? 0 : cx - 1), a1 = row + (xhi ? nx : cx + 2);
+    j0 = cstart[a0];
+    len = cstart[a1] - j0;
+    img = img_bits(0, vy, vz);
+  } else if (xlo || xhi) {
+    const int b0 = xlo ? row + nx - 1 : row;
+    j0 = cstart[b0];
+    len = cstart[b0 + 1] - j0;
+    img = img_bits(xlo ? 1 : -1, vy, vz);
+  } else {
+    j0 = 0;
+    len = 0;
+    img = 0u;
+  }
+}
+
+constexpr int kSlots = 12;                  // candidates per pass: kSlots * 64
+constexpr int kPassCand = kSlots * 64;
+constexpr int kSegs = 2 * kWaveRows;         // row ranges: 9 direct-x + 9 wrapped-x
+
+// One pass over the flattened candidates [c0, c0 + kPassCand) of the home
+// cell against home atoms [h0, h1).  cand (LDS) holds the pass's
+// candidate entries; STAGED (single-pass cells): each home atom's accepted
+// entries are collected in buf (LDS) and written as whole 4-entry groups;
+// otherwise they are stored one by one and counts carried in cnt_io.
+template <bool STAGED>
+__device__ void cell_wave_pass(const double4 *__restrict__ pos, const uint2 *__restrict__ cpos,
+                               const CellWaveBox &b, const BoxDev *__restrict__ bdev,
+                               const uint32_t *cand, int ncand, int c0,
+                               int self_base, int self_j0, int h0, int h1, const unsigned o[3],
+                               uint32_t *__restrict__ nbr, int *__restrict__ nnbr, size_t stride,
+                               int cap, uint32_t *buf, int *cnt_io, const float4 *home,
+                               int lane, bool first, int &wmax, int &went) {
+  float px[kSlots], py[kSlots], pz[kSlots], tin[kSlots];
+  uint32_t ent[kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const int t = k * 64 + lane;
+    px[k] = 1e30f;  // empty slot: never accepted, never in the band
+    py[k] = 0.0f;
+    pz[k] = 0.0f;
+    tin[k] = b.cthr_in;
+    ent[k] = 0u;
+    if (t < ncand) {
+      const uint32_t e = cand[t];
+      unsigned qx, qy, qz;
+      unpack_coarse(cpos[e & kIndexMask], qx, qy, qz);
+      px[k] = coarse_rel(qx, o[0], b.cunit[0]);
+      py[k] = coarse_rel(qy, o[1], b.cunit[1]);
+      pz[k] = coarse_rel(qz, o[2], b.cunit[2]);
+      // images whose ghost may not exist always take the exact path
+      if ((e >> 26) != kNoImage && !b.ghost_ok) tin[k] = -1.0f;
+      ent[k] = e;
+    }
+  }
+  // lane-private scratch word for lanes that accept nothing (STAGED)
+  const int dummy_at = kPassCand + 4 + lane;
+  for (int h = h0; h < h1; ++h) {
+    // home atom coordinates from LDS: a scalar load here would make every
+    // LDS wait also wait for it
+    const float4 hc = home[h - h0];
+    const float hx = hc.x, hy = hc.y, hz = hc.z;
+    // position of the home atom itself among the candidates
+    const int sp = self_base + (h - self_j0) - c0;
+    const int kself = sp >= 0 && sp < kPassCand ? sp >> 6 : -1;
+    const unsigned long long selfbit = 1ull << (sp & 63);
+    int cnt = (STAGED || first) ? 0 : cnt_io[h - h0];
+    // Accepted entries go to the list in candidate order: a lane's entry
+    // position is cnt + (accepted lanes below it).  The accept decisions
+    // are 64-bit lane masks (scalar registers); lanes that accept nothing
+    // write a scratch word, so the common path has no branches.
+    auto append = [&](unsigned long long m, uint32_t e) {
+      int at = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)cnt));
+      const bool take = __builtin_amdgcn_inverse_ballot_w64(m);
+      if (STAGED) {
+        __asm__ volatile("" : "+v"(at));  // computed by every lane: a select, not a branch
+        buf[take ? at : dummy_at] = e;
+      } else if (take && at < cap) {
+        nbr[((size_t)(at >> 2) * stride + h) * 4 + (at & 3)] = e;
+      }
+      cnt += __popcll(m);
+    };
+    unsigned long long anyband = 0ull;
+#pragma unroll
+    for (int g = 0; g < kSlots; g += 3) {
+      if (g * 64 >= ncand) continue;  // uniform; keeps the loops unrolled
+#pragma unroll
+      for (int k = g; k < g + 3; ++k) {
+        const float dx = px[k] - hx, dy = py[k] - hy, dz = pz[k] - hz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const unsigned long long m_in = __builtin_amdgcn_ballot_w64(r2 < tin[k]);
+        const unsigned long long m_out = __builtin_amdgcn_ballot_w64(r2 < b.cthr_out);
+        anyband |= m_out & ~m_in;  // the home atom itself is in m_in
+        append(k == kself ? m_in & ~selfbit : m_in, ent[k]);
+      }
+    }
+    // Candidates in the band between the coarse thresholds (rare): the FP64
+    // predicate decides, with the ghost-existence rule for images.  Kept
+    // out of the loop above so that its global loads never make the common
+    // path wait for outstanding list stores.
+    if (__builtin_expect(anyband != 0ull, 0)) {
+      const BoxDev &B = *bdev;
+      const double *pq = reinterpret_cast<const double *>(pos + h);
+      const double pix = pq[0], piy = pq[1], piz = pq[2];
+#pragma unroll
+      for (int k = 0; k < kSlots; ++k) {
+        if (k * 64 >= ncand) continue;
+        const float dx = px[k] - hx, dy = py[k] - hy, dz = pz[k] - hz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool band = (r2 < b.cthr_out) & !(r2 < tin[k]);
+        bool take = false;
+        if (band) {
+          uint32_t e = ent[k];
+          __asm__ volatile("" : "+v"(e));  // decode here, not hoisted
+          // x, y, z only: a loaded but unused w would leave a load pending
+          // that the next reuse of its register has to wait for
+          const double *pp = reinterpret_cast<const double *>(pos + (e & kIndexMask));
+          const int v[3] = {(int)((e >> 26) & 3u) - 1, (int)((e >> 28) & 3u) - 1, (int)(e >> 30) - 1};
+          const double pjc[3] = {pp[0], pp[1], pp[2]};
+          // selects, not branches: every loaded value is used on every path,
+          // so no load is left outstanding past this block
+          double gc[3];
+          bool exists = true;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const int flag = (pjc[d] < B.lo[d] + B.ghost_cut) ? -1 : ((pjc[d] > B.hi[d] - B.ghost_cut) ? 1 : 0);
+            exists = exists & ((v[d] == 0) | (flag == v[d]));
+            gc[d] = v[d] != 0 ? __dsub_rn(pjc[d], (double)v[d] * B.L[d]) : pjc[d];
+          }
+          take = list_pair(pix, piy, piz, gc[0], gc[1], gc[2], B.ce_sq) & exists;
+        }
+        append(__builtin_amdgcn_ballot_w64(take), ent[k]);
+      }
+    }
+    if (STAGED) {
+      // whole groups, the last one padded with the atom itself
+      const int kept = min(cnt, cap);
+      if (lane < 3) buf[kept + lane] = (uint32_t)h | (kNoImage << 26);
+      for (int g = lane; g * 4 < kept; g += 64)
+        reinterpret_cast<u32x4 *>(nbr)[(size_t)g * stride + h] = reinterpret_cast<const u32x4 *>(buf)[g];
+      if (lane == 0) nnbr[h] = kept;
+      wmax = max(wmax, cnt);
+      went += kept;
+    } else if (lane == 0) {
+      cnt_io[h - h0] = cnt;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_nlist_cells(const double4 *__restrict__ pos, const uint2 *__restrict__ cpos, CellWaveBox b,
+                  const BoxDev *__restrict__ bdev, const int *__restrict__ cstart,
+                  uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
+                  const int *__restrict__ flags, int nblk_real, int2 *__restrict__ wstat) {
   if (flags[F_REBUILD] == 0) return;
-  flags[F_REBUILD] = 0;
-  flags[F_REBUILDS] += 1;
+  // consecutive cells (which share most candidates) on the same XCD
+  const int per = gridDim.x >> 3;
+  const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (blk >= nblk_real) return;
+  // the home cell is uniform across the wavefront
+  const int wv = (int)(threadIdx.x >> 6);
+  const int c = __builtin_amdgcn_readfirstlane(blk * 4 + wv);
+  const int lane = threadIdx.x & 63;
+  const int nx = b.nc[0], ny = b.nc[1];
+  if (c >= nx * ny * b.nc[2]) return;
+  const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+  const int h0 = cstart[c], h1 = cstart[c + 1];
+  if (h0 == h1) {
+    if (lane == 0) wstat[c] = make_int2(0, 0);
+    return;
+  }
+  // candidates of a pass, then (staged) one home atom's entries + 3 pads;
+  // 64 scratch words
+  __shared__ __attribute__((aligned(16))) uint32_t lds[4][kPassCand + 4 + 64];
+  __shared__ int cnt_sh[4][kMaxHomeCell];
+  __shared__ float4 home_sh[4][kMaxHomeCell];  // home atoms relative to the first
+  uint32_t *cand = lds[wv];
+  // segment table: flattened position of each row range
+  int total = 0, self_base = 0, self_j0 = 0;
+  for (int sg = 0; sg < kSegs; ++sg) {
+    int j0, len;
+    uint32_t img;
+    row_range(cstart, b, cx, cy, cz, sg % kWaveRows, sg / kWaveRows, j0, len, img);
+    if (sg == kWaveRows / 2) {  // dy = dz = 0, direct x: contains the home cell
+      self_base = total;
+      self_j0 = j0;
+    }
+    total += len;
+  }
+  unsigned o[3];
+  unpack_coarse(cpos[h0], o[0], o[1], o[2]);
+  int wmax = 0, went = 0;
+  const bool staged = total <= kPassCand;
+  for (int h = h0; h < h1; h += kMaxHomeCell) {
+    const int he = min(h1, h + kMaxHomeCell);
+    if (h + lane < he) {
+      unsigned qx, qy, qz;
+      unpack_coarse(cpos[h + lane], qx, qy, qz);
+      home_sh[wv][lane] = make_float4(coarse_rel(qx, o[0], b.cunit[0]), coarse_rel(qy, o[1], b.cunit[1]),
+                                      coarse_rel(qz, o[2], b.cunit[2]), 0.0f);
+    }
+    for (int c0 = 0; c0 < total; c0 += kPassCand) {
+      // gather this pass's candidate entries into LDS
+      int at = 0;
+      for (int sg = 0; sg < kSegs; ++sg) {
+        int j0, len;
+        uint32_t img;
+        row_range(cstart, b, cx, cy, cz, sg % kWaveRows, sg / kWaveRows, j0, len, img);
+        for (int t = lane; t < len; t += 64) {
+          const int p = at + t - c0;
+          if (p >= 0 && p < kPassCand) cand[p] = (uint32_t)(j0 + t) | img;
+        }
+        at += len;
+      }
+      const int ncand = min(kPassCand, total - c0);
+      if (staged)
+        cell_wave_pass<true>(pos, cpos, b, bdev, cand, ncand, c0, self_base, self_j0, h, he, o, nbr,
+                             nnbr, (size_t)stride, cap, cand, cnt_sh[wv], home_sh[wv], lane, c0 == 0,
+                             wmax, went);
+      else
+        cell_wave_pass<false>(pos, cpos, b, bdev, cand, ncand, c0, self_base, self_j0, h, he, o, nbr,
+                              nnbr, (size_t)stride, cap, nullptr, cnt_sh[wv], home_sh[wv], lane,
+                              c0 == 0, wmax, went);
+    }
+    if (!staged) {
+      // pad the last group with the atom itself, store lengths
+      for (int hh = h; hh < he; ++hh) {
+        const int cnt = cnt_sh[wv][hh - h];
+        const int kept = min(cnt, cap);
+        const int padded = (kept + 3) & ~3;
+        if (lane < padded - kept)
+          nbr[((size_t)((kept + lane) >> 2) * stride + hh) * 4 + ((kept + lane) & 3)] =
+              (uint32_t)hh | (kNoImage << 26);
+        if (lane == 0) nnbr[hh] = kept;
+        wmax = max(wmax, cnt);
+        went += kept;
+      }
+    }
+  }
+  if (lane == 0) wstat[c] = make_int2(wmax, went);
+}
+
+// Close a build: reduce the per-wavefront statistics, clear the request.
+__global__ void __launch_bounds__(256)
+    k_build_done(int *__restrict__ flags, const int2 *__restrict__ wstat, int nw, int cap) {
+  if (flags[F_REBUILD] == 0) return;
+  __shared__ int smax[256], ssum[256];
+  int m = 0, t = 0;
+  for (int k = threadIdx.x; k < nw; k += 256) {
+    const int2 v = wstat[k];
+    m = max(m, v.x);
+    t += v.y;
+  }
+  smax[threadIdx.x] = m;
+  ssum[threadIdx.x] = t;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + o]);
+      ssum[threadIdx.x] += ssum[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    flags[F_MAXNBR] = smax[0];
+    flags[F_ENTRIES] = ssum[0];
+    if (smax[0] > cap) flags[F_OVERFLOW] = 1;
+    flags[F_REBUILD] = 0;
+    flags[F_REBUILDS] += 1;
+  }
 }
 
 __global__ void k_build_begin(int *__restrict__ flags, int *__restrict__ cell_count, int ncell,
@@ -475,10 +826,6 @@ __global__ void k_build_begin(int *__restrict__ flags, int *__restrict__ cell_co
   if (c < ncell) {
     cell_count[c] = 0;
     if (gcount) gcount[c] = 0;
-  }
-  if (c == 0) {
-    flags[F_MAXNBR] = 0;
-    flags[F_ENTRIES] = 0;
   }
 }
 
@@ -521,15 +868,15 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) 
     k_wrap_bin<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell, s.slot, s.cell_count, s.flags);
     launch_scan(s.cell_count, s.cell_start, s.tile_sum, nc, s.flags, st);
     k_scatter<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.slot, s.cell_start, s.perm, n, s.flags);
-    k_sort_cells<<<nblk(nc, 4), 256, 0, st>>>(s.perm, s.cell_start, s.tag, nc, s.flags);
-    k_permute<<<nblk(n, 256), 256, 0, st>>>(s.perm, n, s.pos, s.pos_alt, s.pos_ref, s.vel,
+    k_rank<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.cell_start, s.perm, s.tag, s.order, n, s.flags);
+    k_permute<<<nblk(n, 256), 256, 0, st>>>(s.order, n, s.pos, s.pos_alt, s.pos_ref, s.vel,
                                             s.vel_alt, s.acc, s.acc_alt, s.tag, s.tag_alt, s.id,
                                             s.id_alt, s.flags);
     // The host cannot know whether the device ran the build, so the new
     // order is copied back instead of swapping buffer pointers.
     k_copy_back<<<nblk(n, 256), 256, 0, st>>>(n, s.pos_alt, s.pos, s.vel_alt, s.vel, s.acc_alt,
                                               s.acc, s.tag_alt, s.tag, s.id_alt, s.id,
-                                              s.write_q ? s.q : nullptr, b, s.flags);
+                                              s.write_q ? s.q : nullptr, s.cpos, b, s.flags);
   }
   if (ghosts) {
     k_ghost_bin<<<nblk(s.ng, 256), 256, 0, st>>>(s.pos, s.pos_ref, n, s.ng, b, s.cell + n,
@@ -539,11 +886,27 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) 
                                                       s.gcell_items, s.ng, s.flags);
     k_ghost_sort_cells<<<nblk(nc, 256), 256, 0, st>>>(s.gcell_items, s.gcell_start, nc, s.flags);
   }
-  if (n > 0)
+  int nw = 0;  // per-wavefront statistics written by the list kernel
+  if (n > 0 && b.coarse) {
+    CellWaveBox cb{};
+    for (int d = 0; d < 3; ++d) {
+      cb.nc[d] = b.nc[d];
+      cb.cunit[d] = b.cunit[d];
+    }
+    cb.cthr_in = b.cthr_in;
+    cb.cthr_out = b.cthr_out;
+    cb.ghost_ok = b.ghost_ok;
+    const int real = nblk(nc, 4), grid = (real + 7) / 8 * 8;
+    k_nlist_cells<<<grid, 256, 0, st>>>(s.pos, s.cpos, cb, s.box_dev, s.cell_start, s.nbr, s.nnbr,
+                                        s.stride, s.cap, s.flags, real, s.wstat);
+    nw = nc;
+  } else if (n > 0) {
     k_nlist<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell_start, s.gcell_start,
                                           s.gcell_items, s.nbr, s.nnbr, s.stride, s.cap, s.flags,
-                                          ghosts ? 1 : 0);
-  k_build_done<<<1, 1, 0, st>>>(s.flags);
+                                          ghosts ? 1 : 0, s.wstat);
+    nw = nblk(n, 256) * 4;
+  }
+  k_build_done<<<1, 256, 0, st>>>(s.flags, s.wstat, nw, s.cap);
 }
 
 }  // namespace cv2

@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(kBlock)
     k_lj_f32fx(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
                const int *__restrict__ nnbr, int stride, int n, const int *__restrict__ id,
                const LJDev *__restrict__ lj, BoxDev b, double *__restrict__ acc,
-               long long *__restrict__ fx_acc, int *__restrict__ flags, int nblk_real) {
+               long long *__restrict__ partials, int *__restrict__ flags, int nblk_real) {
   __shared__ long long sh[kBlock / 64];
   const int blk = xcd_block(nblk_real);
   if (blk < 0) return;
@@ -240,10 +240,28 @@ __global__ void __launch_bounds__(kBlock)
     const long long v = block_sum_i64(vir, sh);
     __syncthreads();
     const long long u = block_sum_i64(pe, sh);
-    if (threadIdx.x == 0) {
-      atomicAdd((unsigned long long *)&fx_acc[0], (unsigned long long)v);
-      atomicAdd((unsigned long long *)&fx_acc[1], (unsigned long long)u);
+    if (threadIdx.x == 0) {  // per-block partials, summed by k_reduce_i64
+      partials[2 * blk] = v;
+      partials[2 * blk + 1] = u;
     }
+  }
+}
+
+// Sum of per-block fixed-point partials (integer: exact in any order).
+__global__ void __launch_bounds__(1024) k_reduce_i64(const long long *__restrict__ partials,
+                                                     int nblk, long long *__restrict__ out) {
+  __shared__ long long sh[1024];
+  for (int c = 0; c < 2; ++c) {
+    long long t = 0;
+    for (int k = threadIdx.x; k < nblk; k += 1024) t += partials[(size_t)k * 2 + c];
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0];
+    __syncthreads();
   }
 }
 
@@ -266,10 +284,6 @@ __global__ void __launch_bounds__(1024) k_reduce_partials(const double *__restri
   }
 }
 
-__global__ void k_zero_i64(long long *p, int k) {
-  if ((int)threadIdx.x < k) p[threadIdx.x] = 0;
-}
-
 template <bool MULTI, bool ENERGY, bool HALO>
 void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precision, hipStream_t st) {
   const int real = (s.n + kBlock - 1) / kBlock;
@@ -280,10 +294,11 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
                                                            s.flags, real);
     if (ENERGY) k_reduce_partials<<<1, 1024, 0, st>>>(s.partials, real, 2, s.reduce_out);
   } else {
-    if (ENERGY) k_zero_i64<<<1, 64, 0, st>>>(s.fx_acc, 2);
+    long long *part = reinterpret_cast<long long *>(s.partials);
     k_lj_f32fx<MULTI, ENERGY, HALO><<<grid, kBlock, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n,
-                                                             s.id, lj_dev, b, s.acc, s.fx_acc,
+                                                             s.id, lj_dev, b, s.acc, part,
                                                              s.flags, real);
+    if (ENERGY) k_reduce_i64<<<1, 1024, 0, st>>>(part, real, s.fx_acc);
   }
 }
 

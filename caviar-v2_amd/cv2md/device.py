@@ -27,7 +27,8 @@ _i64 = C.POINTER(C.c_int64)
 class Stats(C.Structure):
     _fields_ = [("natoms", C.c_int64), ("nghost", C.c_int64), ("steps", C.c_int64),
                 ("rebuilds", C.c_int64), ("list_entries", C.c_int64), ("max_neighbors", C.c_int32),
-                ("capacity", C.c_int32), ("ncell", C.c_int32 * 3), ("precision", C.c_int32)]
+                ("capacity", C.c_int32), ("ncell", C.c_int32 * 3), ("precision", C.c_int32),
+                ("build_kind", C.c_int32)]
 
 
 class Timing(C.Structure):
@@ -188,7 +189,8 @@ class Context:
         self._chk(self.L.cv2_get_stats(self.h, C.byref(s)))
         return dict(natoms=s.natoms, nghost=s.nghost, steps=s.steps, rebuilds=s.rebuilds,
                     list_entries=s.list_entries, max_neighbors=s.max_neighbors, capacity=s.capacity,
-                    ncell=tuple(s.ncell), precision=s.precision)
+                    ncell=tuple(s.ncell), precision=s.precision,
+                    build_kind=('cell_wave' if s.build_kind == 1 else 'per_atom'))
 
     def set_timing(self, enable=True):
         self._chk(self.L.cv2_set_timing(self.h, int(bool(enable))))

@@ -47,7 +47,14 @@ typedef struct {
   int32_t capacity;         /* list slots per atom */
   int32_t ncell[3];         /* cell grid */
   int32_t precision;
+  int32_t build_kind;       /* CV2_BUILD_CELL_WAVE or CV2_BUILD_PER_ATOM */
 } cv2_stats;
+
+/* list build in use (cv2_stats.build_kind): one wavefront per cell with a
+ * coarse distance filter (fully periodic boxes with >= 5 cells of width
+ * >= cutoff*coef per axis), or one thread per atom otherwise */
+#define CV2_BUILD_PER_ATOM 0
+#define CV2_BUILD_CELL_WAVE 1
 
 /* Context lifetime.  Stands in for Caviar2 (caviar2.hpp:39-66), which owns
  * Atom_data, Domain, Neighborlist and Md_simulator by value. */

@@ -73,6 +73,59 @@ def test_pair_set_bit_exact(case):
     assert np.array_equal(got, exp)
 
 
+def wave_ghost_below_cutoff():
+    # cell-wave build (5 cells of width >= 3.0 per axis) with images whose
+    # ghost may not exist: those candidates always take the FP64 path
+    s = perturbed(lj_fluid(10, seed=7), 0.08, 7)
+    s["nl_coef"] = 1.2
+    s["ghost_cutoff"] = 2.8
+    return s
+
+
+def wave_dense_cells():
+    # about 65 atoms per cell and 1750 candidates per cell: several home-atom
+    # chunks and several candidate passes (unstaged stores) per wavefront
+    rng = np.random.default_rng(11)
+    s = lj_fluid(4)
+    L = 5.2 * 2.8
+    n = int(2.6 * L ** 3)
+    s["pos"] = rng.uniform(0.0, L, (n, 3))
+    s["vel"] = np.zeros((n, 3))
+    s["type"] = np.zeros(n, np.int64)
+    s["lo"], s["hi"] = (0.0, 0.0, 0.0), (L, L, L)
+    return s
+
+
+WAVE_CASES = {
+    "c1_fcc": lambda: lj_fluid(10),
+    "ka_4000": lambda: ka_mixture(4000, seed=4),
+    "wave_ghost_below_cutoff": wave_ghost_below_cutoff,
+    "wave_dense_cells": wave_dense_cells,
+}
+
+
+@pytest.mark.parametrize("name", sorted(WAVE_CASES))
+def test_pair_set_cell_wave_build(name):
+    """Systems that take the wavefront-per-cell build: bit-exact pair sets."""
+    s = WAVE_CASES[name]()
+    print(f"[pairs/cell-wave] {name} n={len(s['pos'])}", flush=True)
+    with D.Context.from_sysdef(s) as c:
+        c.build(True)
+        st = c.stats()
+        got = sort_rows(c.pairs())
+    assert st["build_kind"] == "cell_wave"
+    exp = oracle_pairs(s["pos"], s)
+    assert st["max_neighbors"] <= st["capacity"]
+    assert len(got) == len(exp)
+    assert np.array_equal(got, exp)
+
+
+def test_per_atom_build_in_use_for_small_boxes():
+    with D.Context.from_sysdef(ka_mixture(500, seed=3)) as c:
+        c.build(True)
+        assert c.stats()["build_kind"] == "per_atom"
+
+
 @pytest.mark.parametrize("prec", [D.CV2_PREC_FP64, D.CV2_PREC_FP32_FIXED])
 def test_forces_virial_energy(case, prec):
     name, s = case
