@@ -125,7 +125,7 @@ hipError_t dalloc(T **p, size_t count) {
 
 void free_state(DevState &s) {
   void *ptrs[] = {s.pos, s.pos_alt, s.pos_ref, s.q, s.vel, s.vel_alt, s.acc, s.acc_alt, s.tag,
-                  s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.order, s.cpos, s.box_dev, s.wstat, s.cell_count, s.cell_start,
+                  s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.order, s.box_dev, s.wstat, s.cell_count, s.cell_start,
                   s.gcell_count, s.gcell_start, s.gcell_items, s.tile_sum, s.nbr, s.nnbr, s.flags,
                   s.partials, s.fx_acc, s.reduce_out};
   for (void *p : ptrs)
@@ -184,15 +184,14 @@ int setup_box(cv2_ctx *ctx) {
     wmax = std::max(wmax, ext / nc);
     ncell *= nc;
   }
-  // Coarse filter bounds.  Flooring to 2^21 units per box length moves a
+  // Coarse filter bounds.  Rounding to 2^32 units per box length moves a
   // distance by less than sqrt(sum u_d^2); the float arithmetic on
   // coordinates relative to a home atom (|x| < 2 w) adds at most
   // e_f = 8 * 2^-24 * (2 w + ce) per axis.
   {
     double u2 = 0.0;
     for (int d = 0; d < 3; ++d) {
-      b.cscale[d] = 2097152.0 / b.L[d];
-      const double u = b.L[d] / 2097152.0;
+      const double u = b.L[d] / 4294967296.0;
       b.cunit[d] = (float)u;
       u2 += u * u;
     }
@@ -201,7 +200,7 @@ int setup_box(cv2_ctx *ctx) {
     b.cthr_in = std::nextafter((float)((ce - e) * (ce - e) * (1.0 - 4e-6)), 0.0f);
     b.cthr_out = std::nextafter((float)((ce + e) * (ce + e) * (1.0 + 4e-6)), 1e30f);
     bool ok = fast && ce > 2.0 * e;
-    for (int d = 0; d < 3; ++d) ok = ok && 0.5 * b.L[d] > ce + 2.0 * e + 2.0 * b.L[d] / 2097152.0;
+    for (int d = 0; d < 3; ++d) ok = ok && 0.5 * b.L[d] > ce + 2.0 * e;
     if (fast && !ok) return fail(ctx, CV2_ERR_BAD_ARG, "box too large for the coarse list filter");
     b.coarse = fast ? 1 : 0;
     b.ghost_ok = ctx->ghost_cut >= ce ? 1 : 0;
@@ -492,7 +491,6 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
     int rc = ensure_wstat(ctx);
     if (rc) return rc;
   }
-  HIPCHK(dalloc(&s.cpos, n));
   HIPCHK(dalloc(&s.nnbr, n));
   // list capacity from the mean density: 1.6x the expected count + 64
   const double vol = (ctx->hi[0] - ctx->lo[0]) * (ctx->hi[1] - ctx->lo[1]) * (ctx->hi[2] - ctx->lo[2]);

@@ -49,14 +49,12 @@ struct BoxDev {
   double qinv[3];   // length per fixed-point unit
   int bc[3];        // 1 = periodic (images generated on the device)
   int nc[3];
-  // Coarse coordinates of the list build (pack_coarse): 21-bit fixed point,
-  // 2^21 units per box length on every axis.  Used only by the wave-per-cell
-  // build of fully periodic boxes (coarse = 1).  A candidate whose coarse squared distance is
-  // below cthr_in is within the list cutoff for certain, one at or above
-  // cthr_out is outside it for certain; the band between is decided by the
-  // FP64 predicate.
-  double cscale[3];  // units per length
-  float cunit[3];    // length per unit
+  // Coarse filter of the wave-per-cell list build (fully periodic boxes,
+  // coarse = 1), on the 32-bit fixed-point coordinates q: a candidate whose
+  // float squared distance from them is below cthr_in is within the list
+  // cutoff for certain, one at or above cthr_out is outside it for certain;
+  // the band between is decided by the FP64 predicate.
+  float cunit[3];    // length per q unit (L / 2^32)
   float cthr_in, cthr_out;
   int coarse;
   int ghost_ok;  // ghost cutoff >= list cutoff: every image within it exists
@@ -114,24 +112,6 @@ __device__ inline int4 pack_q(double x, double y, double z, int type, const doub
   return make_int4(q[0], q[1], q[2], (int)w);
 }
 
-constexpr unsigned kCoarseMask = (1u << 21) - 1u;
-
-// Coarse position: x in bits 0-20, y in bits 21-41, z in bits 42-62.
-__device__ inline uint2 pack_coarse(double x, double y, double z, const BoxDev &b) {
-  const double c[3] = {x, y, z};
-  unsigned u[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-    u[d] = (unsigned)(long long)floor((c[d] - b.lo[d]) * b.cscale[d]) & kCoarseMask;
-  return make_uint2(u[0] | (u[1] << 21), (u[1] >> 11) | (u[2] << 10));
-}
-
-__device__ inline void unpack_coarse(uint2 c, unsigned &x, unsigned &y, unsigned &z) {
-  x = c.x & kCoarseMask;
-  y = __builtin_amdgcn_alignbit(c.y, c.x, 21) & kCoarseMask;
-  z = c.y >> 10;
-}
-
 struct DevState {
   int n;        // owned atoms
   int ng;       // host-halo ghosts
@@ -142,7 +122,6 @@ struct DevState {
   double *vel, *vel_alt;  // 3 * n each (SoA: x block, y block, z block)
   double *acc, *acc_alt;
   int *tag, *tag_alt, *id, *id_alt;
-  uint2 *cpos;                  // coarse positions (pack_coarse), written by each build
   BoxDev *box_dev;              // device copy of the box constants
   int2 *wstat;                  // per-wavefront build statistics (longest list, entries)
   int64_t wstat_cap;

@@ -198,7 +198,7 @@ __global__ void k_copy_back(int n, const double4 *__restrict__ pos_alt, double4 
                             const double *__restrict__ acc_alt, double *__restrict__ acc,
                             const int *__restrict__ tag_alt, int *__restrict__ tag,
                             const int *__restrict__ id_alt, int *__restrict__ id,
-                            int4 *__restrict__ q, uint2 *__restrict__ cpos, BoxDev b,
+                            int4 *__restrict__ q, BoxDev b,
                             const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -206,7 +206,6 @@ __global__ void k_copy_back(int n, const double4 *__restrict__ pos_alt, double4 
   const double4 p = pos_alt[k];
   pos[k] = p;
   if (q) q[k] = qpack(p, b);
-  if (b.coarse) cpos[k] = pack_coarse(p.x, p.y, p.z, b);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     vel[d * n + k] = vel_alt[d * n + k];
@@ -509,10 +508,17 @@ __device__ void nlist_atom(const double4 *__restrict__ pos, int n, const BoxDev 
 // in between, the FP64 predicate of the general build decides (with the
 // reference's ghost-existence rule for images when the ghost cutoff is
 // below the list cutoff).
-__device__ inline int sext21(unsigned v) { return (int)(v << 11) >> 11; }
+// Coordinate relative to a reference atom, from the 32-bit fixed-point
+// coordinates (the int32 difference is the minimum image on periodic axes).
+__device__ inline float q_rel(int q, int o, float unit) {
+  return (float)(int)((unsigned)q - (unsigned)o) * unit;
+}
 
-__device__ inline float coarse_rel(unsigned q, unsigned o, float unit) {
-  return (float)sext21(q - o) * unit;
+// x, y, z of q[j] only (w is not needed, and a loaded but unused w would
+// leave a load outstanding for the next reuse of its register to wait for)
+__device__ inline int3 load_q3(const int4 *q, int j) {
+  const int *p = reinterpret_cast<const int *>(q + j);
+  return make_int3(p[0], p[1], p[2]);
 }
 
 __device__ inline uint32_t img_bits(int vx, int vy, int vz) {
@@ -529,7 +535,6 @@ struct CellWaveBox {
   int nc[3];
   float cunit[3];
   float cthr_in, cthr_out;
-  int ghost_ok;
 };
 
 // Range of sorted atoms of stencil row r (dy = r % 3 - 1, dz = r / 3 - 1) of
@@ -562,7 +567,7 @@ __device__ inline void row_range(const int *__restrict__ cstart, const CellWaveB
   }
 }
 
-constexpr int kSlots = 12;                  // candidates per pass: kSlots * 64
+constexpr int kSlots = 10;                  // candidates per pass: kSlots * 64
 constexpr int kPassCand = kSlots * 64;
 constexpr int kSegs = 2 * kWaveRows;         // row ranges: 9 direct-x + 9 wrapped-x
 
@@ -571,15 +576,17 @@ constexpr int kSegs = 2 * kWaveRows;         // row ranges: 9 direct-x + 9 wrapp
 // candidate entries; STAGED (single-pass cells): each home atom's accepted
 // entries are collected in buf (LDS) and written as whole 4-entry groups;
 // otherwise they are stored one by one and counts carried in cnt_io.
-template <bool STAGED>
-__device__ void cell_wave_pass(const double4 *__restrict__ pos, const uint2 *__restrict__ cpos,
+template <bool STAGED, bool GHOST_OK>
+__device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__restrict__ q,
                                const CellWaveBox &b, const BoxDev *__restrict__ bdev,
                                const uint32_t *cand, int ncand, int c0,
-                               int self_base, int self_j0, int h0, int h1, const unsigned o[3],
+                               int self_base, int self_j0, int h0, int h1, const int3 o,
                                uint32_t *__restrict__ nbr, int *__restrict__ nnbr, size_t stride,
-                               int cap, uint32_t *buf, int *cnt_io, const float4 *home,
-                               int lane, bool first, int &wmax, int &went) {
-  float px[kSlots], py[kSlots], pz[kSlots], tin[kSlots];
+                               int cap, uint32_t *buf, int *cnt_io, float hxl, float hyl,
+                               float hzl, int lane, bool first, int &kept_l, int &wmax,
+                               int &went) {
+  float px[kSlots], py[kSlots], pz[kSlots];
+  float tin[GHOST_OK ? 1 : kSlots];  // per-slot inner threshold when images may lack ghosts
   uint32_t ent[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) {
@@ -587,27 +594,26 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const uint2 *__r
     px[k] = 1e30f;  // empty slot: never accepted, never in the band
     py[k] = 0.0f;
     pz[k] = 0.0f;
-    tin[k] = b.cthr_in;
+    if (!GHOST_OK) tin[k] = b.cthr_in;
     ent[k] = 0u;
     if (t < ncand) {
       const uint32_t e = cand[t];
-      unsigned qx, qy, qz;
-      unpack_coarse(cpos[e & kIndexMask], qx, qy, qz);
-      px[k] = coarse_rel(qx, o[0], b.cunit[0]);
-      py[k] = coarse_rel(qy, o[1], b.cunit[1]);
-      pz[k] = coarse_rel(qz, o[2], b.cunit[2]);
+      const int3 c = load_q3(q, e & kIndexMask);
+      px[k] = q_rel(c.x, o.x, b.cunit[0]);
+      py[k] = q_rel(c.y, o.y, b.cunit[1]);
+      pz[k] = q_rel(c.z, o.z, b.cunit[2]);
       // images whose ghost may not exist always take the exact path
-      if ((e >> 26) != kNoImage && !b.ghost_ok) tin[k] = -1.0f;
+      if (!GHOST_OK && (e >> 26) != kNoImage) tin[k] = -1.0f;
       ent[k] = e;
     }
   }
   // lane-private scratch word for lanes that accept nothing (STAGED)
   const int dummy_at = kPassCand + 4 + lane;
   for (int h = h0; h < h1; ++h) {
-    // home atom coordinates from LDS: a scalar load here would make every
-    // LDS wait also wait for it
-    const float4 hc = home[h - h0];
-    const float hx = hc.x, hy = hc.y, hz = hc.z;
+    // home atom coordinates: lane (h - h0) holds them
+    const float hx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hxl), h - h0));
+    const float hy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hyl), h - h0));
+    const float hz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hzl), h - h0));
     // position of the home atom itself among the candidates
     const int sp = self_base + (h - self_j0) - c0;
     const int kself = sp >= 0 && sp < kPassCand ? sp >> 6 : -1;
@@ -634,10 +640,10 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const uint2 *__r
     for (int g = 0; g < kSlots; g += 3) {
       if (g * 64 >= ncand) continue;  // uniform; keeps the loops unrolled
 #pragma unroll
-      for (int k = g; k < g + 3; ++k) {
+      for (int k = g; k < (g + 3 < kSlots ? g + 3 : kSlots); ++k) {
         const float dx = px[k] - hx, dy = py[k] - hy, dz = pz[k] - hz;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const unsigned long long m_in = __builtin_amdgcn_ballot_w64(r2 < tin[k]);
+        const unsigned long long m_in = __builtin_amdgcn_ballot_w64(r2 < (GHOST_OK ? b.cthr_in : tin[GHOST_OK ? 0 : k]));
         const unsigned long long m_out = __builtin_amdgcn_ballot_w64(r2 < b.cthr_out);
         anyband |= m_out & ~m_in;  // the home atom itself is in m_in
         append(k == kself ? m_in & ~selfbit : m_in, ent[k]);
@@ -656,7 +662,7 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const uint2 *__r
         if (k * 64 >= ncand) continue;
         const float dx = px[k] - hx, dy = py[k] - hy, dz = pz[k] - hz;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool band = (r2 < b.cthr_out) & !(r2 < tin[k]);
+        const bool band = (r2 < b.cthr_out) & !(r2 < (GHOST_OK ? b.cthr_in : tin[GHOST_OK ? 0 : k]));
         bool take = false;
         if (band) {
           uint32_t e = ent[k];
@@ -687,7 +693,7 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const uint2 *__r
       if (lane < 3) buf[kept + lane] = (uint32_t)h | (kNoImage << 26);
       for (int g = lane; g * 4 < kept; g += 64)
         reinterpret_cast<u32x4 *>(nbr)[(size_t)g * stride + h] = reinterpret_cast<const u32x4 *>(buf)[g];
-      if (lane == 0) nnbr[h] = kept;
+      kept_l = lane == h - h0 ? kept : kept_l;  // stored per chunk
       wmax = max(wmax, cnt);
       went += kept;
     } else if (lane == 0) {
@@ -696,8 +702,9 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const uint2 *__r
   }
 }
 
+template <bool GHOST_OK>
 __global__ void __launch_bounds__(256)
-    k_nlist_cells(const double4 *__restrict__ pos, const uint2 *__restrict__ cpos, CellWaveBox b,
+    k_nlist_cells(const double4 *__restrict__ pos, const int4 *__restrict__ q, CellWaveBox b,
                   const BoxDev *__restrict__ bdev, const int *__restrict__ cstart,
                   uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
                   const int *__restrict__ flags, int nblk_real, int2 *__restrict__ wstat) {
@@ -722,7 +729,6 @@ __global__ void __launch_bounds__(256)
   // 64 scratch words
   __shared__ __attribute__((aligned(16))) uint32_t lds[4][kPassCand + 4 + 64];
   __shared__ int cnt_sh[4][kMaxHomeCell];
-  __shared__ float4 home_sh[4][kMaxHomeCell];  // home atoms relative to the first
   uint32_t *cand = lds[wv];
   // segment table: flattened position of each row range
   int total = 0, self_base = 0, self_j0 = 0;
@@ -736,17 +742,19 @@ __global__ void __launch_bounds__(256)
     }
     total += len;
   }
-  unsigned o[3];
-  unpack_coarse(cpos[h0], o[0], o[1], o[2]);
+  const int3 o = load_q3(q, h0);  // coordinates are relative to the first home atom
   int wmax = 0, went = 0;
   const bool staged = total <= kPassCand;
   for (int h = h0; h < h1; h += kMaxHomeCell) {
     const int he = min(h1, h + kMaxHomeCell);
+    // lane l holds home atom h + l (relative coordinates) and its kept count
+    float hxl = 0.0f, hyl = 0.0f, hzl = 0.0f;
+    int kept_l = 0;
     if (h + lane < he) {
-      unsigned qx, qy, qz;
-      unpack_coarse(cpos[h + lane], qx, qy, qz);
-      home_sh[wv][lane] = make_float4(coarse_rel(qx, o[0], b.cunit[0]), coarse_rel(qy, o[1], b.cunit[1]),
-                                      coarse_rel(qz, o[2], b.cunit[2]), 0.0f);
+      const int3 c = load_q3(q, h + lane);
+      hxl = q_rel(c.x, o.x, b.cunit[0]);
+      hyl = q_rel(c.y, o.y, b.cunit[1]);
+      hzl = q_rel(c.z, o.z, b.cunit[2]);
     }
     for (int c0 = 0; c0 < total; c0 += kPassCand) {
       // gather this pass's candidate entries into LDS
@@ -763,14 +771,15 @@ __global__ void __launch_bounds__(256)
       }
       const int ncand = min(kPassCand, total - c0);
       if (staged)
-        cell_wave_pass<true>(pos, cpos, b, bdev, cand, ncand, c0, self_base, self_j0, h, he, o, nbr,
-                             nnbr, (size_t)stride, cap, cand, cnt_sh[wv], home_sh[wv], lane, c0 == 0,
-                             wmax, went);
+        cell_wave_pass<true, GHOST_OK>(pos, q, b, bdev, cand, ncand, c0, self_base, self_j0, h, he, o, nbr,
+                             nnbr, (size_t)stride, cap, cand, cnt_sh[wv], hxl, hyl, hzl, lane,
+                             c0 == 0, kept_l, wmax, went);
       else
-        cell_wave_pass<false>(pos, cpos, b, bdev, cand, ncand, c0, self_base, self_j0, h, he, o, nbr,
-                              nnbr, (size_t)stride, cap, nullptr, cnt_sh[wv], home_sh[wv], lane,
-                              c0 == 0, wmax, went);
+        cell_wave_pass<false, GHOST_OK>(pos, q, b, bdev, cand, ncand, c0, self_base, self_j0, h, he, o, nbr,
+                              nnbr, (size_t)stride, cap, nullptr, cnt_sh[wv], hxl, hyl, hzl,
+                              lane, c0 == 0, kept_l, wmax, went);
     }
+    if (staged && h + lane < he) nnbr[h + lane] = kept_l;
     if (!staged) {
       // pad the last group with the atom itself, store lengths
       for (int hh = h; hh < he; ++hh) {
@@ -876,7 +885,7 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) 
     // order is copied back instead of swapping buffer pointers.
     k_copy_back<<<nblk(n, 256), 256, 0, st>>>(n, s.pos_alt, s.pos, s.vel_alt, s.vel, s.acc_alt,
                                               s.acc, s.tag_alt, s.tag, s.id_alt, s.id,
-                                              s.write_q ? s.q : nullptr, s.cpos, b, s.flags);
+                                              (s.write_q || b.coarse) ? s.q : nullptr, b, s.flags);
   }
   if (ghosts) {
     k_ghost_bin<<<nblk(s.ng, 256), 256, 0, st>>>(s.pos, s.pos_ref, n, s.ng, b, s.cell + n,
@@ -895,10 +904,13 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) 
     }
     cb.cthr_in = b.cthr_in;
     cb.cthr_out = b.cthr_out;
-    cb.ghost_ok = b.ghost_ok;
     const int real = nblk(nc, 4), grid = (real + 7) / 8 * 8;
-    k_nlist_cells<<<grid, 256, 0, st>>>(s.pos, s.cpos, cb, s.box_dev, s.cell_start, s.nbr, s.nnbr,
-                                        s.stride, s.cap, s.flags, real, s.wstat);
+    if (b.ghost_ok)
+      k_nlist_cells<true><<<grid, 256, 0, st>>>(s.pos, s.q, cb, s.box_dev, s.cell_start, s.nbr,
+                                                s.nnbr, s.stride, s.cap, s.flags, real, s.wstat);
+    else
+      k_nlist_cells<false><<<grid, 256, 0, st>>>(s.pos, s.q, cb, s.box_dev, s.cell_start, s.nbr,
+                                                 s.nnbr, s.stride, s.cap, s.flags, real, s.wstat);
     nw = nc;
   } else if (n > 0) {
     k_nlist<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell_start, s.gcell_start,
