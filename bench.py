@@ -109,6 +109,47 @@ def cpu_baseline(s, budget_s):
                 cpu=platform.processor() or platform.machine(), nproc=os.cpu_count())
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lj_f32fx_lj2m_traffic.txt")
+
+
+def committed_traffic():
+    """HBM bytes per launch of the force kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled),
+    or None when there is none."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def traffic_probe(a):
+    """Workload for the rocprofv3 --pmc traffic runs: a few force launches on
+    the melted 2M-atom configuration, then a plain copy of known size (the
+    calibration kernel: reads and writes COPY_BYTES each)."""
+    import torch
+
+    s = workload(a.ncell)
+    with D.Context.from_sysdef(s, precision=D.CV2_PREC_FP32_FIXED) as c:
+        c.compute_forces()
+        c.run_langevin(50, s["dt"], s["gamma"], s["kbt"], 7)
+        for _ in range(5):
+            c.compute_forces()
+        c.synchronize()
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 2.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"traffic probe done: {len(s['pos'])} atoms, copy {COPY_BYTES} B")
+
+
+COPY_BYTES = 256 << 20
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,12 +159,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run only the force kernel and a calibration copy (for rocprofv3 --pmc)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); multi-GPU sharding is a design "
               "note in DESIGN.md, not a measured configuration", file=sys.stderr)
         sys.exit(2)
 
+    if a.traffic_probe:
+        traffic_probe(a)
+        return
     s = workload(a.ncell)
     n = len(s["pos"])
     log(f"workload: {n} atoms, L={s['hi'][0]:.4f}")
@@ -163,7 +209,7 @@ def main():
             "natoms": n,
             "precision": "fp32+fixed64",
             "mean_list_length": nl,
-            "list": "full (both directions), transposed ELL",
+            "list": "full (both directions), grouped ELL (4-entry groups)",
             "rebuilds_in_timed_steps": r32["rebuilds"],
         },
         "fp64_gpu": None if r64 is None else {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
@@ -174,13 +220,22 @@ def main():
         "phase_ms_per_step": {k: tm[k] / a.steps for k in ("force_ms", "build_ms", "integrate_ms")},
         "temperature_end": r32["temperature"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(),
                      "kernel": "k_lj_f32fx", "bytes_per_launch": b_force,
                      "avg_launch_ms": t_force * 1e3},
         "cpu_baseline": None,
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s, a.cpu_seconds)
+        # the compiled reference logs to stdout; keep stdout for the JSON line
+        sys.stdout.flush()
+        saved = os.dup(1)
+        os.dup2(2, 1)
+        try:
+            out["cpu_baseline"] = cpu_baseline(s, a.cpu_seconds)
+        finally:
+            sys.stdout.flush()
+            os.dup2(saved, 1)
+            os.close(saved)
         log(f"cpu: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 

@@ -133,11 +133,11 @@ void free_state(DevState &s) {
   s = DevState{};
 }
 
-// Per-wavefront statistics of the list build: one entry per cell (wave-
-// per-cell build) or per wavefront of owned atoms (per-atom build).
+// Per-block statistics of the list build: one entry per block of 4 cells
+// (wave-per-cell build) or of 256 owned atoms (per-atom build).
 int ensure_wstat(cv2_ctx *ctx) {
   DevState &s = ctx->s;
-  const int64_t need = std::max<int64_t>((int64_t)s.ncell, (ctx->n + 255) / 256 * 4) + 8;
+  const int64_t need = std::max<int64_t>(((int64_t)s.ncell + 3) / 4, (ctx->n + 255) / 256) + 8;
   if (need > s.wstat_cap) {
     HIPCHK(dalloc(&s.wstat, need));
     s.wstat_cap = need;

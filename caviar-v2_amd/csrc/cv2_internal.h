@@ -123,7 +123,7 @@ struct DevState {
   double *acc, *acc_alt;
   int *tag, *tag_alt, *id, *id_alt;
   BoxDev *box_dev;              // device copy of the box constants
-  int2 *wstat;                  // per-wavefront build statistics (longest list, entries)
+  int2 *wstat;                  // per-block build statistics (longest list, entries)
   int64_t wstat_cap;
   int *cell, *slot, *perm, *order;  // per owned atom during a build
   int *cell_count, *cell_start; // ncell (+1)

@@ -390,7 +390,7 @@ __device__ inline void scan_range(const double4 *__restrict__ pos, int j0, int j
   }
 }
 
-__device__ inline void wave_stat(int2 *wstat, int w, int cnt, int kept);
+__device__ inline void block_stat(int2 *bstat, int cnt, int kept);
 __device__ void nlist_atom(const double4 *__restrict__ pos, int n, const BoxDev &b,
                            const int *__restrict__ cstart, const int *__restrict__ gstart,
                            const int *__restrict__ gitems, uint32_t *__restrict__ nbr,
@@ -406,17 +406,24 @@ __global__ void __launch_bounds__(256)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int cnt = 0, kept = 0;
   if (i < n) nlist_atom(pos, n, b, cstart, gstart, gitems, nbr, nnbr, stride, cap, host_halo, i, cnt, kept);
-  wave_stat(wstat, i >> 6, cnt, kept);
+  block_stat(wstat, cnt, kept);
 }
 
-// Per-wavefront build statistics (longest list, entries kept), reduced by
-// k_build_done: no same-address atomics from thousands of wavefronts.
-__device__ inline void wave_stat(int2 *wstat, int w, int cnt, int kept) {
+// Per-block build statistics (longest list, entries kept), reduced by
+// k_build_done: no same-address atomics from thousands of blocks.
+__device__ inline void block_stat(int2 *bstat, int cnt, int kept) {
+  __shared__ int2 sh[4];
   for (int o = 32; o > 0; o >>= 1) {
     cnt = max(cnt, __shfl_xor(cnt, o, 64));
     kept += __shfl_xor(kept, o, 64);
   }
-  if ((threadIdx.x & 63) == 0) wstat[w] = make_int2(cnt, kept);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = make_int2(cnt, kept);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int2 t = sh[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) t = make_int2(max(t.x, sh[w].x), t.y + sh[w].y);
+    bstat[blockIdx.x] = t;
+  }
 }
 
 __device__ void nlist_atom(const double4 *__restrict__ pos, int n, const BoxDev &b,
@@ -702,33 +709,18 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
   }
 }
 
+// One home cell c (wavefront wv of its block); returns (longest list,
+// entries kept).
 template <bool GHOST_OK>
-__global__ void __launch_bounds__(256)
-    k_nlist_cells(const double4 *__restrict__ pos, const int4 *__restrict__ q, CellWaveBox b,
-                  const BoxDev *__restrict__ bdev, const int *__restrict__ cstart,
-                  uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
-                  const int *__restrict__ flags, int nblk_real, int2 *__restrict__ wstat) {
-  if (flags[F_REBUILD] == 0) return;
-  // consecutive cells (which share most candidates) on the same XCD
-  const int per = gridDim.x >> 3;
-  const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  if (blk >= nblk_real) return;
-  // the home cell is uniform across the wavefront
-  const int wv = (int)(threadIdx.x >> 6);
-  const int c = __builtin_amdgcn_readfirstlane(blk * 4 + wv);
-  const int lane = threadIdx.x & 63;
+__device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restrict__ q,
+                          const CellWaveBox &b, const BoxDev *__restrict__ bdev,
+                          const int *__restrict__ cstart, uint32_t *__restrict__ nbr,
+                          int *__restrict__ nnbr, int stride, int cap, int c, int wv, int lane,
+                          uint32_t (*lds)[kPassCand + 4 + 64], int (*cnt_sh)[kMaxHomeCell]) {
   const int nx = b.nc[0], ny = b.nc[1];
-  if (c >= nx * ny * b.nc[2]) return;
   const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
   const int h0 = cstart[c], h1 = cstart[c + 1];
-  if (h0 == h1) {
-    if (lane == 0) wstat[c] = make_int2(0, 0);
-    return;
-  }
-  // candidates of a pass, then (staged) one home atom's entries + 3 pads;
-  // 64 scratch words
-  __shared__ __attribute__((aligned(16))) uint32_t lds[4][kPassCand + 4 + 64];
-  __shared__ int cnt_sh[4][kMaxHomeCell];
+  if (h0 == h1) return make_int2(0, 0);
   uint32_t *cand = lds[wv];
   // segment table: flattened position of each row range
   int total = 0, self_base = 0, self_j0 = 0;
@@ -795,16 +787,48 @@ __global__ void __launch_bounds__(256)
       }
     }
   }
-  if (lane == 0) wstat[c] = make_int2(wmax, went);
+  return make_int2(wmax, went);
 }
 
-// Close a build: reduce the per-wavefront statistics, clear the request.
+template <bool GHOST_OK>
 __global__ void __launch_bounds__(256)
+    k_nlist_cells(const double4 *__restrict__ pos, const int4 *__restrict__ q, CellWaveBox b,
+                  const BoxDev *__restrict__ bdev, const int *__restrict__ cstart,
+                  uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
+                  const int *__restrict__ flags, int nblk_real, int2 *__restrict__ bstat) {
+  if (flags[F_REBUILD] == 0) return;
+  // consecutive cells (which share most candidates) on the same XCD
+  const int per = gridDim.x >> 3;
+  const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (blk >= nblk_real) return;
+  // candidates of a pass, then (staged) one home atom's entries + 3 pads;
+  // 64 scratch words
+  __shared__ __attribute__((aligned(16))) uint32_t lds[4][kPassCand + 4 + 64];
+  __shared__ int cnt_sh[4][kMaxHomeCell];
+  __shared__ int2 st_sh[4];
+  const int wv = (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  // the home cell is uniform across the wavefront
+  const int c = __builtin_amdgcn_readfirstlane(blk * 4 + wv);
+  int2 st = make_int2(0, 0);
+  if (c < b.nc[0] * b.nc[1] * b.nc[2])
+    st = cell_wave<GHOST_OK>(pos, q, b, bdev, cstart, nbr, nnbr, stride, cap, c, wv, lane, lds, cnt_sh);
+  if (lane == 0) st_sh[wv] = st;
+  __syncthreads();
+  if (threadIdx.x == 0) {  // per-block statistics, reduced by k_build_done
+    int2 t = st_sh[0];
+    for (int w = 1; w < 4; ++w) t = make_int2(max(t.x, st_sh[w].x), t.y + st_sh[w].y);
+    bstat[blk] = t;
+  }
+}
+
+// Close a build: reduce the per-block statistics, clear the request.
+__global__ void __launch_bounds__(1024)
     k_build_done(int *__restrict__ flags, const int2 *__restrict__ wstat, int nw, int cap) {
   if (flags[F_REBUILD] == 0) return;
-  __shared__ int smax[256], ssum[256];
+  __shared__ int smax[1024], ssum[1024];
   int m = 0, t = 0;
-  for (int k = threadIdx.x; k < nw; k += 256) {
+  for (int k = threadIdx.x; k < nw; k += 1024) {
     const int2 v = wstat[k];
     m = max(m, v.x);
     t += v.y;
@@ -812,7 +836,7 @@ __global__ void __launch_bounds__(256)
   smax[threadIdx.x] = m;
   ssum[threadIdx.x] = t;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = 512; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
       smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + o]);
       ssum[threadIdx.x] += ssum[threadIdx.x + o];
@@ -911,14 +935,14 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) 
     else
       k_nlist_cells<false><<<grid, 256, 0, st>>>(s.pos, s.q, cb, s.box_dev, s.cell_start, s.nbr,
                                                  s.nnbr, s.stride, s.cap, s.flags, real, s.wstat);
-    nw = nc;
+    nw = real;
   } else if (n > 0) {
     k_nlist<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell_start, s.gcell_start,
                                           s.gcell_items, s.nbr, s.nnbr, s.stride, s.cap, s.flags,
                                           ghosts ? 1 : 0, s.wstat);
-    nw = nblk(n, 256) * 4;
+    nw = nblk(n, 256);
   }
-  k_build_done<<<1, 256, 0, st>>>(s.flags, s.wstat, nw, s.cap);
+  k_build_done<<<1, 1024, 0, st>>>(s.flags, s.wstat, nw, s.cap);
 }
 
 }  // namespace cv2
