@@ -109,6 +109,38 @@ def cpu_baseline(s, budget_s):
                 cpu=platform.processor() or platform.machine(), nproc=os.cpu_count())
 
 
+def run_ensemble(steps, seed=1004):
+    """Batched ensemble: 512 independent LJ systems (N log-uniform in
+    [500, 20000], rho in [0.60, 0.90], T* in [0.8, 2.0]; rc 2.5, skin 0.3),
+    FP32+fixed-point, velocity Verlet dt=0.005; launch shapes and rebuild
+    policy from the autotuner; each step one captured graph."""
+    from cv2md.systems import ensemble_systems
+
+    systems = ensemble_systems(512, seed=seed)
+    natoms = sum(len(x["pos"]) for x in systems)
+    with D.Ensemble.from_systems(systems, rc=2.5, nl_cutoff=2.5, nl_coef=1.12) as e:
+        t0 = time.perf_counter()
+        tuned = e.tune(0.005, trial_steps=20)
+        t_tune = time.perf_counter() - t0
+        e.run_vv(20, 0.005)  # warm-up after the tuner's restore
+        st0 = e.stats()
+        t0 = time.perf_counter()
+        e.run_vv(steps, 0.005)
+        el = time.perf_counter() - t0
+        st = e.stats()
+    return dict(value=natoms * steps / el, unit="atom-timesteps/s", ms_per_step=1e3 * el / steps,
+                steps=steps, systems=len(systems), natoms=natoms,
+                natoms_min=min(len(x["pos"]) for x in systems),
+                natoms_max=max(len(x["pos"]) for x in systems),
+                tuning=tuned, tune_seconds=t_tune,
+                system_builds=st["system_builds"] - st0["system_builds"],
+                build_steps=st["build_steps"] - st0["build_steps"],
+                mean_list_length=st["list_entries"] / natoms, graph_nodes=st["graph_nodes"],
+                config="512 LJ systems, N log-uniform in [500, 20000], rho* U[0.60, 0.90], T* U[0.8, 2.0], "
+                       "random/jittered FCC, rc=2.5, skin 0.3, FP32 + 64-bit fixed point, velocity Verlet "
+                       "dt=0.005, one captured HIP graph per step")
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lj_f32fx_lj2m_traffic.txt")
 
 
@@ -159,6 +191,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-steps", type=int, default=2000)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only the force kernel and a calibration copy (for rocprofv3 --pmc)")
     a = ap.parse_args()
@@ -224,7 +258,12 @@ def main():
                      "kernel": "k_lj_f32fx", "bytes_per_launch": b_force,
                      "avg_launch_ms": t_force * 1e3},
         "cpu_baseline": None,
+        "ensemble": None,
     }
+    if not a.no_ensemble:
+        out["ensemble"] = run_ensemble(a.ens_steps)
+        log(f"ensemble: {out['ensemble']['ms_per_step']:.4f} ms/step, {out['ensemble']['value']:.3e} "
+            f"atom-steps/s, tuning {out['ensemble']['tuning']}")
     if not a.no_cpu:
         # the compiled reference logs to stdout; keep stdout for the JSON line
         sys.stdout.flush()

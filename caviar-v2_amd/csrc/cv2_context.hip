@@ -302,6 +302,65 @@ int alloc_positions(cv2_ctx *ctx, int64_t ntot) {
 
 }  // namespace
 
+namespace cv2 {
+// Pair tables of the LJ force (lj.cpp:475-484), its energy and the FP32
+// path's fixed-point overflow bound, for ntypes x ntypes row-major inputs.
+bool build_lj_table(LJDev &L, int ntypes, const double *eps, const double *sigma,
+                    const double *cutoff_list, double cutoff, const double *mass, std::string &err) {
+  L = LJDev{};
+  L.nt = ntypes;
+  for (int t = 0; t < ntypes; ++t) {
+    if (!(mass[t] > 0.0)) {
+      err = "masses must be positive";
+      return false;
+    }
+    L.mass[t] = mass[t];
+    L.mass_inv[t] = 1.0 / mass[t];
+  }
+  for (int k = 0; k < ntypes * ntypes; ++k) {
+    double rc = cutoff;
+    if (cutoff_list) {
+      if (cutoff_list[k] < 0.0) {
+        err = "Cutoff list values must be non-negative.";
+        return false;
+      }
+      rc = cutoff_list[k];
+    }
+    const double c = rc * rc;
+    const double rcinv = 1.0 / c;
+    const double s2 = sigma[k] * sigma[k];
+    const double rho_c2 = s2 * rcinv, rho_c6 = rho_c2 * rho_c2 * rho_c2, rho_c12 = rho_c6 * rho_c6;
+    L.eps4[k] = 4.0 * eps[k];
+    L.sig2[k] = s2;
+    L.rc2[k] = eps[k] == 0.0 ? -1.0 : c;  // eps == 0 pairs are skipped (lj.cpp:470-472)
+    L.fshift[k] = 12.0 * rho_c12 * rcinv - 6.0 * rho_c6 * rcinv;
+    L.ecut[k] = (6.0 * rho_c12 - 3.0 * rho_c6) * rcinv;
+    L.ecut2[k] = 7.0 * rho_c12 - 4.0 * rho_c6;
+    L.eps4f[k] = (float)L.eps4[k];
+    L.sig2f[k] = (float)L.sig2[k];
+    L.rc2f[k] = (float)L.rc2[k];
+    L.fshiftf[k] = (float)L.fshift[k];
+    // FP32 path: largest r^2 at which |F(r)| = |forceCoef| r reaches 0.9 * 2^19
+    // (bisection on the repulsive branch, where |F| falls with r)
+    L.r2ovf[k] = 0.0;
+    if (eps[k] != 0.0 && sigma[k] > 0.0) {
+      auto force = [&](double r) {
+        const double r2 = r * r, r2i = 1.0 / r2, s6 = std::pow(s2 * r2i, 3);
+        return std::fabs(L.eps4[k] * (r2i * (s6 * (-12.0 * s6 + 6.0)) + L.fshift[k])) * r;
+      };
+      double a = 1e-3 * sigma[k], z = sigma[k];
+      for (int it = 0; it < 200; ++it) {
+        const double m = 0.5 * (a + z);
+        (force(m) > 0.9 * 524288.0 ? a : z) = m;
+      }
+      L.r2ovf[k] = z * z;
+    }
+  }
+  return true;
+}
+
+}  // namespace cv2
+
 extern "C" {
 
 int cv2_create(cv2_ctx **out, int device) {
@@ -371,49 +430,10 @@ int cv2_set_lj(cv2_ctx *ctx, int ntypes, const double *eps, const double *sigma,
   if (!ctx || !eps || !sigma || !mass) return fail(ctx, CV2_ERR_BAD_ARG, "null argument");
   if (ntypes < 1 || ntypes > kMaxTypes) return fail(ctx, CV2_ERR_BAD_ARG, "ntypes must be 1..8");
   if (cutoff < 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "Force field cutoff must be non-negative.");
-  LJDev &L = ctx->lj;
-  L = LJDev{};
-  L.nt = ntypes;
-  for (int t = 0; t < ntypes; ++t) {
-    if (!(mass[t] > 0.0)) return fail(ctx, CV2_ERR_BAD_ARG, "masses must be positive");
-    L.mass[t] = mass[t];
-    L.mass_inv[t] = 1.0 / mass[t];
-  }
-  for (int k = 0; k < ntypes * ntypes; ++k) {
-    double rc = cutoff;
-    if (cutoff_list) {
-      if (cutoff_list[k] < 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "Cutoff list values must be non-negative.");
-      rc = cutoff_list[k];
-    }
-    const double c = rc * rc;
-    const double rcinv = 1.0 / c;
-    const double s2 = sigma[k] * sigma[k];
-    const double rho_c2 = s2 * rcinv, rho_c6 = rho_c2 * rho_c2 * rho_c2, rho_c12 = rho_c6 * rho_c6;
-    L.eps4[k] = 4.0 * eps[k];
-    L.sig2[k] = s2;
-    L.rc2[k] = eps[k] == 0.0 ? -1.0 : c;  // eps == 0 pairs are skipped (lj.cpp:470-472)
-    L.fshift[k] = 12.0 * rho_c12 * rcinv - 6.0 * rho_c6 * rcinv;
-    L.ecut[k] = (6.0 * rho_c12 - 3.0 * rho_c6) * rcinv;
-    L.ecut2[k] = 7.0 * rho_c12 - 4.0 * rho_c6;
-    L.eps4f[k] = (float)L.eps4[k];
-    L.sig2f[k] = (float)L.sig2[k];
-    L.rc2f[k] = (float)L.rc2[k];
-    L.fshiftf[k] = (float)L.fshift[k];
-    // FP32 path: largest r^2 at which |F(r)| = |forceCoef| r reaches 0.9 * 2^19
-    // (bisection on the repulsive branch, where |F| falls with r)
-    L.r2ovf[k] = 0.0;
-    if (eps[k] != 0.0 && sigma[k] > 0.0) {
-      auto force = [&](double r) {
-        const double r2 = r * r, r2i = 1.0 / r2, s6 = std::pow(s2 * r2i, 3);
-        return std::fabs(L.eps4[k] * (r2i * (s6 * (-12.0 * s6 + 6.0)) + L.fshift[k])) * r;
-      };
-      double a = 1e-3 * sigma[k], z = sigma[k];
-      for (int it = 0; it < 200; ++it) {
-        const double m = 0.5 * (a + z);
-        (force(m) > 0.9 * 524288.0 ? a : z) = m;
-      }
-      L.r2ovf[k] = z * z;
-    }
+  {
+    std::string err;
+    if (!build_lj_table(ctx->lj, ntypes, eps, sigma, cutoff_list, cutoff, mass, err))
+      return fail(ctx, CV2_ERR_BAD_ARG, err);
   }
   ctx->lj_set = true;
   ctx->need_build = true;

@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace cv2 {
 
 constexpr int kMaxTypes = 8;
@@ -139,6 +141,24 @@ struct DevState {
   bool write_q;                // keep q current (FP32 pair kernel in use)
 };
 
+// Ensemble of independent periodic systems sharing one set of arrays: the
+// atoms of a system are contiguous, and so are its cells (global cell index
+// = cell_off[s] + local index).  A system is rebuilt when its flag is set.
+struct EnsDev {
+  int nsys;
+  const int *sys;       // per atom: system
+  const int *cell_sys;  // per cell: system
+  const int *cell_off;  // per system: first global cell
+  const int *atom_off;  // per system: first atom (nsys + 1 entries)
+  const BoxDev *box;    // per system: box, cell grid, fixed-point scales
+  int *sflag;           // per system: rebuild requested
+  int *srebuilds;       // per system: builds done
+};
+
+// Host: LJ pair tables (false + message on invalid input).
+bool build_lj_table(LJDev &L, int ntypes, const double *eps, const double *sigma,
+                    const double *cutoff_list, double cutoff, const double *mass, std::string &err);
+
 // ---- launchers (defined in the .hip files) ----
 void launch_pack_q(const DevState &s, const BoxDev &b, hipStream_t st);
 void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo);
@@ -156,5 +176,19 @@ void launch_update(DevState &s, const BoxDev &b, const UpdateArgs &u, hipStream_
 void launch_check_disp(DevState &s, const BoxDev &b, hipStream_t st);
 void launch_thermo(DevState &s, const LJDev *lj_dev, hipStream_t st);
 void launch_set_flag(int *flags, int which, int value, hipStream_t st);
+
+// Ensemble: list build of the flagged systems (policy_all: a request from
+// any system rebuilds all), FP32+fixed-point forces (eatom: per-atom
+// fixed-point virial and energy when energy is set), velocity Verlet with
+// the per-system rebuild test, per-system (ke, pe, virial).
+void launch_build_ens(DevState &s, const EnsDev &e, int ncell, int block, bool policy_all,
+                      hipStream_t st);
+void launch_forces_ens(DevState &s, const EnsDev &e, const LJDev *lj_dev, int block, bool energy,
+                       long long *eatom, hipStream_t st);
+void launch_update_ens(DevState &s, const EnsDev &e, bool finish, bool start, double dt,
+                       double thresh_sq, hipStream_t st);
+void launch_thermo_ens(const DevState &s, const EnsDev &e, const LJDev *lj_dev,
+                       const long long *eatom, double *out, hipStream_t st);
+void launch_fill_int(int *p, int n, int value, hipStream_t st);
 
 }  // namespace cv2

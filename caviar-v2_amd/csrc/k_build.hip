@@ -870,6 +870,185 @@ __global__ void k_pack_q(const double4 *__restrict__ pos, int4 *__restrict__ q, 
 
 __global__ void k_set_flag(int *flags, int which, int value) { flags[which] = value; }
 
+__global__ void k_fill_int(int *p, int n, int value) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+// ---- ensemble of independent periodic systems ----
+//
+// The same steps as the single-box build, restricted to the systems whose
+// flag is set; each system has its own box and cell grid (per-atom list
+// build over cells at least ce/2 wide).
+
+// "all" rebuild policy: a request from any system rebuilds every system.
+__global__ void k_ens_policy(EnsDev e, const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < e.nsys; s += gridDim.x * blockDim.x)
+    e.sflag[s] = 1;
+}
+
+__global__ void k_ens_begin(EnsDev e, int *__restrict__ cell_count, int ncell,
+                            const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncell && e.sflag[e.cell_sys[c]]) cell_count[c] = 0;
+}
+
+// Wrap and bin the atoms of flagged systems; runs of equal cells in
+// consecutive lanes take their slots with one atomic add.
+__global__ void k_ens_wrap_bin(double4 *__restrict__ pos, int n, EnsDev e, int *__restrict__ cell,
+                               int *__restrict__ slot, int *__restrict__ cell_count,
+                               const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int ci = -1;
+  const int sy = i < n ? e.sys[i] : 0;
+  const bool valid = i < n && e.sflag[sy] != 0;
+  if (valid) {
+    const BoxDev &b = e.box[sy];
+    double4 p = pos[i];
+    double *c[3] = {&p.x, &p.y, &p.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (*c[d] < b.lo[d])
+        *c[d] += b.L[d];
+      else if (*c[d] > b.hi[d])
+        *c[d] -= b.L[d];
+    }
+    pos[i] = p;
+    ci = e.cell_off[sy] + cell_of(p, b);
+    cell[i] = ci;
+  }
+  const int prev = __shfl_up(ci, 1, 64);
+  const bool lead = valid && (lane == 0 || prev != ci);
+  const unsigned long long lm = __ballot(lead);
+  const unsigned long long ends = __ballot(lead || !valid);  // a run ends where the next starts
+  const unsigned long long upto = lane == 63 ? lm : lm & ((2ull << lane) - 1ull);
+  const int leader = upto ? 63 - __clzll(upto) : 0;
+  int base = 0;
+  if (lead) {
+    const unsigned long long above = lane == 63 ? 0ull : ends & ~((2ull << lane) - 1ull);
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    base = atomicAdd(&cell_count[ci], next - lane);
+  }
+  base = __shfl(base, leader, 64);
+  if (valid) slot[i] = base + lane - leader;
+}
+
+__global__ void k_ens_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
+                              const int *__restrict__ start, int *__restrict__ perm, int n, EnsDev e,
+                              const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !e.sflag[e.sys[i]]) return;
+  perm[start[cell[i]] + slot[i]] = i;
+}
+
+__global__ void k_ens_rank(const int *__restrict__ cell, const int *__restrict__ start,
+                           const int *__restrict__ perm, const int *__restrict__ tag,
+                           int *__restrict__ order, int n, EnsDev e, const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !e.sflag[e.sys[i]]) return;
+  const int c = cell[i];
+  const int a = start[c], end = start[c + 1];
+  const int t = tag[i];
+  int rank = 0;
+  for (int k = a; k < end; ++k) rank += tag[perm[k]] < t;
+  order[a + rank] = i;
+}
+
+// Gather the flagged systems' atoms into their new order (k and order[k]
+// are in the same system: atoms never change system).
+__global__ void k_ens_permute(const int *__restrict__ order, int n, EnsDev e,
+                              const double4 *__restrict__ pos, double4 *__restrict__ pos_out,
+                              double4 *__restrict__ pos_ref, const double *__restrict__ vel,
+                              double *__restrict__ vel_out, const double *__restrict__ acc,
+                              double *__restrict__ acc_out, const int *__restrict__ tag,
+                              int *__restrict__ tag_out, const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || !e.sflag[e.sys[k]]) return;
+  const int o = order[k];
+  const double4 p = pos[o];
+  pos_out[k] = p;
+  pos_ref[k] = p;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vel_out[d * n + k] = vel[d * n + o];
+    acc_out[d * n + k] = acc[d * n + o];
+  }
+  tag_out[k] = tag[o];
+}
+
+__global__ void k_ens_copy_back(int n, EnsDev e, const double4 *__restrict__ pos_alt,
+                                double4 *__restrict__ pos, const double *__restrict__ vel_alt,
+                                double *__restrict__ vel, const double *__restrict__ acc_alt,
+                                double *__restrict__ acc, const int *__restrict__ tag_alt,
+                                int *__restrict__ tag, int4 *__restrict__ q,
+                                const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int sy = e.sys[k];
+  if (!e.sflag[sy]) return;
+  const double4 p = pos_alt[k];
+  pos[k] = p;
+  q[k] = qpack(p, e.box[sy]);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vel[d * n + k] = vel_alt[d * n + k];
+    acc[d * n + k] = acc_alt[d * n + k];
+  }
+  tag[k] = tag_alt[k];
+}
+
+// Per-atom Verlet list of the flagged systems (the FP64 predicate of the
+// single-box per-atom build, with each system's box and cell grid).
+__global__ void k_ens_nlist(const double4 *__restrict__ pos, int n, EnsDev e,
+                            const int *__restrict__ cstart, uint32_t *__restrict__ nbr,
+                            int *__restrict__ nnbr, int stride, int cap,
+                            const int *__restrict__ flags, int2 *__restrict__ bstat) {
+  if (flags[F_REBUILD] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int cnt = 0, kept = 0;
+  if (i < n) {
+    const int sy = e.sys[i];
+    if (e.sflag[sy])
+      nlist_atom(pos, n, e.box[sy], cstart + e.cell_off[sy], nullptr, nullptr, nbr, nnbr, stride,
+                 cap, 0, i, cnt, kept);
+  }
+  block_stat(bstat, cnt, kept);
+}
+
+// Close an ensemble build: longest list and overflow from the per-block
+// statistics, per-system build counts, clear the requests.
+__global__ void __launch_bounds__(1024)
+    k_ens_done(int *__restrict__ flags, const int2 *__restrict__ bstat, int nw, int cap, EnsDev e) {
+  if (flags[F_REBUILD] == 0) return;
+  __shared__ int smax[1024];
+  int m = 0;
+  for (int k = threadIdx.x; k < nw; k += 1024) m = max(m, bstat[k].x);
+  smax[threadIdx.x] = m;
+  for (int s = threadIdx.x; s < e.nsys; s += 1024) {
+    e.srebuilds[s] += e.sflag[s];
+    e.sflag[s] = 0;
+  }
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    flags[F_MAXNBR] = max(flags[F_MAXNBR], smax[0]);
+    if (smax[0] > cap) flags[F_OVERFLOW] = 1;
+    flags[F_REBUILD] = 0;
+    flags[F_REBUILDS] += 1;
+  }
+}
+
 inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
 void launch_scan(const int *count, int *start, int *tile_sum, int ncell, const int *flags,
@@ -880,6 +1059,32 @@ void launch_scan(const int *count, int *start, int *tile_sum, int ncell, const i
 }
 
 }  // namespace
+
+void launch_fill_int(int *p, int n, int value, hipStream_t st) {
+  if (n > 0) k_fill_int<<<nblk(n, 256), 256, 0, st>>>(p, n, value);
+}
+
+void launch_build_ens(DevState &s, const EnsDev &e, int ncell, int block, bool policy_all,
+                      hipStream_t st) {
+  const int n = s.n;
+  if (n == 0) return;
+  if (policy_all) k_ens_policy<<<nblk(e.nsys, 256), 256, 0, st>>>(e, s.flags);
+  k_ens_begin<<<nblk(ncell, 256), 256, 0, st>>>(e, s.cell_count, ncell, s.flags);
+  k_ens_wrap_bin<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, e, s.cell, s.slot, s.cell_count, s.flags);
+  launch_scan(s.cell_count, s.cell_start, s.tile_sum, ncell, s.flags, st);
+  k_ens_scatter<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.slot, s.cell_start, s.perm, n, e, s.flags);
+  k_ens_rank<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.cell_start, s.perm, s.tag, s.order, n, e,
+                                            s.flags);
+  k_ens_permute<<<nblk(n, 256), 256, 0, st>>>(s.order, n, e, s.pos, s.pos_alt, s.pos_ref, s.vel,
+                                               s.vel_alt, s.acc, s.acc_alt, s.tag, s.tag_alt,
+                                               s.flags);
+  k_ens_copy_back<<<nblk(n, 256), 256, 0, st>>>(n, e, s.pos_alt, s.pos, s.vel_alt, s.vel,
+                                                 s.acc_alt, s.acc, s.tag_alt, s.tag, s.q, s.flags);
+  const int nb = nblk(n, block);
+  k_ens_nlist<<<nb, block, 0, st>>>(s.pos, n, e, s.cell_start, s.nbr, s.nnbr, s.stride, s.cap,
+                                     s.flags, s.wstat);
+  k_ens_done<<<1, 1024, 0, st>>>(s.flags, s.wstat, nb, s.cap, e);
+}
 
 void launch_set_flag(int *flags, int which, int value, hipStream_t st) {
   k_set_flag<<<1, 1, 0, st>>>(flags, which, value);

@@ -187,6 +187,81 @@ __global__ void __launch_bounds__(1024) k_reduce5(const double *__restrict__ par
   }
 }
 
+// Ensemble velocity Verlet: the same half steps as k_update, fixed-point
+// coordinates and rebuild test per system.
+template <bool FINISH, bool START>
+__global__ void __launch_bounds__(kBlock)
+    k_ens_update(double4 *__restrict__ pos, const double4 *__restrict__ pos_ref,
+                 double *__restrict__ vel, const double *__restrict__ acc, int4 *__restrict__ q,
+                 int n, EnsDev e, double dt, double thresh_sq, int *__restrict__ flags) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double hdt = 0.5 * dt;
+  double v[3] = {vel[i], vel[n + i], vel[2 * n + i]};
+  const double a[3] = {acc[i], acc[n + i], acc[2 * n + i]};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) v[d] += a[d] * hdt;
+  if (FINISH && START) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[d] += a[d] * hdt;
+  }
+  if (START) {
+    const int sy = e.sys[i];
+    const BoxDev &b = e.box[sy];
+    double4 x = pos[i];
+    x.x += v[0] * dt;
+    x.y += v[1] * dt;
+    x.z += v[2] * dt;
+    pos[i] = x;
+    q[i] = pack_q(x.x, x.y, x.z, (int)x.w, b.lo, b.qscale);
+    const double4 r = pos_ref[i];
+    const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
+    if (dx * dx + dy * dy + dz * dz > thresh_sq) {
+      atomicOr(&e.sflag[sy], 1);
+      atomicOr(&flags[F_REBUILD], 1);
+    }
+  }
+  vel[i] = v[0];
+  vel[n + i] = v[1];
+  vel[2 * n + i] = v[2];
+}
+
+// Per-system (kinetic energy, potential energy, virial): one block per
+// system, fixed order.
+__global__ void __launch_bounds__(kBlock)
+    k_ens_thermo(const double *__restrict__ vel, int n, EnsDev e, const LJDev *__restrict__ lj,
+                 const long long *__restrict__ eatom, double *__restrict__ out) {
+  __shared__ double sk[kBlock];
+  __shared__ long long sv[kBlock], sp[kBlock];
+  const int sy = blockIdx.x;
+  const int a0 = e.atom_off[sy], a1 = e.atom_off[sy + 1];
+  double k2 = 0.0;
+  long long v = 0, p = 0;
+  for (int i = a0 + threadIdx.x; i < a1; i += kBlock) {
+    const double vx = vel[i], vy = vel[n + i], vz = vel[2 * n + i];
+    k2 += vx * vx + vy * vy + vz * vz;
+    v += eatom[2 * (size_t)i];
+    p += eatom[2 * (size_t)i + 1];
+  }
+  sk[threadIdx.x] = k2;
+  sv[threadIdx.x] = v;
+  sp[threadIdx.x] = p;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sk[threadIdx.x] += sk[threadIdx.x + o];
+      sv[threadIdx.x] += sv[threadIdx.x + o];
+      sp[threadIdx.x] += sp[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[3 * sy] = 0.5 * lj->mass[0] * sk[0];
+    out[3 * sy + 1] = (double)sp[0] / kEnergyScale;
+    out[3 * sy + 2] = (double)sv[0] / kEnergyScale;
+  }
+}
+
 template <bool L, bool F, bool S>
 void update_q(DevState &s, const StepParams &p, bool writeq, hipStream_t st) {
   const int g = nblk(s.n, kBlock);
@@ -224,6 +299,24 @@ void launch_update(DevState &s, const BoxDev &b, const UpdateArgs &u, hipStream_
   CV2_UPD(true, true, false)
   CV2_UPD(true, false, true)
 #undef CV2_UPD
+}
+
+void launch_update_ens(DevState &s, const EnsDev &e, bool finish, bool start, double dt,
+                       double thresh_sq, hipStream_t st) {
+  if (s.n == 0 || (!finish && !start)) return;
+  const int g = nblk(s.n, kBlock);
+  if (finish && start)
+    k_ens_update<true, true><<<g, kBlock, 0, st>>>(s.pos, s.pos_ref, s.vel, s.acc, s.q, s.n, e, dt, thresh_sq, s.flags);
+  else if (finish)
+    k_ens_update<true, false><<<g, kBlock, 0, st>>>(s.pos, s.pos_ref, s.vel, s.acc, s.q, s.n, e, dt, thresh_sq, s.flags);
+  else
+    k_ens_update<false, true><<<g, kBlock, 0, st>>>(s.pos, s.pos_ref, s.vel, s.acc, s.q, s.n, e, dt, thresh_sq, s.flags);
+}
+
+void launch_thermo_ens(const DevState &s, const EnsDev &e, const LJDev *lj_dev,
+                       const long long *eatom, double *out, hipStream_t st) {
+  if (e.nsys == 0) return;
+  k_ens_thermo<<<e.nsys, kBlock, 0, st>>>(s.vel, s.n, e, lj_dev, eatom, out);
 }
 
 void launch_check_disp(DevState &s, const BoxDev &b, hipStream_t st) {

@@ -265,6 +265,77 @@ __global__ void __launch_bounds__(1024) k_reduce_i64(const long long *__restrict
   }
 }
 
+// Ensemble form of k_lj_f32fx (single species): each atom reads the
+// fixed-point scale of its own system; ENERGY writes per-atom fixed-point
+// virial and energy (each pair once: i < j) for the per-system sums.
+template <int BLOCK, bool ENERGY>
+__global__ void __launch_bounds__(BLOCK)
+    k_lj_ens(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
+             const int *__restrict__ nnbr, int stride, int n, EnsDev e,
+             const LJDev *__restrict__ lj, double *__restrict__ acc, long long *__restrict__ eatom,
+             int *__restrict__ flags) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const BoxDev &b = e.box[e.sys[i]];
+  const double qu0 = b.qinv[0] / kResidScale, qu1 = b.qinv[1] / kResidScale,
+               qu2 = b.qinv[2] / kResidScale;
+  const int4 qi = q[i];
+  const double rc2_0 = lj->rc2[0], sig2_0 = lj->sig2[0], r2ovf_0 = lj->r2ovf[0];
+  const float eps4_0 = lj->eps4f[0], fsh_0 = lj->fshiftf[0];
+  const double ecut = lj->ecut[0], ecut2 = lj->ecut2[0];
+  const int rix = q_resid(qi.w, 0), riy = q_resid(qi.w, 1), riz = q_resid(qi.w, 2);
+  const int cp = (nnbr[i] + 3) & ~3;
+  unsigned long long sx = 0, sy = 0, sz = 0;
+  long long vir = 0, pe = 0;
+  int terms = 0;
+  bool ovf = false;
+  for (int k = 0; k < cp; k += 4) {
+    const u32x4 g = load_group(nbr, (size_t)stride, i, k);
+    const uint32_t ent[4] = {g.x, g.y, g.z, g.w};
+    int4 qj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) qj[u] = q[ent[u] & kIndexMask];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = (int)(ent[u] & kIndexMask);
+      const double dx = fma((double)(qj[u].x - qi.x), kResidScale, (double)(q_resid(qj[u].w, 0) - rix)) * qu0;
+      const double dy = fma((double)(qj[u].y - qi.y), kResidScale, (double)(q_resid(qj[u].w, 1) - riy)) * qu1;
+      const double dz = fma((double)(qj[u].z - qi.z), kResidScale, (double)(q_resid(qj[u].w, 2) - riz)) * qu2;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 <= rc2_0 && r2 > 0.0) {
+        double r2i = __builtin_amdgcn_rcp(r2);
+        r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
+        const double s2 = sig2_0 * r2i;
+        const float s6 = (float)(s2 * s2 * s2);
+        const float r2if = (float)r2i;
+        const float fc = eps4_0 * (r2if * (s6 * (-12.0f * s6 + 6.0f)) + fsh_0);
+        ovf |= r2 < r2ovf_0;
+        const double fcd = (double)fc;
+        const double fcs = fcd * kForceScale;
+        sx += (unsigned long long)__double_as_longlong(fma(fcs, dx, kMagic));
+        sy += (unsigned long long)__double_as_longlong(fma(fcs, dy, kMagic));
+        sz += (unsigned long long)__double_as_longlong(fma(fcs, dz, kMagic));
+        ++terms;
+        if (ENERGY && i < j) {
+          vir += __double_as_longlong(fma(-fcd * r2, kEnergyScale, kMagic)) - kMagicBits;
+          const float u = eps4_0 * ((s6 * s6 - s6) + (float)(ecut * r2) - (float)ecut2);
+          pe += __double_as_longlong(fma((double)u, kEnergyScale, kMagic)) - kMagicBits;
+        }
+      }
+    }
+  }
+  const unsigned long long base = (unsigned long long)terms * (unsigned long long)kMagicBits;
+  const double mi = lj->mass_inv[0] * (1.0 / kForceScale);
+  acc[i] = (double)(long long)(sx - base) * mi;
+  acc[n + i] = (double)(long long)(sy - base) * mi;
+  acc[2 * n + i] = (double)(long long)(sz - base) * mi;
+  if (ovf) atomicOr(&flags[F_FXOVF], 1);
+  if (ENERGY) {
+    eatom[2 * (size_t)i] = vir;
+    eatom[2 * (size_t)i + 1] = pe;
+  }
+}
+
 // Fixed-order sum of per-block partials (deterministic).
 __global__ void __launch_bounds__(1024) k_reduce_partials(const double *__restrict__ partials,
                                                          int nblk, int width,
@@ -303,6 +374,26 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
 }
 
 }  // namespace
+
+void launch_forces_ens(DevState &s, const EnsDev &e, const LJDev *lj_dev, int block, bool energy,
+                       long long *eatom, hipStream_t st) {
+  if (s.n == 0) return;
+#define CV2_LJE(B)                                                                            \
+  if (block == B) {                                                                           \
+    const int g = (s.n + B - 1) / B;                                                          \
+    if (energy)                                                                               \
+      k_lj_ens<B, true><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev, s.acc, \
+                                          eatom, s.flags);                                    \
+    else                                                                                      \
+      k_lj_ens<B, false><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev,       \
+                                           s.acc, eatom, s.flags);                            \
+    return;                                                                                   \
+  }
+  CV2_LJE(64)
+  CV2_LJE(128)
+  CV2_LJE(256)
+#undef CV2_LJE
+}
 
 void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                    int precision, bool energy, hipStream_t st) {

@@ -36,6 +36,17 @@ class Timing(C.Structure):
                 ("integrate_ms", C.c_double)]
 
 
+class EnsTuning(C.Structure):
+    _fields_ = [("force_block", C.c_int32), ("build_block", C.c_int32), ("rebuild_all", C.c_int32),
+                ("ms_per_step", C.c_double)]
+
+
+class EnsStats(C.Structure):
+    _fields_ = [("natoms", C.c_int64), ("nsys", C.c_int32), ("capacity", C.c_int32),
+                ("steps", C.c_int64), ("build_steps", C.c_int64), ("system_builds", C.c_int64),
+                ("list_entries", C.c_int64), ("max_neighbors", C.c_int32), ("graph_nodes", C.c_int32)]
+
+
 class Cv2Error(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"cv2 error {code} ({ERRORS.get(code, '?')}): {msg}")
@@ -74,6 +85,21 @@ def lib():
             "cv2_synchronize": ([vp], C.c_int),
             "cv2_set_timing": ([vp, C.c_int], C.c_int),
             "cv2_get_timing": ([vp, C.POINTER(Timing)], C.c_int),
+            "cv2_ens_create": ([C.POINTER(vp), C.c_int], C.c_int),
+            "cv2_ens_destroy": ([vp], C.c_int),
+            "cv2_ens_last_error": ([vp], C.c_char_p),
+            "cv2_ens_set_lj": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
+            "cv2_ens_set_neighbor": ([vp, C.c_double, C.c_double], C.c_int),
+            "cv2_ens_upload": ([vp, C.c_int32, _i64, _d, _d, _d, _d], C.c_int),
+            "cv2_ens_set_tuning": ([vp, C.POINTER(EnsTuning)], C.c_int),
+            "cv2_ens_tune": ([vp, C.c_double, C.c_int32, C.POINTER(EnsTuning)], C.c_int),
+            "cv2_ens_run_vv": ([vp, C.c_int64, C.c_double], C.c_int),
+            "cv2_ens_build_neighbors": ([vp], C.c_int),
+            "cv2_ens_thermo": ([vp, _d, _d, _d], C.c_int),
+            "cv2_ens_download": ([vp, _d, _d, _d], C.c_int),
+            "cv2_ens_get_pairs": ([vp, C.c_int32, C.c_int64, _i64, _i64], C.c_int),
+            "cv2_ens_get_stats": ([vp, C.POINTER(EnsStats)], C.c_int),
+            "cv2_ens_use_graph": ([vp, C.c_int], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -204,6 +230,107 @@ class Context:
     def close(self):
         if getattr(self, "h", None):
             self.L.cv2_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Many independent periodic LJ systems advanced together (batched
+    ensemble of include/cv2_md.h; FP32 pair arithmetic with 64-bit
+    fixed-point accumulation, velocity Verlet, one captured graph per step)."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.cv2_ens_create(C.byref(h), device)
+        if rc:
+            raise Cv2Error(rc, "cv2_ens_create failed (no GPU?)")
+        self.h = h
+        self.natoms = np.zeros(0, np.int64)
+
+    def _chk(self, rc):
+        if rc:
+            raise Cv2Error(rc, self.L.cv2_ens_last_error(self.h).decode())
+
+    @classmethod
+    def from_systems(cls, systems, eps=1.0, sigma=1.0, rc=2.5, mass=1.0, nl_cutoff=2.5, nl_coef=1.12,
+                     device=0):
+        """systems: list of dicts with pos (n,3), vel (n,3), lo (3,), hi (3,)."""
+        e = cls(device)
+        e._chk(e.L.cv2_ens_set_lj(e.h, eps, sigma, rc, mass))
+        e._chk(e.L.cv2_ens_set_neighbor(e.h, nl_cutoff, nl_coef))
+        e.upload(systems)
+        return e
+
+    def upload(self, systems):
+        nat = np.array([len(s["pos"]) for s in systems], np.int64)
+        lo = _f64(np.concatenate([np.asarray(s["lo"], np.float64) for s in systems]))
+        hi = _f64(np.concatenate([np.asarray(s["hi"], np.float64) for s in systems]))
+        pos = _f64(np.concatenate([s["pos"] for s in systems]))
+        vel = _f64(np.concatenate([s["vel"] for s in systems]))
+        self._chk(self.L.cv2_ens_upload(self.h, len(systems), _p(nat, _i64), _p(lo, _d), _p(hi, _d),
+                                        _p(pos, _d), _p(vel, _d)))
+        self.natoms = nat
+        self.offsets = np.concatenate([[0], np.cumsum(nat)])
+
+    def set_tuning(self, force_block=256, build_block=128, rebuild_all=0):
+        t = EnsTuning(force_block, build_block, rebuild_all, 0.0)
+        self._chk(self.L.cv2_ens_set_tuning(self.h, C.byref(t)))
+
+    def tune(self, dt, trial_steps=20):
+        t = EnsTuning()
+        self._chk(self.L.cv2_ens_tune(self.h, dt, int(trial_steps), C.byref(t)))
+        return dict(force_block=t.force_block, build_block=t.build_block, rebuild_all=t.rebuild_all,
+                    ms_per_step=t.ms_per_step)
+
+    def use_graph(self, enable=True):
+        self._chk(self.L.cv2_ens_use_graph(self.h, int(bool(enable))))
+
+    def run_vv(self, nsteps, dt):
+        self._chk(self.L.cv2_ens_run_vv(self.h, int(nsteps), float(dt)))
+
+    def build(self):
+        """Rebuild every system's list now (and recompute forces)."""
+        self._chk(self.L.cv2_ens_build_neighbors(self.h))
+
+    def thermo(self):
+        k = len(self.natoms)
+        ke, pe, vir = np.empty(k), np.empty(k), np.empty(k)
+        self._chk(self.L.cv2_ens_thermo(self.h, _p(ke, _d), _p(pe, _d), _p(vir, _d)))
+        return ke, pe, vir
+
+    def download(self):
+        n = int(self.natoms.sum())
+        p = np.empty((n, 3)); v = np.empty((n, 3)); a = np.empty((n, 3))
+        self._chk(self.L.cv2_ens_download(self.h, _p(p, _d), _p(v, _d), _p(a, _d)))
+        return p, v, a
+
+    def pairs(self, sys):
+        k = C.c_int64()
+        self._chk(self.L.cv2_ens_get_pairs(self.h, int(sys), 0, C.byref(k), None))
+        out = np.empty((max(k.value, 1), 3), np.int64)
+        self._chk(self.L.cv2_ens_get_pairs(self.h, int(sys), k.value, C.byref(k), _p(out, _i64)))
+        return out[: k.value]
+
+    def stats(self):
+        s = EnsStats()
+        self._chk(self.L.cv2_ens_get_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in EnsStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cv2_ens_destroy(self.h)
             self.h = None
 
     def __enter__(self):

@@ -77,3 +77,30 @@ def ka_mixture(n: int, rho: float = 1.2, frac_a: float = 0.8, seed: int = 1005,
                 lj_cutoff=rc_max, nl_cutoff=rc_max, nl_coef=ce / rc_max, ghost_cutoff=ce, dt=dt,
                 integrator=1, gamma=gamma, kbt=temperature, k_b=1.0, n_r_df=0,
                 temperature=temperature, seed=seed)
+
+
+def ensemble_systems(nsys: int = 512, seed: int = 1004, nmin: int = 500, nmax: int = 20000,
+                     rho=(0.60, 0.90), temperature=(0.8, 2.0), jitter: float = 0.05):
+    """Batched-ensemble workload: N_s log-uniform in [nmin, nmax], density and
+    temperature uniform in the given ranges; half the systems start on a
+    random FCC lattice (the smallest 4k^3 >= N_s lattice with random
+    vacancies), half on the same lattice jittered by `jitter` sigma.
+    Returns a list of dicts (lo, hi, pos, vel, rho, temperature)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nsys):
+        n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        rho_s = float(rng.uniform(*rho))
+        t_s = float(rng.uniform(*temperature))
+        k = int(np.ceil((n / 4.0) ** (1.0 / 3.0) - 1e-9))
+        L = (n / rho_s) ** (1.0 / 3.0)
+        a = L / k
+        g = np.stack(np.meshgrid(np.arange(k), np.arange(k), np.arange(k), indexing="ij"), -1)
+        sites = ((g.reshape(-1, 3)[:, None, :] + FCC_BASIS[None, :, :]) * a).reshape(-1, 3)
+        pos = sites[np.sort(rng.choice(len(sites), n, replace=False))]
+        if rng.random() < 0.5:
+            pos = pos + rng.normal(0.0, jitter, pos.shape)
+        pos = np.mod(pos, L)
+        vel = maxwell_velocities(n, t_s, int(rng.integers(1 << 31)))
+        out.append(dict(lo=np.zeros(3), hi=np.full(3, L), pos=pos, vel=vel, rho=rho_s, temperature=t_s))
+    return out

@@ -159,6 +159,70 @@ int cv2_get_timing(cv2_ctx *ctx, cv2_timing *out);
 /* Wait for all work queued on the context's stream. */
 int cv2_synchronize(cv2_ctx *ctx);
 
+/* ---- Batched ensemble ----------------------------------------------------
+ * Many independent periodic single-species LJ systems of uneven size,
+ * advanced together: one set of device arrays, each step one captured HIP
+ * graph (update, list build of the systems that need one, FP32+fixed-point
+ * forces).  The reference has no counterpart: it runs one system per
+ * Caviar2 object; an ensemble stands in for nsys independent
+ * Md_simulator::run calls (md_simulator.cpp:317-356) with velocity Verlet,
+ * the same force law, list rule and rebuild test as the single-box context. */
+typedef struct cv2_ens cv2_ens;
+
+typedef struct {
+  int32_t force_block;   /* threads per block of the force kernel: 64, 128 or 256 */
+  int32_t build_block;   /* threads per block of the list kernel: 64, 128 or 256 */
+  int32_t rebuild_all;   /* 0: rebuild only the systems that ask; 1: rebuild all when any asks */
+  double ms_per_step;    /* cv2_ens_tune: measured time per step of the chosen setting */
+} cv2_ens_tuning;
+
+typedef struct {
+  int64_t natoms;        /* all systems */
+  int32_t nsys;
+  int32_t capacity;      /* list slots per atom */
+  int64_t steps;         /* steps taken */
+  int64_t build_steps;   /* steps on which any system was rebuilt */
+  int64_t system_builds; /* sum over systems of their builds */
+  int64_t list_entries;  /* entries of the current lists */
+  int32_t max_neighbors; /* longest list built so far */
+  int32_t graph_nodes;   /* nodes of the captured step graph (0 before the first run) */
+} cv2_ens_stats;
+
+int cv2_ens_create(cv2_ens **e, int device);
+int cv2_ens_destroy(cv2_ens *e);
+const char *cv2_ens_last_error(const cv2_ens *e);
+/* One species: eps, sigma, LJ cutoff, mass (lj.cpp:475-484 force law). */
+int cv2_ens_set_lj(cv2_ens *e, double eps, double sigma, double cutoff, double mass);
+/* List cutoff and skin coefficient, as cv2_set_neighbor. */
+int cv2_ens_set_neighbor(cv2_ens *e, double cutoff, double extra_coef);
+/* nsys periodic boxes [lo, hi) (3 doubles each), natoms[s] atoms each;
+ * pos and vel are AoS xyz of all systems in system order.  Each box must be
+ * wider than twice the list cutoff. */
+int cv2_ens_upload(cv2_ens *e, int32_t nsys, const int64_t *natoms, const double *lo,
+                   const double *hi, const double *pos, const double *vel);
+/* Set the launch shapes and rebuild policy (defaults 256, 128, 0). */
+int cv2_ens_set_tuning(cv2_ens *e, const cv2_ens_tuning *t);
+/* Autotuner: times trial_steps velocity-Verlet steps (after a warm-up) for
+ * every force block x build block x rebuild policy, from the same uploaded
+ * state each time, keeps the fastest and restores the state. */
+int cv2_ens_tune(cv2_ens *e, double dt, int32_t trial_steps, cv2_ens_tuning *best);
+/* nsteps of velocity Verlet for every system. */
+int cv2_ens_run_vv(cv2_ens *e, int64_t nsteps, double dt);
+/* Rebuild every system's list now and recompute forces (as
+ * Neighborlist::build(true), neighborlist.cpp:77-98, per system). */
+int cv2_ens_build_neighbors(cv2_ens *e);
+/* Per-system kinetic energy (raw), potential energy and virial of the
+ * current state (arrays of nsys; any may be NULL). */
+int cv2_ens_thermo(cv2_ens *e, double *ke, double *pe, double *virial);
+/* State in upload order (AoS xyz of all systems; any pointer may be NULL). */
+int cv2_ens_download(cv2_ens *e, double *pos, double *vel, double *acc);
+/* Pairs of system sys, as cv2_get_pairs (indices local to the system). */
+int cv2_ens_get_pairs(cv2_ens *e, int32_t sys, int64_t cap, int64_t *npairs, int64_t *pairs);
+int cv2_ens_get_stats(cv2_ens *e, cv2_ens_stats *out);
+/* enable != 0 (default): steps 2..n replay the captured step graph;
+ * 0: the same launches are issued one by one (tests compare the two). */
+int cv2_ens_use_graph(cv2_ens *e, int enable);
+
 #ifdef __cplusplus
 }
 #endif
