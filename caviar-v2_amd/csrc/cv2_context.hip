@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -127,7 +128,7 @@ void free_state(DevState &s) {
   void *ptrs[] = {s.pos, s.pos_alt, s.pos_ref, s.q, s.vel, s.vel_alt, s.acc, s.acc_alt, s.tag,
                   s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.order, s.box_dev, s.wstat, s.cell_count, s.cell_start,
                   s.gcell_count, s.gcell_start, s.gcell_items, s.tile_sum, s.nbr, s.nnbr, s.flags,
-                  s.partials, s.fx_acc, s.reduce_out};
+                  s.partials, s.fx_acc, s.reduce_out, s.tpiece, s.tstage};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   s = DevState{};
@@ -159,8 +160,6 @@ int setup_box(cv2_ctx *ctx) {
     b.L[d] = ctx->hi[d] - ctx->lo[d];
     b.halfL[d] = 0.5 * b.L[d];
     b.bc[d] = ctx->host_halo ? 0 : ctx->bc[d];
-    b.qscale[d] = 4294967296.0 / b.L[d];
-    b.qinv[d] = b.L[d] / 4294967296.0;
     if (ctx->bc[d] && !ctx->host_halo && b.L[d] <= 2.0 * ce)
       return fail(ctx, CV2_ERR_BAD_ARG, "periodic box must be wider than twice the list cutoff");
   }
@@ -184,17 +183,17 @@ int setup_box(cv2_ctx *ctx) {
     wmax = std::max(wmax, ext / nc);
     ncell *= nc;
   }
-  // Coarse filter bounds.  Rounding to 2^32 units per box length moves a
+  // Fixed-point units: the force kernel forms separations of list pairs
+  // (below ce + skin < 2 ce); the coarse filter forms them between a home
+  // cell's atoms and those of the adjacent cells (below 2 w).
+  set_qscale(b, fast ? 2.0 * std::max(wmax, ce) : 2.0 * ce);
+  // Coarse filter bounds.  Rounding to the fixed-point unit u moves a
   // distance by less than sqrt(sum u_d^2); the float arithmetic on
   // coordinates relative to a home atom (|x| < 2 w) adds at most
   // e_f = 8 * 2^-24 * (2 w + ce) per axis.
   {
     double u2 = 0.0;
-    for (int d = 0; d < 3; ++d) {
-      const double u = b.L[d] / 4294967296.0;
-      b.cunit[d] = (float)u;
-      u2 += u * u;
-    }
+    for (int d = 0; d < 3; ++d) u2 += b.qinv[d] * b.qinv[d];
     const double ef = 8.0 * 5.9604644775390625e-08 * (2.0 * wmax + ce) * std::sqrt(3.0);
     const double e = 1.01 * std::sqrt(u2) + ef + 1e-6 * ce;
     b.cthr_in = std::nextafter((float)((ce - e) * (ce - e) * (1.0 - 4e-6)), 0.0f);
@@ -242,6 +241,33 @@ int check_flags(cv2_ctx *ctx) {
                     std::to_string(f[F_MAXNBR]) + ")");
   if (f[F_NAN]) return fail(ctx, CV2_ERR_NAN, "NaN acceleration");
   if (f[F_FXOVF]) return fail(ctx, CV2_ERR_FIXED_OVERFLOW, "fixed-point force range exceeded (r < sigma/2)");
+  if (f[F_STAGE])
+    return fail(ctx, CV2_ERR_CAPACITY, "a force task's neighbourhood exceeds 65536 atoms (16-bit slots)");
+  return CV2_OK;
+}
+
+// List form of the next build: task slots (16-bit) for the FP32 kernel on
+// wave-build boxes, 32-bit entries otherwise.  Called when the host forces
+// a build, so the task geometry only changes with the list it belongs to.
+int prepare_list_form(cv2_ctx *ctx) {
+  DevState &s = ctx->s;
+  // the task sentinel is more than sqrt(2) list cutoffs from its home atoms
+  double rc2max = 0.0;
+  for (int k = 0; k < ctx->lj.nt * ctx->lj.nt; ++k) rc2max = std::max(rc2max, ctx->lj.rc2[k]);
+  s.slots = ctx->precision != CV2_PREC_FP64 && ctx->b.coarse && s.n > 0 &&
+            rc2max < 1.9 * ctx->b.ce_sq;
+  if (!s.slots) return CV2_OK;
+  s.tg = task_geom(s.n, ctx->b);
+  if (s.tg.ntask > s.task_cap) {
+    HIPCHK(dalloc(&s.tpiece, (size_t)s.tg.ntask * kTaskPieces));
+    HIPCHK(dalloc(&s.tstage, (size_t)s.tg.ntask));
+    s.task_cap = s.tg.ntask;
+  }
+  const int nb = s.tg.ntask + 8;  // per-task energy partials
+  if (nb > ctx->partial_blocks) {
+    HIPCHK(dalloc(&s.partials, (size_t)nb * 8));
+    ctx->partial_blocks = nb;
+  }
   return CV2_OK;
 }
 
@@ -262,12 +288,17 @@ int fetch_energy(cv2_ctx *ctx) {
   return CV2_OK;
 }
 
-void queue_build(cv2_ctx *ctx, bool force) {
+int queue_build(cv2_ctx *ctx, bool force) {
   force = force || ctx->need_build;
   ctx->need_build = false;
+  if (force) {
+    const int rc = prepare_list_form(ctx);
+    if (rc) return rc;
+  }
   if (force || (ctx->nl_flags & CV2_NL_REBUILD_EVERY_STEP))
     launch_set_flag(ctx->s.flags, F_REBUILD, 1, ctx->stream);
   launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo);
+  return CV2_OK;
 }
 
 int alloc_positions(cv2_ctx *ctx, int64_t ntot) {
@@ -337,9 +368,9 @@ bool build_lj_table(LJDev &L, int ntypes, const double *eps, const double *sigma
     L.ecut[k] = (6.0 * rho_c12 - 3.0 * rho_c6) * rcinv;
     L.ecut2[k] = 7.0 * rho_c12 - 4.0 * rho_c6;
     L.eps4f[k] = (float)L.eps4[k];
-    L.sig2f[k] = (float)L.sig2[k];
-    L.rc2f[k] = (float)L.rc2[k];
-    L.fshiftf[k] = (float)L.fshift[k];
+    L.e6f[k] = (float)(6.0 * L.eps4[k] * kForceScale);
+    L.e12f[k] = (float)(12.0 * L.eps4[k] * kForceScale);
+    L.fshf[k] = (float)(L.eps4[k] * L.fshift[k] * kForceScale);
     // FP32 path: largest r^2 at which |F(r)| = |forceCoef| r reaches 0.9 * 2^19
     // (bisection on the repulsive branch, where |F| falls with r)
     L.r2ovf[k] = 0.0;
@@ -357,6 +388,29 @@ bool build_lj_table(LJDev &L, int ntypes, const double *eps, const double *sigma
     }
   }
   return true;
+}
+
+void set_qscale(BoxDev &b, double span) {
+  // periodic axes: the largest k with span < L / 2^(k+1) (0.1% margin), or
+  // 0 (int32 differences are then minimum-image separations)
+  double ufine = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    if (!b.bc[d]) continue;
+    int k = 0;
+    while (k < 30 && std::ldexp(span * 1.001, k + 2) < b.L[d]) ++k;
+    b.qinv[d] = std::ldexp(b.L[d], -(32 + k));
+    ufine = ufine == 0.0 ? b.qinv[d] : std::min(ufine, b.qinv[d]);
+  }
+  // open axes need only span < 2^31 u: take the finest periodic unit, which
+  // satisfies it, or span / 2^30 when no axis is periodic
+  if (ufine == 0.0) ufine = std::ldexp(span, -30);
+  for (int d = 0; d < 3; ++d)
+    if (!b.bc[d]) b.qinv[d] = ufine;
+  for (int d = 0; d < 3; ++d) {
+    b.qscale[d] = 1.0 / b.qinv[d];
+    b.cunit[d] = (float)b.qinv[d];
+  }
+  b.qiso = b.qinv[0] == b.qinv[1] && b.qinv[1] == b.qinv[2];
 }
 
 }  // namespace cv2
@@ -378,6 +432,11 @@ int cv2_create(cv2_ctx **out, int device) {
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_flags, sizeof(int) * F_COUNT, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_red, sizeof(double) * 16, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_fx, sizeof(long long) * 4, hipHostMallocDefault);
+  // LDS staging capacity of the FP32 task kernel; CV2_STAGE_LIMIT lowers it
+  // (tests use it to exercise the unstaged path)
+  ctx->s.stage_limit = kStage;
+  if (const char *lim = std::getenv("CV2_STAGE_LIMIT"))
+    ctx->s.stage_limit = std::max(0, std::min(kStage, std::atoi(lim)));
   if (e != hipSuccess) {
     fprintf(stderr, "cv2_create: %s\n", hipGetErrorString(e));
     cv2_destroy(ctx);
@@ -465,6 +524,7 @@ int cv2_set_precision(cv2_ctx *ctx, int precision) {
   ctx->precision = precision;
   ctx->s.write_q = precision != CV2_PREC_FP64;
   ctx->forces_valid = false;
+  ctx->need_build = true;  // the list form follows the precision
   return CV2_OK;
 }
 
@@ -585,7 +645,8 @@ int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt) {
     HIPCHK(hipMemcpy(&before, ctx->s.flags + F_REBUILDS, sizeof(int), hipMemcpyDeviceToHost));
   }
   if (!force) launch_check_disp(ctx->s, ctx->b, ctx->stream);
-  queue_build(ctx, force != 0);
+  rc = queue_build(ctx, force != 0);
+  if (rc) return rc;
   ctx->forces_valid = false;
   rc = check_flags(ctx);
   if (rebuilt) *rebuilt = ctx->h_flags[F_REBUILDS] != before;
@@ -597,7 +658,10 @@ int cv2_compute_forces(cv2_ctx *ctx) {
   int rc = ensure_ready(ctx);
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
-  if (ctx->need_build) queue_build(ctx, true);
+  if (ctx->need_build) {
+    rc = queue_build(ctx, true);
+    if (rc) return rc;
+  }
   if (ctx->precision != CV2_PREC_FP64) launch_pack_q(ctx->s, ctx->b, ctx->stream);
   const int t = span_begin(ctx);
   launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, true, ctx->stream);
@@ -625,7 +689,10 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
     launch_update(ctx->s, ctx->b, u, ctx->stream);
     span_end(ctx, t, 2);
     t = span_begin(ctx);
-    queue_build(ctx, false);
+    {
+      const int rc = queue_build(ctx, false);
+      if (rc) return rc;
+    }
     span_end(ctx, t, 1);
     t = span_begin(ctx);
     launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
@@ -747,10 +814,42 @@ int cv2_get_pairs(cv2_ctx *ctx, int64_t cap, int64_t *npairs, int64_t *pairs) {
   HIPCHK(hipMemcpyAsync(tg.data(), s.tag, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipMemcpyAsync(nb.data(), s.nbr, sizeof(uint32_t) * nb.size(), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  // task-slot form: rebuild the 32-bit entries (atom | image) from the task
+  // piece tables; the image of j follows from its cell relative to i's cell
+  // (periodic grid, at least 5 cells per axis)
+  std::vector<int> cst, tst;
+  std::vector<int4> tpc;
+  if (s.slots) {
+    cst.resize(s.ncell + 1);
+    tpc.resize((size_t)s.tg.ntask * kTaskPieces);
+    HIPCHK(hipMemcpyAsync(cst.data(), s.cell_start, sizeof(int) * cst.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(tpc.data(), s.tpiece, sizeof(int4) * tpc.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  const TaskGeom &g = s.tg;
+  auto cell_of = [&](int64_t a) {
+    return (int)(std::upper_bound(cst.begin(), cst.end(), (int)a) - cst.begin()) - 1;
+  };
+  auto entry = [&](int64_t i, int m) -> uint32_t {
+    if (!s.slots) return nb[((size_t)(m >> 2) * s.stride + i) * 4 + (m & 3)];
+    const uint16_t *nb16 = reinterpret_cast<const uint16_t *>(nb.data());
+    const int sl = nb16[((size_t)(m >> 3) * s.stride + i) * 8 + (m & 7)];
+    const int ci = cell_of(i);
+    const int cx = ci % g.nx, cy = (ci / g.nx) % g.ny, cz = ci / (g.nx * g.ny);
+    const int4 *tp = &tpc[(size_t)((cy + g.ny * cz) * g.tpr + task_of_x(g, cx)) * kTaskPieces];
+    int p = 0;
+    for (int k2 = 1; k2 < kTaskPieces; ++k2) p = tp[k2].y <= sl ? k2 : p;
+    const int j = tp[p].x + (sl - tp[p].y);
+    const int cj = cell_of(j);
+    const int d[3] = {cj % g.nx - cx, (cj / g.nx) % g.ny - cy, cj / (g.nx * g.ny) - cz};
+    uint32_t img = 0;
+    for (int a = 0; a < 3; ++a) img |= (uint32_t)((d[a] > 1 ? 1 : (d[a] < -1 ? -1 : 0)) + 1) << (26 + 2 * a);
+    return (uint32_t)j | img;
+  };
   int64_t k = 0;
   for (int64_t i = 0; i < n; ++i) {
     for (int m = 0; m < cnt[i]; ++m) {
-      const uint32_t e = nb[((size_t)(m >> 2) * s.stride + i) * 4 + (m & 3)];
+      const uint32_t e = entry(i, m);
       const int64_t j = e & kIndexMask;
       const uint32_t img = e >> 26;
       const int code = (int)(img & 3u) + 3 * (int)((img >> 2) & 3u) + 9 * (int)(img >> 4);

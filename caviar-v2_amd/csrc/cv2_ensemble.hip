@@ -38,6 +38,7 @@ struct cv2_ens {
   std::vector<double> lo, hi;
   int ncell = 0;
   double thresh_sq = 0.0;
+  bool iso = true;  // every system has one fixed-point unit on all axes
   // device state and tables
   DevState s{};
   int *d_sys = nullptr, *d_cell_sys = nullptr, *d_cell_off = nullptr, *d_atom_off = nullptr;
@@ -132,8 +133,7 @@ void free_all(cv2_ens *ens) {
 }
 
 // Box constants of one periodic system: cells at least half the list
-// cutoff wide (the per-atom build scans +-2 cells), 2^32 fixed-point units
-// per box length.
+// cutoff wide (the per-atom build scans +-2 cells).
 BoxDev system_box(const double lo[3], const double hi[3], double ce, double rc) {
   BoxDev b{};
   b.ce_sq = ce * ce;
@@ -150,9 +150,8 @@ BoxDev system_box(const double lo[3], const double hi[3], double ce, double rc) 
     nc = std::max(1, std::min(nc, 1024));
     b.nc[d] = nc;
     b.cinv[d] = nc / b.L[d];
-    b.qscale[d] = 4294967296.0 / b.L[d];
-    b.qinv[d] = b.L[d] / 4294967296.0;
   }
+  set_qscale(b, 2.0 * ce);  // list pairs stay below ce + skin < 2 ce
   b.coarse = 0;
   b.ghost_ok = 1;
   return b;
@@ -176,7 +175,7 @@ int prime(cv2_ens *ens) {
   launch_fill_int(ens->d_sflag, ens->nsys, 1, ens->stream);
   launch_set_flag(ens->s.flags, F_REBUILD, 1, ens->stream);
   launch_build_ens(ens->s, E, ens->ncell, ens->tune.build_block, false, ens->stream);
-  launch_forces_ens(ens->s, E, ens->lj_dev, ens->tune.force_block, false, ens->eatom, ens->stream);
+  launch_forces_ens(ens->s, E, ens->lj_dev, ens->tune.force_block, ens->iso, false, ens->eatom, ens->stream);
   ens->forces_valid = true;
   return check_flags(ens);
 }
@@ -216,7 +215,7 @@ int ensure_graph(cv2_ens *ens, double dt) {
   EHIP(hipStreamBeginCapture(ens->stream, hipStreamCaptureModeThreadLocal));
   launch_update_ens(ens->s, E, true, true, dt, ens->thresh_sq, ens->stream);
   launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->stream);
-  launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, false, ens->eatom, ens->stream);
+  launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream);
   EHIP(hipStreamEndCapture(ens->stream, &ens->graph));
   EHIP(hipGraphInstantiate(&ens->gexec, ens->graph, nullptr, nullptr, 0));
   EHIP(hipGraphGetNodes(ens->graph, nullptr, &ens->gnodes));
@@ -239,7 +238,7 @@ int run(cv2_ens *ens, int64_t nsteps, double dt) {
   // half kick closes the last step
   launch_update_ens(ens->s, E, false, true, dt, ens->thresh_sq, ens->stream);
   launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->stream);
-  launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, false, ens->eatom, ens->stream);
+  launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream);
   if (nsteps > 1) {
     if (ens->use_graph) {
       int rc = ensure_graph(ens, dt);
@@ -249,7 +248,7 @@ int run(cv2_ens *ens, int64_t nsteps, double dt) {
       for (int64_t k = 1; k < nsteps; ++k) {
         launch_update_ens(ens->s, E, true, true, dt, ens->thresh_sq, ens->stream);
         launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->stream);
-        launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, false, ens->eatom, ens->stream);
+        launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream);
       }
     }
   }
@@ -333,12 +332,14 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
   std::vector<int> coff(nsys, 0);
   int64_t ncell = 0;
   double max_expect = 0.0;
+  bool iso = true;
   for (int s = 0; s < nsys; ++s) {
     if (natoms[s] < 1) return efail(ens, CV2_ERR_BAD_ARG, "every system needs atoms");
     for (int d = 0; d < 3; ++d)
       if (!(hi[3 * s + d] - lo[3 * s + d] > 2.0 * ce))
         return efail(ens, CV2_ERR_BAD_ARG, "periodic box must be wider than twice the list cutoff");
     boxes[s] = system_box(lo + 3 * s, hi + 3 * s, ce, ens->nl_cutoff);
+    iso = iso && boxes[s].qiso;
     aoff[s + 1] = aoff[s] + natoms[s];
     coff[s] = (int)ncell;
     ncell += (int64_t)boxes[s].nc[0] * boxes[s].nc[1] * boxes[s].nc[2];
@@ -351,6 +352,7 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
   free_all(ens);
   ens->nsys = nsys;
   ens->n = n;
+  ens->iso = iso;
   ens->natoms.assign(natoms, natoms + nsys);
   ens->atom_off = aoff;
   ens->lo.assign(lo, lo + 3 * nsys);
@@ -479,7 +481,7 @@ int cv2_ens_thermo(cv2_ens *ens, double *ke, double *pe, double *virial) {
   }
   // an energy pass at the current positions (accelerations are recomputed
   // identically), then fixed-order per-system sums
-  launch_forces_ens(ens->s, E, ens->lj_dev, ens->tune.force_block, true, ens->eatom, ens->stream);
+  launch_forces_ens(ens->s, E, ens->lj_dev, ens->tune.force_block, ens->iso, true, ens->eatom, ens->stream);
   launch_thermo_ens(ens->s, E, ens->lj_dev, ens->eatom, ens->d_thermo, ens->stream);
   std::vector<double> out(3 * ens->nsys);
   EHIP(hipMemcpyAsync(out.data(), ens->d_thermo, sizeof(double) * out.size(), hipMemcpyDeviceToHost, ens->stream));

@@ -23,6 +23,9 @@
 // that ghost (owner within ghost_cutoff of the face, same priority rule).
 // The cell grid is only a candidate filter (cells at least as wide as the
 // list cutoff), so the pair set does not depend on it.
+#include <algorithm>
+#include <cmath>
+
 #include "cv2_internal.h"
 
 namespace cv2 {
@@ -583,17 +586,22 @@ constexpr int kSegs = 2 * kWaveRows;         // row ranges: 9 direct-x + 9 wrapp
 // candidate entries; STAGED (single-pass cells): each home atom's accepted
 // entries are collected in buf (LDS) and written as whole 4-entry groups;
 // otherwise they are stored one by one and counts carried in cnt_io.
-template <bool STAGED, bool GHOST_OK>
+// SLOTS: the list takes the candidates' task slots (cslot) instead of their
+// entries, in 16-bit groups of 8, padded with pad_slot (the task's sentinel
+// slot, staged far from every home atom).
+template <bool STAGED, bool GHOST_OK, bool SLOTS>
 __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__restrict__ q,
                                const CellWaveBox &b, const BoxDev *__restrict__ bdev,
-                               const uint32_t *cand, int ncand, int c0,
-                               int self_base, int self_j0, int h0, int h1, const int3 o,
-                               uint32_t *__restrict__ nbr, int *__restrict__ nnbr, size_t stride,
-                               int cap, uint32_t *buf, int *cnt_io, float hxl, float hyl,
-                               float hzl, int lane, bool first, int &kept_l, int &wmax,
+                               const uint32_t *cand, const uint16_t *cslot, int ncand, int c0,
+                               int self_base, int self_j0, int pad_slot, int h0, int h1,
+                               const int3 o, uint32_t *__restrict__ nbr, int *__restrict__ nnbr,
+                               size_t stride, int cap, uint32_t *buf, int *cnt_io, float hxl,
+                               float hyl, float hzl, int lane, bool first, int &kept_l, int &wmax,
                                int &went) {
   float px[kSlots], py[kSlots], pz[kSlots];
   float tin[GHOST_OK ? 1 : kSlots];  // per-slot inner threshold when images may lack ghosts
+  // the value each candidate appends: its entry, or (SLOTS) its task slot,
+  // the entry then being re-read from cand on the rare exact path
   uint32_t ent[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) {
@@ -611,11 +619,13 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
       pz[k] = q_rel(c.z, o.z, b.cunit[2]);
       // images whose ghost may not exist always take the exact path
       if (!GHOST_OK && (e >> 26) != kNoImage) tin[k] = -1.0f;
-      ent[k] = e;
+      ent[k] = SLOTS ? (uint32_t)cslot[t] : e;
     }
   }
+  uint16_t *buf16 = reinterpret_cast<uint16_t *>(buf);
+  uint16_t *nbr16 = reinterpret_cast<uint16_t *>(nbr);
   // lane-private scratch word for lanes that accept nothing (STAGED)
-  const int dummy_at = kPassCand + 4 + lane;
+  const int dummy_at = SLOTS ? kPassCand + 8 + lane : kPassCand + 4 + lane;
   for (int h = h0; h < h1; ++h) {
     // home atom coordinates: lane (h - h0) holds them
     const float hx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hxl), h - h0));
@@ -636,9 +646,15 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
       const bool take = __builtin_amdgcn_inverse_ballot_w64(m);
       if (STAGED) {
         __asm__ volatile("" : "+v"(at));  // computed by every lane: a select, not a branch
-        buf[take ? at : dummy_at] = e;
+        if (SLOTS)
+          buf16[take ? at : dummy_at] = (uint16_t)e;
+        else
+          buf[take ? at : dummy_at] = e;
       } else if (take && at < cap) {
-        nbr[((size_t)(at >> 2) * stride + h) * 4 + (at & 3)] = e;
+        if (SLOTS)
+          nbr16[((size_t)(at >> 3) * stride + h) * 8 + (at & 7)] = (uint16_t)e;
+        else
+          nbr[((size_t)(at >> 2) * stride + h) * 4 + (at & 3)] = e;
       }
       cnt += __popcll(m);
     };
@@ -672,7 +688,7 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
         const bool band = (r2 < b.cthr_out) & !(r2 < (GHOST_OK ? b.cthr_in : tin[GHOST_OK ? 0 : k]));
         bool take = false;
         if (band) {
-          uint32_t e = ent[k];
+          uint32_t e = SLOTS ? cand[k * 64 + lane] : ent[k];
           __asm__ volatile("" : "+v"(e));  // decode here, not hoisted
           // x, y, z only: a loaded but unused w would leave a load pending
           // that the next reuse of its register has to wait for
@@ -695,11 +711,17 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
       }
     }
     if (STAGED) {
-      // whole groups, the last one padded with the atom itself
+      // whole groups (16 bytes), the last one padded with the atom itself
       const int kept = min(cnt, cap);
-      if (lane < 3) buf[kept + lane] = (uint32_t)h | (kNoImage << 26);
-      for (int g = lane; g * 4 < kept; g += 64)
-        reinterpret_cast<u32x4 *>(nbr)[(size_t)g * stride + h] = reinterpret_cast<const u32x4 *>(buf)[g];
+      if (SLOTS) {
+        if (lane < 7) buf16[kept + lane] = (uint16_t)pad_slot;
+        for (int g = lane; g * 8 < kept; g += 64)
+          reinterpret_cast<u32x4 *>(nbr)[(size_t)g * stride + h] = reinterpret_cast<const u32x4 *>(buf)[g];
+      } else {
+        if (lane < 3) buf[kept + lane] = (uint32_t)h | (kNoImage << 26);
+        for (int g = lane; g * 4 < kept; g += 64)
+          reinterpret_cast<u32x4 *>(nbr)[(size_t)g * stride + h] = reinterpret_cast<const u32x4 *>(buf)[g];
+      }
       kept_l = lane == h - h0 ? kept : kept_l;  // stored per chunk
       wmax = max(wmax, cnt);
       went += kept;
@@ -711,17 +733,33 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
 
 // One home cell c (wavefront wv of its block); returns (longest list,
 // entries kept).
-template <bool GHOST_OK>
+// Slot offset (slot - atom index) of the atoms of segment (r, part) of home
+// cell cx in its task's staging order: the piece of row r that holds the
+// segment's first x-cell.
+__device__ inline int seg_slot_delta(const int4 *__restrict__ tp, int nx, int cx, int r, int part) {
+  const int xs = part == 0 ? (cx == 0 ? 0 : cx - 1) : (cx == 0 ? nx - 1 : 0);
+  const int4 a = tp[2 * r];
+  const int4 pc = (xs >= a.z && xs < a.w) ? a : tp[2 * r + 1];
+  return pc.y - pc.x;
+}
+
+template <bool GHOST_OK, bool SLOTS>
 __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restrict__ q,
                           const CellWaveBox &b, const BoxDev *__restrict__ bdev,
                           const int *__restrict__ cstart, uint32_t *__restrict__ nbr,
                           int *__restrict__ nnbr, int stride, int cap, int c, int wv, int lane,
-                          uint32_t (*lds)[kPassCand + 4 + 64], int (*cnt_sh)[kMaxHomeCell]) {
+                          uint32_t (*lds)[kPassCand + 4 + 64], uint16_t *cslot,
+                          int (*cnt_sh)[kMaxHomeCell], const TaskGeom &tg,
+                          const int4 *__restrict__ tpiece, const int *__restrict__ tstage) {
   const int nx = b.nc[0], ny = b.nc[1];
   const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
   const int h0 = cstart[c], h1 = cstart[c + 1];
   if (h0 == h1) return make_int2(0, 0);
   uint32_t *cand = lds[wv];
+  // SLOTS: this cell's task and its piece table
+  const int4 *tp = SLOTS ? tpiece + (size_t)((cy + ny * cz) * tg.tpr + task_of_x(tg, cx)) * kTaskPieces
+                         : nullptr;
+  const int pad_slot = SLOTS ? tstage[(cy + ny * cz) * tg.tpr + task_of_x(tg, cx)] : 0;
   // segment table: flattened position of each row range
   int total = 0, self_base = 0, self_j0 = 0;
   for (int sg = 0; sg < kSegs; ++sg) {
@@ -755,21 +793,29 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
         int j0, len;
         uint32_t img;
         row_range(cstart, b, cx, cy, cz, sg % kWaveRows, sg / kWaveRows, j0, len, img);
+        const int delta = SLOTS && len > 0 ? seg_slot_delta(tp, nx, cx, sg % kWaveRows, sg / kWaveRows) : 0;
         for (int t = lane; t < len; t += 64) {
           const int p = at + t - c0;
-          if (p >= 0 && p < kPassCand) cand[p] = (uint32_t)(j0 + t) | img;
+          if (p >= 0 && p < kPassCand) {
+            cand[p] = (uint32_t)(j0 + t) | img;
+            if (SLOTS) cslot[p] = (uint16_t)(j0 + t + delta);
+          }
         }
         at += len;
       }
       const int ncand = min(kPassCand, total - c0);
       if (staged)
-        cell_wave_pass<true, GHOST_OK>(pos, q, b, bdev, cand, ncand, c0, self_base, self_j0, h, he, o, nbr,
-                             nnbr, (size_t)stride, cap, cand, cnt_sh[wv], hxl, hyl, hzl, lane,
-                             c0 == 0, kept_l, wmax, went);
+        cell_wave_pass<true, GHOST_OK, SLOTS>(pos, q, b, bdev, cand, cslot, ncand, c0, self_base,
+                                              self_j0, pad_slot, h, he, o, nbr, nnbr,
+                                              (size_t)stride, cap,
+                                              SLOTS ? reinterpret_cast<uint32_t *>(cslot) : cand,
+                                              cnt_sh[wv], hxl, hyl, hzl, lane, c0 == 0, kept_l,
+                                              wmax, went);
       else
-        cell_wave_pass<false, GHOST_OK>(pos, q, b, bdev, cand, ncand, c0, self_base, self_j0, h, he, o, nbr,
-                              nnbr, (size_t)stride, cap, nullptr, cnt_sh[wv], hxl, hyl, hzl,
-                              lane, c0 == 0, kept_l, wmax, went);
+        cell_wave_pass<false, GHOST_OK, SLOTS>(pos, q, b, bdev, cand, cslot, ncand, c0, self_base,
+                                               self_j0, pad_slot, h, he, o, nbr, nnbr,
+                                               (size_t)stride, cap, nullptr, cnt_sh[wv], hxl,
+                                               hyl, hzl, lane, c0 == 0, kept_l, wmax, went);
     }
     if (staged && h + lane < he) nnbr[h + lane] = kept_l;
     if (!staged) {
@@ -777,8 +823,11 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
       for (int hh = h; hh < he; ++hh) {
         const int cnt = cnt_sh[wv][hh - h];
         const int kept = min(cnt, cap);
-        const int padded = (kept + 3) & ~3;
-        if (lane < padded - kept)
+        const int padded = SLOTS ? (kept + 7) & ~7 : (kept + 3) & ~3;
+        if (SLOTS && lane < padded - kept)
+          reinterpret_cast<uint16_t *>(nbr)[((size_t)((kept + lane) >> 3) * stride + hh) * 8 + ((kept + lane) & 7)] =
+              (uint16_t)pad_slot;
+        if (!SLOTS && lane < padded - kept)
           nbr[((size_t)((kept + lane) >> 2) * stride + hh) * 4 + ((kept + lane) & 3)] =
               (uint32_t)hh | (kNoImage << 26);
         if (lane == 0) nnbr[hh] = kept;
@@ -790,12 +839,13 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
   return make_int2(wmax, went);
 }
 
-template <bool GHOST_OK>
+template <bool GHOST_OK, bool SLOTS>
 __global__ void __launch_bounds__(256)
     k_nlist_cells(const double4 *__restrict__ pos, const int4 *__restrict__ q, CellWaveBox b,
                   const BoxDev *__restrict__ bdev, const int *__restrict__ cstart,
                   uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
-                  const int *__restrict__ flags, int nblk_real, int2 *__restrict__ bstat) {
+                  const int *__restrict__ flags, int nblk_real, int2 *__restrict__ bstat,
+                  TaskGeom tg, const int4 *__restrict__ tpiece, const int *__restrict__ tstage) {
   if (flags[F_REBUILD] == 0) return;
   // consecutive cells (which share most candidates) on the same XCD
   const int per = gridDim.x >> 3;
@@ -806,13 +856,17 @@ __global__ void __launch_bounds__(256)
   __shared__ __attribute__((aligned(16))) uint32_t lds[4][kPassCand + 4 + 64];
   __shared__ int cnt_sh[4][kMaxHomeCell];
   __shared__ int2 st_sh[4];
+  // candidates' task slots; then (staged) one home atom's slots + 7 pads,
+  // 64 scratch entries
+  __shared__ __attribute__((aligned(16))) uint16_t cslot_sh[4][SLOTS ? kPassCand + 8 + 64 : 8];
   const int wv = (int)(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   // the home cell is uniform across the wavefront
   const int c = __builtin_amdgcn_readfirstlane(blk * 4 + wv);
   int2 st = make_int2(0, 0);
   if (c < b.nc[0] * b.nc[1] * b.nc[2])
-    st = cell_wave<GHOST_OK>(pos, q, b, bdev, cstart, nbr, nnbr, stride, cap, c, wv, lane, lds, cnt_sh);
+    st = cell_wave<GHOST_OK, SLOTS>(pos, q, b, bdev, cstart, nbr, nnbr, stride, cap, c, wv, lane,
+                                    lds, cslot_sh[wv], cnt_sh, tg, tpiece, tstage);
   if (lane == 0) st_sh[wv] = st;
   __syncthreads();
   if (threadIdx.x == 0) {  // per-block statistics, reduced by k_build_done
@@ -822,34 +876,73 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Close a build: reduce the per-block statistics, clear the request.
-__global__ void __launch_bounds__(1024)
-    k_build_done(int *__restrict__ flags, const int2 *__restrict__ wstat, int nw, int cap) {
+// Close a build: reduce the per-block statistics (k_build_stats: 256
+// blocks, one atomic per block into flags[F_MAXNBR] / flags[F_ENTRIES],
+// which k_build_begin cleared), then check them and clear the request.
+__global__ void __launch_bounds__(256)
+    k_build_stats(int *__restrict__ flags, const int2 *__restrict__ wstat, int nw) {
   if (flags[F_REBUILD] == 0) return;
-  __shared__ int smax[1024], ssum[1024];
+  __shared__ int smax[4], ssum[4];
   int m = 0, t = 0;
-  for (int k = threadIdx.x; k < nw; k += 1024) {
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < nw; k += gridDim.x * 256) {
     const int2 v = wstat[k];
     m = max(m, v.x);
     t += v.y;
   }
-  smax[threadIdx.x] = m;
-  ssum[threadIdx.x] = t;
-  __syncthreads();
-  for (int o = 512; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + o]);
-      ssum[threadIdx.x] += ssum[threadIdx.x + o];
-    }
-    __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    m = max(m, __shfl_xor(m, o, 64));
+    t += __shfl_xor(t, o, 64);
   }
+  if ((threadIdx.x & 63) == 0) {
+    smax[threadIdx.x >> 6] = m;
+    ssum[threadIdx.x >> 6] = t;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    flags[F_MAXNBR] = smax[0];
-    flags[F_ENTRIES] = ssum[0];
-    if (smax[0] > cap) flags[F_OVERFLOW] = 1;
+    m = max(max(smax[0], smax[1]), max(smax[2], smax[3]));
+    t = ssum[0] + ssum[1] + ssum[2] + ssum[3];
+    if (m > 0) atomicMax(&flags[F_MAXNBR], m);
+    if (t > 0) atomicAdd(&flags[F_ENTRIES], t);
+  }
+}
+
+__global__ void k_build_done(int *__restrict__ flags, int cap) {
+  if (flags[F_REBUILD] == 0) return;
+  if (threadIdx.x == 0) {
+    if (flags[F_MAXNBR] > cap) flags[F_OVERFLOW] = 1;
     flags[F_REBUILD] = 0;
     flags[F_REBUILDS] += 1;
   }
+}
+
+// Piece table of every force task of the FP32 path (after the cell starts
+// of a build are known): first atom, slot base and x-cells of each piece,
+// and the staged total S; slot S is the task's sentinel, so S must fit 16
+// bits.
+__global__ void k_task_table(const int *__restrict__ cstart, TaskGeom g, int4 *__restrict__ tpiece,
+                             int *__restrict__ tstage, int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int task = blockIdx.x * blockDim.x + threadIdx.x;
+  if (task >= g.ntask) return;
+  const int t = task % g.tpr, row = task / g.tpr;
+  const int cy = row % g.ny, cz = row / g.ny;
+  const int x0 = task_x0(g, t), x1 = task_x0(g, t + 1);
+  int base = 0;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int rb = task_row_base(g, cy, cz, r);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      int a0, a1;
+      task_piece_cells(g.nx, x0, x1, p, a0, a1);
+      const int j0 = cstart[rb + a0];
+      const int len = a1 > a0 ? cstart[rb + a1] - j0 : 0;
+      tpiece[(size_t)task * kTaskPieces + 2 * r + p] = make_int4(j0, base, a0, a1);
+      base += len;
+    }
+  }
+  tstage[task] = base;
+  if (base > 65535) atomicOr(&flags[F_STAGE], 1);
 }
 
 __global__ void k_build_begin(int *__restrict__ flags, int *__restrict__ cell_count, int ncell,
@@ -859,6 +952,10 @@ __global__ void k_build_begin(int *__restrict__ flags, int *__restrict__ cell_co
   if (c < ncell) {
     cell_count[c] = 0;
     if (gcount) gcount[c] = 0;
+  }
+  if (c == 0) {  // statistics of this build (k_build_stats)
+    flags[F_MAXNBR] = 0;
+    flags[F_ENTRIES] = 0;
   }
 }
 
@@ -1134,12 +1231,19 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) 
     cb.cthr_in = b.cthr_in;
     cb.cthr_out = b.cthr_out;
     const int real = nblk(nc, 4), grid = (real + 7) / 8 * 8;
-    if (b.ghost_ok)
-      k_nlist_cells<true><<<grid, 256, 0, st>>>(s.pos, s.q, cb, s.box_dev, s.cell_start, s.nbr,
-                                                s.nnbr, s.stride, s.cap, s.flags, real, s.wstat);
-    else
-      k_nlist_cells<false><<<grid, 256, 0, st>>>(s.pos, s.q, cb, s.box_dev, s.cell_start, s.nbr,
-                                                 s.nnbr, s.stride, s.cap, s.flags, real, s.wstat);
+    if (s.slots)
+      k_task_table<<<nblk(s.tg.ntask, 256), 256, 0, st>>>(s.cell_start, s.tg, s.tpiece, s.tstage,
+                                                          s.flags);
+#define CV2_NLC(G, S)                                                                            \
+  if (b.ghost_ok == G && s.slots == S)                                                           \
+    k_nlist_cells<G, S><<<grid, 256, 0, st>>>(s.pos, s.q, cb, s.box_dev, s.cell_start, s.nbr,    \
+                                              s.nnbr, s.stride, s.cap, s.flags, real, s.wstat,   \
+                                              s.tg, s.tpiece, s.tstage);
+    CV2_NLC(true, true)
+    CV2_NLC(true, false)
+    CV2_NLC(false, true)
+    CV2_NLC(false, false)
+#undef CV2_NLC
     nw = real;
   } else if (n > 0) {
     k_nlist<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell_start, s.gcell_start,
@@ -1147,7 +1251,24 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) 
                                           ghosts ? 1 : 0, s.wstat);
     nw = nblk(n, 256);
   }
-  k_build_done<<<1, 1024, 0, st>>>(s.flags, s.wstat, nw, s.cap);
+  if (nw > 0) k_build_stats<<<std::min(nblk(nw, 256), 64), 256, 0, st>>>(s.flags, s.wstat, nw);
+  k_build_done<<<1, 64, 0, st>>>(s.flags, s.cap);
+}
+
+TaskGeom task_geom(int n, const BoxDev &b) {
+  TaskGeom g{};
+  g.nx = b.nc[0];
+  g.ny = b.nc[1];
+  g.nz = b.nc[2];
+  // about 224 home atoms per task (256 threads), with the staged
+  // neighbourhood, 9 (T + 2) cells of the mean density, within 95% of the
+  // LDS stage (denser tasks read their neighbours from q)
+  const double rho = (double)n / ((double)g.nx * g.ny * g.nz);
+  double T = rho > 0.0 ? 224.0 / rho : (double)g.nx;
+  while (T > 1.0 && 9.0 * (T + 2.0) * rho > 0.95 * kStage) T -= 0.25;
+  g.tpr = std::max(1, std::min(g.nx, (int)std::lround(g.nx / std::max(T, 1.0))));
+  g.ntask = g.ny * g.nz * g.tpr;
+  return g;
 }
 
 }  // namespace cv2

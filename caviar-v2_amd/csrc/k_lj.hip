@@ -155,94 +155,365 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-template <bool MULTI, bool ENERGY, bool HALO>
+// Constants of the FP32 pair arithmetic for one species pair.  With g = 1
+// the separation is in length units (dx = dq * u_d); on a box with one
+// fixed-point unit u on all axes (ISO) the separation is left in units of u
+// (g = u), which saves three multiplications per pair, and the unit is
+// folded into the constants:
+//   forceCoef * 2^32 * g = r2i' s6 (e6 - e12 s6) + fsh     r2i' = 1 / r2', r2' = r2 / g^2
+// with e6 = 6 (4 eps) 2^32 / g, e12 = 12 (4 eps) 2^32 / g, fsh = 4 eps fshift 2^32 g,
+// s6 = (sigma^2 / r2)^3 = (sig2' r2i')^3, sig2' = sigma^2 / g^2.
+struct PairF32 {
+  double rc2, sig2, r2ovf;  // in units of g^2
+  float e6, e12, fsh;
+  float eps4;
+  double ecut, ecut2;       // energy: ecut in units of 1/g^2
+  double evir;              // virial scale: g 2^-32 2^24 (fixed-point energy units)
+};
+
+// Species pair tp in length units (g = 1), from the precomputed FP32 tables.
+__device__ inline PairF32 pair_f32_table(const LJDev *__restrict__ lj, int tp) {
+  PairF32 c;
+  c.rc2 = lj->rc2[tp];
+  c.sig2 = lj->sig2[tp];
+  c.r2ovf = lj->r2ovf[tp];
+  c.e6 = lj->e6f[tp];
+  c.e12 = lj->e12f[tp];
+  c.fsh = lj->fshf[tp];
+  c.eps4 = lj->eps4f[tp];
+  c.ecut = lj->ecut[tp];
+  c.ecut2 = lj->ecut2[tp];
+  c.evir = kEnergyScale / kForceScale;
+  return c;
+}
+
+__device__ inline PairF32 pair_f32(const LJDev *__restrict__ lj, int tp, double g) {
+  PairF32 c;
+  const double g2i = 1.0 / (g * g);
+  c.rc2 = lj->rc2[tp] * g2i;
+  c.sig2 = lj->sig2[tp] * g2i;
+  c.r2ovf = lj->r2ovf[tp] * g2i;
+  const double e4s = lj->eps4[tp] * kForceScale / g;
+  c.e6 = (float)(6.0 * e4s);
+  c.e12 = (float)(12.0 * e4s);
+  c.fsh = (float)(lj->eps4[tp] * lj->fshift[tp] * kForceScale * g);
+  c.eps4 = lj->eps4f[tp];
+  c.ecut = lj->ecut[tp] * g * g;
+  c.ecut2 = lj->ecut2[tp];
+  c.evir = g * (kEnergyScale / kForceScale);
+  return c;
+}
+
+// One list entry of the FP32 path: separation from the fixed-point
+// coordinates (exact integer difference, one conversion), r^2 and the
+// cutoff test in FP64, r^-2 and (sigma^2/r^2)^3 in FP64 each rounded once to
+// FP32, the force polynomial in FP32, and the force component times 2^32
+// rounded to an integer inside one FP64 FMA and summed as a 64-bit integer.
+struct AccF32 {
+  unsigned long long sx = 0, sy = 0, sz = 0;
+  long long vir = 0, pe = 0;
+  int terms = 0;
+  bool ovf = false;
+};
+
+template <bool ISO, bool ENERGY>
+__device__ inline void pair_f32_entry(const int4 qi, const int4 qj, const double u0, const double u1,
+                                      const double u2, const PairF32 &c, bool once, AccF32 &a) {
+  double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y), dz = (double)(qj.z - qi.z);
+  if (!ISO) {
+    dx *= u0;
+    dy *= u1;
+    dz *= u2;
+  }
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  if (r2 <= c.rc2 && r2 > 0.0) {
+    double r2i = __builtin_amdgcn_rcp(r2);
+    r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
+    const double s2 = c.sig2 * r2i;
+    const float s6 = (float)(s2 * s2 * s2);
+    const float r2if = (float)r2i;
+    const float fc = fmaf(s6 * fmaf(-c.e12, s6, c.e6), r2if, c.fsh);
+    // |F| stays below 2^19 for r^2 >= r2ovf (fixed-point range)
+    a.ovf |= r2 < c.r2ovf;
+    const double fcs = (double)fc;
+    a.sx += (unsigned long long)__double_as_longlong(fma(fcs, dx, kMagic));
+    a.sy += (unsigned long long)__double_as_longlong(fma(fcs, dy, kMagic));
+    a.sz += (unsigned long long)__double_as_longlong(fma(fcs, dz, kMagic));
+    ++a.terms;
+    if (ENERGY && once) {
+      // -forceCoef r^2 and U in 2^-24 units
+      a.vir += __double_as_longlong(fma(-fcs * r2, c.evir, kMagic)) - kMagicBits;
+      const float e = c.eps4 * ((s6 * s6 - s6) + (float)(c.ecut * r2) - (float)c.ecut2);
+      a.pe += __double_as_longlong(fma((double)e, kEnergyScale, kMagic)) - kMagicBits;
+    }
+  }
+}
+
+__device__ inline double fx_value(unsigned long long s, int terms) {
+  const unsigned long long base = (unsigned long long)terms * (unsigned long long)kMagicBits;
+  return (double)(long long)(s - base);
+}
+
+template <bool MULTI, bool ENERGY, bool HALO, bool ISO>
 __global__ void __launch_bounds__(kBlock)
     k_lj_f32fx(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
                const int *__restrict__ nnbr, int stride, int n, const int *__restrict__ id,
                const LJDev *__restrict__ lj, BoxDev b, double *__restrict__ acc,
                long long *__restrict__ partials, int *__restrict__ flags, int nblk_real) {
+  static_assert(!(MULTI && ISO), "species tables are in length units");
   __shared__ long long sh[kBlock / 64];
   const int blk = xcd_block(nblk_real);
   if (blk < 0) return;
   const int i = blk * kBlock + threadIdx.x;
-  // length of one residual unit (2^-9 of the fixed-point quantum) per axis
-  const double qu0 = b.qinv[0] / kResidScale, qu1 = b.qinv[1] / kResidScale,
-               qu2 = b.qinv[2] / kResidScale;
-  long long vir = 0, pe = 0;
+  const double u0 = b.qinv[0], u1 = b.qinv[1], u2 = b.qinv[2];
+  AccF32 a;
   if (i < n) {
     const int4 qi = q[i];
     const int ti = MULTI ? (qi.w & 7) : 0;
     const int nt = MULTI ? lj->nt : 1;
-    const double rc2_0 = lj->rc2[0], sig2_0 = lj->sig2[0], r2ovf_0 = lj->r2ovf[0];
-    const float eps4_0 = lj->eps4f[0], fsh_0 = lj->fshiftf[0];
-    const int rix = q_resid(qi.w, 0), riy = q_resid(qi.w, 1), riz = q_resid(qi.w, 2);
+    const PairF32 c0 = pair_f32(lj, 0, ISO ? u0 : 1.0);
     const int cp = (nnbr[i] + 3) & ~3;
-    unsigned long long sx = 0, sy = 0, sz = 0;
-    int terms = 0;
-    bool ovf = false;
     const int id_i = (ENERGY && HALO) ? id[i] : 0;
     for (int k = 0; k < cp; k += 4) {
       const u32x4 g = load_group(nbr, (size_t)stride, i, k);
       const uint32_t e[4] = {g.x, g.y, g.z, g.w};
       int4 qj[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) qj[u] = q[e[u] & kIndexMask];
+      for (int v = 0; v < 4; ++v) qj[v] = q[e[v] & kIndexMask];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int j = (int)(e[u] & kIndexMask);
-        // exact separation in residual units: quanta * 512 + residual difference
-        const double dx = fma((double)(qj[u].x - qi.x), kResidScale,
-                              (double)(q_resid(qj[u].w, 0) - rix)) * qu0;
-        const double dy = fma((double)(qj[u].y - qi.y), kResidScale,
-                              (double)(q_resid(qj[u].w, 1) - riy)) * qu1;
-        const double dz = fma((double)(qj[u].z - qi.z), kResidScale,
-                              (double)(q_resid(qj[u].w, 2) - riz)) * qu2;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const int tp = MULTI ? ti * nt + (qj[u].w & 7) : 0;
-        if (r2 <= (MULTI ? lj->rc2[tp] : rc2_0) && r2 > 0.0) {
-          // r^-2 and (sigma^2/r^2)^3 in FP64, each rounded once to FP32;
-          // the force polynomial in FP32
-          double r2i = __builtin_amdgcn_rcp(r2);
-          r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
-          const double s2 = (MULTI ? lj->sig2[tp] : sig2_0) * r2i;
-          const float s6 = (float)(s2 * s2 * s2);
-          const float r2if = (float)r2i;
-          const float eps4 = MULTI ? lj->eps4f[tp] : eps4_0;
-          const float fc = eps4 * (r2if * (s6 * (-12.0f * s6 + 6.0f)) + (MULTI ? lj->fshiftf[tp] : fsh_0));
-          // |F| = |fc| r stays below 2^19 for r^2 >= r2ovf (fixed-point range)
-          ovf |= r2 < (MULTI ? lj->r2ovf[tp] : r2ovf_0);
-          // F * 2^32 rounded to an integer inside one FP64 FMA
-          const double fcd = (double)fc;
-          const double fcs = fcd * kForceScale;
-          sx += (unsigned long long)__double_as_longlong(fma(fcs, dx, kMagic));
-          sy += (unsigned long long)__double_as_longlong(fma(fcs, dy, kMagic));
-          sz += (unsigned long long)__double_as_longlong(fma(fcs, dz, kMagic));
-          ++terms;
-          if (ENERGY) {
-            const bool once = HALO && j >= n ? (id_i < id[j]) : (i < j);
-            if (once) {
-              vir += __double_as_longlong(fma(-fcd * r2, kEnergyScale, kMagic)) - kMagicBits;
-              const float u = eps4 * ((s6 * s6 - s6) + (float)(lj->ecut[tp] * r2) - (float)lj->ecut2[tp]);
-              pe += __double_as_longlong(fma((double)u, kEnergyScale, kMagic)) - kMagicBits;
-            }
-          }
-        }
+      for (int v = 0; v < 4; ++v) {
+        const int j = (int)(e[v] & kIndexMask);
+        const bool once = ENERGY && (HALO && j >= n ? (id_i < id[j]) : (i < j));
+        if (MULTI)
+          pair_f32_entry<false, ENERGY>(qi, qj[v], u0, u1, u2,
+                                        pair_f32_table(lj, ti * nt + (qj[v].w & 7)), once, a);
+        else
+          pair_f32_entry<ISO, ENERGY>(qi, qj[v], u0, u1, u2, c0, once, a);
       }
     }
-    const unsigned long long base = (unsigned long long)terms * (unsigned long long)kMagicBits;
+    // fc * dx and fc' * dq are both the force component times 2^32
     const double mi = lj->mass_inv[ti] * (1.0 / kForceScale);
-    acc[i] = (double)(long long)(sx - base) * mi;
-    acc[n + i] = (double)(long long)(sy - base) * mi;
-    acc[2 * n + i] = (double)(long long)(sz - base) * mi;
+    acc[i] = fx_value(a.sx, a.terms) * mi;
+    acc[n + i] = fx_value(a.sy, a.terms) * mi;
+    acc[2 * n + i] = fx_value(a.sz, a.terms) * mi;
+    if (a.ovf) atomicOr(&flags[F_FXOVF], 1);
+  }
+  if (ENERGY) {
+    const long long v = block_sum_i64(a.vir, sh);
+    __syncthreads();
+    const long long u = block_sum_i64(a.pe, sh);
+    if (threadIdx.x == 0) {  // per-block partials, summed by k_reduce_i64
+      partials[2 * blk] = v;
+      partials[2 * blk + 1] = u;
+    }
+  }
+}
+
+// ---- FP32 forces on wave-build boxes: one block per force task ----
+//
+// The block stages the task's neighbourhood (kTaskPieces atom ranges, see
+// TaskGeom) into LDS as int32 x, y, z arrays (+ type, + atom index for the
+// energy pass), in piece order, plus a sentinel at slot S, and each thread
+// takes one home atom of the task: its list holds 16-bit slots into the
+// staging area, 8 per 16-byte group (nbr16[((k/8)*stride + i)*8 + k%8]),
+// padded with the sentinel.  Neighbour reads are LDS reads instead of
+// gathers through the vector memory pipeline, and the list is half the
+// bytes of the 32-bit form.  A task whose neighbourhood exceeds the LDS
+// staging capacity (dense regions) reads its neighbours from q instead.
+//
+// The sentinel is the first home atom moved by 2^31 units in y and z: home
+// atoms share one cell row (|dy|, |dz| < w), and 2^31 units exceed 2 w on a
+// wave-build box, so it is more than w >= list cutoff from all of them.
+// The lists hold no self entries, so the pair arithmetic needs no r^2 > 0
+// test and runs without branches: a pair beyond the cutoff adds a zero
+// force, every entry (padding included) adds one magic bias.
+constexpr int kTaskBlock = 256;
+
+__device__ inline u32x4 load_group8(const uint32_t *__restrict__ nbr, size_t stride, int i, int k) {
+  return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(nbr) + (size_t)(k >> 3) * stride + i);
+}
+
+// Atom index of staging slot sl < S: the last piece whose slot base is <= sl
+// (empty pieces share the base of the next one).
+__device__ inline int slot_to_j(const int4 *pc, int sl) {
+  int p = 0;
+#pragma unroll
+  for (int k = 1; k < kTaskPieces; ++k) p = pc[k].y <= sl ? k : p;
+  return pc[p].x + (sl - pc[p].y);
+}
+
+// Branchless entry (single species): force component * 2^32 rounded inside
+// one FP64 FMA.  fcmax tracks the largest |forceCoef'| of a pair within the
+// cutoff: |fc' d| <= |fc'| rc on every axis, so fcmax * rc < 2^51 keeps each
+// product in the range of the rounding FMA (checked once per atom).
+template <bool ISO>
+__device__ inline void pair_f32_flat(const int4 qi, const int4 qj, double u0, double u1, double u2,
+                                     const PairF32 &c, unsigned long long &sx,
+                                     unsigned long long &sy, unsigned long long &sz,
+                                     float &fcmax) {
+  double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y), dz = (double)(qj.z - qi.z);
+  if (!ISO) {
+    dx *= u0;
+    dy *= u1;
+    dz *= u2;
+  }
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double r2i = __builtin_amdgcn_rcp(r2);
+  r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
+  const double s2 = c.sig2 * r2i;
+  const float s6 = (float)(s2 * s2 * s2);
+  const float r2if = (float)r2i;
+  const float fc = fmaf(s6 * fmaf(-c.e12, s6, c.e6), r2if, c.fsh);
+  float fsel = r2 <= c.rc2 ? fc : 0.0f;
+  __asm__ volatile("" : "+v"(fsel));  // select in FP32, before the conversion
+  fcmax = fmaxf(fcmax, fabsf(fsel));
+  const double fcs = (double)fsel;
+  sx += (unsigned long long)__double_as_longlong(fma(fcs, dx, kMagic));
+  sy += (unsigned long long)__double_as_longlong(fma(fcs, dy, kMagic));
+  sz += (unsigned long long)__double_as_longlong(fma(fcs, dz, kMagic));
+}
+
+// Coordinates (and type) of slot sl: LDS when staged, else q (the sentinel
+// at slot S built from the first home atom).
+template <bool STAGED, bool MULTI>
+__device__ inline int4 task_coord(const int *sx, const int *sy, const int *sz,
+                                  const unsigned char *stype, const int4 *__restrict__ q,
+                                  const int4 *pc, int S, const int4 sent, int sl) {
+  if (STAGED) return make_int4(sx[sl], sy[sl], sz[sl], MULTI ? (int)stype[sl] : 0);
+  const bool is_sent = sl == S;
+  const int4 v = q[slot_to_j(pc, is_sent ? 0 : sl)];  // selects, not a branch
+  return make_int4(is_sent ? sent.x : v.x, is_sent ? sent.y : v.y, is_sent ? sent.z : v.z,
+                   is_sent ? sent.w : v.w);
+}
+
+// All entries of home atom i (8 per group: slots decoded and coordinates
+// read for the whole group before the pair arithmetic).
+template <bool MULTI, bool ENERGY, bool ISO, bool STAGED>
+__device__ inline void task_atom(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
+                                 int stride, int i, int cp, const int4 qi, int ti, int nt,
+                                 const LJDev *__restrict__ lj, const PairF32 &c0, double u0,
+                                 double u1, double u2, const int *sx, const int *sy,
+                                 const int *sz, const unsigned char *stype, const int *sj,
+                                 const int4 *pc, int S, const int4 sent, AccF32 &a,
+                                 float &fcmax) {
+  if (cp == 0) return;
+  // the next group is loaded while this one is processed
+  u32x4 gn = load_group8(nbr, (size_t)stride, i, 0);
+  for (int k = 0; k < cp; k += 8) {
+    const u32x4 g = gn;
+    gn = load_group8(nbr, (size_t)stride, i, min(k + 8, cp - 8));
+    __builtin_amdgcn_sched_barrier(0);  // keep the load ahead of this group's work
+    const uint32_t w[4] = {g.x, g.y, g.z, g.w};
+    int sl[8];
+    int4 qj[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      sl[v] = (int)((w[v >> 1] >> (16 * (v & 1))) & 0xFFFFu);
+      qj[v] = task_coord<STAGED, MULTI>(sx, sy, sz, stype, q, pc, S, sent, sl[v]);
+    }
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      if (ENERGY || MULTI) {
+        int j = 0;
+        if (ENERGY) j = sl[v] == S ? 0x7fffffff : (STAGED ? sj[sl[v]] : slot_to_j(pc, sl[v]));
+        const PairF32 c = MULTI ? pair_f32_table(lj, ti * nt + (qj[v].w & 7)) : c0;
+        pair_f32_entry<ISO, ENERGY>(qi, qj[v], u0, u1, u2, c, ENERGY && i < j, a);
+      } else {
+        pair_f32_flat<ISO>(qi, qj[v], u0, u1, u2, c0, a.sx, a.sy, a.sz, fcmax);
+      }
+    }
+  }
+}
+
+template <bool MULTI, bool ENERGY, bool ISO>
+__global__ void __launch_bounds__(kTaskBlock)
+    k_lj_f32_task(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
+                  const int *__restrict__ nnbr, int stride, int n, const int *__restrict__ cstart,
+                  TaskGeom tg, const int4 *__restrict__ tpiece, const int *__restrict__ tstage,
+                  int stage_limit, const LJDev *__restrict__ lj, BoxDev b,
+                  double *__restrict__ acc, long long *__restrict__ partials,
+                  int *__restrict__ flags, int nblk_real) {
+  static_assert(!(MULTI && ISO), "species tables are in length units");
+  __shared__ int sx[kStage], sy[kStage], sz[kStage];
+  __shared__ unsigned char stype[MULTI ? kStage : 1];
+  __shared__ int sj[ENERGY ? kStage : 1];
+  __shared__ int4 pc[kTaskPieces];
+  __shared__ long long sh[kTaskBlock / 64];
+  const int task = xcd_block(nblk_real);
+  if (task < 0) return;
+  if (threadIdx.x < kTaskPieces) pc[threadIdx.x] = tpiece[(size_t)task * kTaskPieces + threadIdx.x];
+  const int S = tstage[task];
+  const bool staged = S + 1 <= stage_limit;  // uniform over the block (sentinel included)
+  const int t = task % tg.tpr, row = task / tg.tpr;
+  const int h0 = cstart[row * tg.nx + task_x0(tg, t)];
+  const int h1 = cstart[row * tg.nx + task_x0(tg, t + 1)];
+  int4 sent = make_int4(0, 0, 0, 0);
+  if (h0 < h1) {
+    sent = q[h0];
+    sent.y = (int)((unsigned)sent.y + 0x80000000u);
+    sent.z = (int)((unsigned)sent.z + 0x80000000u);
+  }
+  __syncthreads();
+  if (staged) {
+    // slots threadIdx.x + k * kTaskBlock; their pieces are non-decreasing in k
+    constexpr int K = kStage / kTaskBlock;
+    int jj[K];
+    int p = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int sl = k * kTaskBlock + (int)threadIdx.x;
+      while (p + 1 < kTaskPieces && pc[p + 1].y <= sl) ++p;
+      jj[k] = pc[p].x + (sl - pc[p].y);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int sl = k * kTaskBlock + (int)threadIdx.x;
+      if (sl <= S) {
+        const int4 v = sl < S ? q[jj[k]] : sent;
+        sx[sl] = v.x;
+        sy[sl] = v.y;
+        sz[sl] = v.z;
+        if (MULTI) stype[MULTI ? sl : 0] = (unsigned char)(v.w & 7);
+        if (ENERGY) sj[ENERGY ? sl : 0] = sl < S ? jj[k] : 0x7fffffff;
+      }
+    }
+  }
+  __syncthreads();
+  const double u0 = b.qinv[0], u1 = b.qinv[1], u2 = b.qinv[2];
+  const PairF32 c0 = pair_f32(lj, 0, ISO ? u0 : 1.0);
+  const double fc_rc = sqrt(c0.rc2);  // cutoff in the units of the separations
+  const int nt = MULTI ? lj->nt : 1;
+  long long vir = 0, pe = 0;
+  for (int i = h0 + (int)threadIdx.x; i < h1; i += kTaskBlock) {
+    const int4 qi = q[i];
+    const int ti = MULTI ? (qi.w & 7) : 0;
+    const int cp = (nnbr[i] + 7) & ~7;
+    AccF32 a;
+    float fcmax = 0.0f;
+    if (staged)
+      task_atom<MULTI, ENERGY, ISO, true>(q, nbr, stride, i, cp, qi, ti, nt, lj, c0, u0, u1, u2, sx,
+                                          sy, sz, stype, sj, pc, S, sent, a, fcmax);
+    else
+      task_atom<MULTI, ENERGY, ISO, false>(q, nbr, stride, i, cp, qi, ti, nt, lj, c0, u0, u1, u2,
+                                           sx, sy, sz, stype, sj, pc, S, sent, a, fcmax);
+    // the flat form biases every entry, the branched form counts its terms
+    const int terms = (ENERGY || MULTI) ? a.terms : cp;
+    const bool ovf = (ENERGY || MULTI) ? a.ovf : !((double)fcmax * fc_rc < 0.9 * 2251799813685248.0);
+    const double mi = lj->mass_inv[ti] * (1.0 / kForceScale);
+    acc[i] = fx_value(a.sx, terms) * mi;
+    acc[n + i] = fx_value(a.sy, terms) * mi;
+    acc[2 * n + i] = fx_value(a.sz, terms) * mi;
     if (ovf) atomicOr(&flags[F_FXOVF], 1);
+    vir += a.vir;
+    pe += a.pe;
   }
   if (ENERGY) {
     const long long v = block_sum_i64(vir, sh);
     __syncthreads();
     const long long u = block_sum_i64(pe, sh);
     if (threadIdx.x == 0) {  // per-block partials, summed by k_reduce_i64
-      partials[2 * blk] = v;
-      partials[2 * blk + 1] = u;
+      partials[2 * task] = v;
+      partials[2 * task + 1] = u;
     }
   }
 }
@@ -266,9 +537,10 @@ __global__ void __launch_bounds__(1024) k_reduce_i64(const long long *__restrict
 }
 
 // Ensemble form of k_lj_f32fx (single species): each atom reads the
-// fixed-point scale of its own system; ENERGY writes per-atom fixed-point
-// virial and energy (each pair once: i < j) for the per-system sums.
-template <int BLOCK, bool ENERGY>
+// fixed-point unit of its own system (ISO: every system has one unit on all
+// axes); ENERGY writes per-atom fixed-point virial and energy (each pair
+// once: i < j) for the per-system sums.
+template <int BLOCK, bool ENERGY, bool ISO>
 __global__ void __launch_bounds__(BLOCK)
     k_lj_ens(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
              const int *__restrict__ nnbr, int stride, int n, EnsDev e,
@@ -277,62 +549,29 @@ __global__ void __launch_bounds__(BLOCK)
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
   const BoxDev &b = e.box[e.sys[i]];
-  const double qu0 = b.qinv[0] / kResidScale, qu1 = b.qinv[1] / kResidScale,
-               qu2 = b.qinv[2] / kResidScale;
+  const double u0 = b.qinv[0], u1 = b.qinv[1], u2 = b.qinv[2];
   const int4 qi = q[i];
-  const double rc2_0 = lj->rc2[0], sig2_0 = lj->sig2[0], r2ovf_0 = lj->r2ovf[0];
-  const float eps4_0 = lj->eps4f[0], fsh_0 = lj->fshiftf[0];
-  const double ecut = lj->ecut[0], ecut2 = lj->ecut2[0];
-  const int rix = q_resid(qi.w, 0), riy = q_resid(qi.w, 1), riz = q_resid(qi.w, 2);
+  const PairF32 c = pair_f32(lj, 0, ISO ? u0 : 1.0);
   const int cp = (nnbr[i] + 3) & ~3;
-  unsigned long long sx = 0, sy = 0, sz = 0;
-  long long vir = 0, pe = 0;
-  int terms = 0;
-  bool ovf = false;
+  AccF32 a;
   for (int k = 0; k < cp; k += 4) {
     const u32x4 g = load_group(nbr, (size_t)stride, i, k);
     const uint32_t ent[4] = {g.x, g.y, g.z, g.w};
     int4 qj[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) qj[u] = q[ent[u] & kIndexMask];
+    for (int v = 0; v < 4; ++v) qj[v] = q[ent[v] & kIndexMask];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int j = (int)(ent[u] & kIndexMask);
-      const double dx = fma((double)(qj[u].x - qi.x), kResidScale, (double)(q_resid(qj[u].w, 0) - rix)) * qu0;
-      const double dy = fma((double)(qj[u].y - qi.y), kResidScale, (double)(q_resid(qj[u].w, 1) - riy)) * qu1;
-      const double dz = fma((double)(qj[u].z - qi.z), kResidScale, (double)(q_resid(qj[u].w, 2) - riz)) * qu2;
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 <= rc2_0 && r2 > 0.0) {
-        double r2i = __builtin_amdgcn_rcp(r2);
-        r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
-        const double s2 = sig2_0 * r2i;
-        const float s6 = (float)(s2 * s2 * s2);
-        const float r2if = (float)r2i;
-        const float fc = eps4_0 * (r2if * (s6 * (-12.0f * s6 + 6.0f)) + fsh_0);
-        ovf |= r2 < r2ovf_0;
-        const double fcd = (double)fc;
-        const double fcs = fcd * kForceScale;
-        sx += (unsigned long long)__double_as_longlong(fma(fcs, dx, kMagic));
-        sy += (unsigned long long)__double_as_longlong(fma(fcs, dy, kMagic));
-        sz += (unsigned long long)__double_as_longlong(fma(fcs, dz, kMagic));
-        ++terms;
-        if (ENERGY && i < j) {
-          vir += __double_as_longlong(fma(-fcd * r2, kEnergyScale, kMagic)) - kMagicBits;
-          const float u = eps4_0 * ((s6 * s6 - s6) + (float)(ecut * r2) - (float)ecut2);
-          pe += __double_as_longlong(fma((double)u, kEnergyScale, kMagic)) - kMagicBits;
-        }
-      }
-    }
+    for (int v = 0; v < 4; ++v)
+      pair_f32_entry<ISO, ENERGY>(qi, qj[v], u0, u1, u2, c, i < (int)(ent[v] & kIndexMask), a);
   }
-  const unsigned long long base = (unsigned long long)terms * (unsigned long long)kMagicBits;
   const double mi = lj->mass_inv[0] * (1.0 / kForceScale);
-  acc[i] = (double)(long long)(sx - base) * mi;
-  acc[n + i] = (double)(long long)(sy - base) * mi;
-  acc[2 * n + i] = (double)(long long)(sz - base) * mi;
-  if (ovf) atomicOr(&flags[F_FXOVF], 1);
+  acc[i] = fx_value(a.sx, a.terms) * mi;
+  acc[n + i] = fx_value(a.sy, a.terms) * mi;
+  acc[2 * n + i] = fx_value(a.sz, a.terms) * mi;
+  if (a.ovf) atomicOr(&flags[F_FXOVF], 1);
   if (ENERGY) {
-    eatom[2 * (size_t)i] = vir;
-    eatom[2 * (size_t)i + 1] = pe;
+    eatom[2 * (size_t)i] = a.vir;
+    eatom[2 * (size_t)i + 1] = a.pe;
   }
 }
 
@@ -366,28 +605,54 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
     if (ENERGY) k_reduce_partials<<<1, 1024, 0, st>>>(s.partials, real, 2, s.reduce_out);
   } else {
     long long *part = reinterpret_cast<long long *>(s.partials);
-    k_lj_f32fx<MULTI, ENERGY, HALO><<<grid, kBlock, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n,
-                                                             s.id, lj_dev, b, s.acc, part,
-                                                             s.flags, real);
+    if (s.slots) {
+      const int treal = s.tg.ntask, tgrid = ((treal + 7) / 8) * 8;
+#define CV2_LJT(I)                                                                             \
+  k_lj_f32_task<MULTI, ENERGY, I><<<tgrid, kTaskBlock, 0, st>>>(                               \
+      s.q, s.nbr, s.nnbr, s.stride, s.n, s.cell_start, s.tg, s.tpiece, s.tstage, s.stage_limit, \
+      lj_dev, b, s.acc, part, s.flags, treal)
+      if (!MULTI && b.qiso)
+        CV2_LJT(!MULTI);
+      else
+        CV2_LJT(false);
+#undef CV2_LJT
+      if (ENERGY) k_reduce_i64<<<1, 1024, 0, st>>>(part, treal, s.fx_acc);
+      return;
+    }
+    if (!MULTI && b.qiso)
+      k_lj_f32fx<false, ENERGY, HALO, true><<<grid, kBlock, 0, st>>>(
+          s.q, s.nbr, s.nnbr, s.stride, s.n, s.id, lj_dev, b, s.acc, part, s.flags, real);
+    else
+      k_lj_f32fx<MULTI, ENERGY, HALO, false><<<grid, kBlock, 0, st>>>(
+          s.q, s.nbr, s.nnbr, s.stride, s.n, s.id, lj_dev, b, s.acc, part, s.flags, real);
     if (ENERGY) k_reduce_i64<<<1, 1024, 0, st>>>(part, real, s.fx_acc);
   }
 }
 
 }  // namespace
 
-void launch_forces_ens(DevState &s, const EnsDev &e, const LJDev *lj_dev, int block, bool energy,
-                       long long *eatom, hipStream_t st) {
+template <int B, bool ISO>
+void lj_ens_dispatch(DevState &s, const EnsDev &e, const LJDev *lj_dev, bool energy,
+                     long long *eatom, hipStream_t st) {
+  const int g = (s.n + B - 1) / B;
+  if (energy)
+    k_lj_ens<B, true, ISO><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev, s.acc,
+                                            eatom, s.flags);
+  else
+    k_lj_ens<B, false, ISO><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev,
+                                             s.acc, eatom, s.flags);
+}
+
+void launch_forces_ens(DevState &s, const EnsDev &e, const LJDev *lj_dev, int block, bool iso,
+                       bool energy, long long *eatom, hipStream_t st) {
   if (s.n == 0) return;
-#define CV2_LJE(B)                                                                            \
-  if (block == B) {                                                                           \
-    const int g = (s.n + B - 1) / B;                                                          \
-    if (energy)                                                                               \
-      k_lj_ens<B, true><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev, s.acc, \
-                                          eatom, s.flags);                                    \
-    else                                                                                      \
-      k_lj_ens<B, false><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev,       \
-                                           s.acc, eatom, s.flags);                            \
-    return;                                                                                   \
+#define CV2_LJE(B)                                                    \
+  if (block == B) {                                                   \
+    if (iso)                                                          \
+      lj_ens_dispatch<B, true>(s, e, lj_dev, energy, eatom, st);      \
+    else                                                              \
+      lj_ens_dispatch<B, false>(s, e, lj_dev, energy, eatom, st);     \
+    return;                                                           \
   }
   CV2_LJE(64)
   CV2_LJE(128)

@@ -1,7 +1,7 @@
 """Soundness of the coarse distance filter of the wave-per-cell list build.
 
 The build accepts a candidate without the FP64 predicate when its float
-squared distance (from the 32-bit fixed-point coordinates, relative to a
+squared distance (from the int32 fixed-point coordinates, relative to a
 reference atom of the home cell) is below cthr_in, and rejects it when at or
 above cthr_out.  This test reproduces the device arithmetic in float32 with
 numpy on pairs placed within 3e-5 of the list cutoff, in boxes of several
@@ -15,8 +15,19 @@ import numpy as np
 import pytest
 
 
-def thresholds(L, ce, wmax):
-    u = np.asarray(L, np.float64) / 4294967296.0
+def qunit(L, span):
+    # set_qscale on a fully periodic box: u_d = L_d / 2^(32 + k_d), k_d the
+    # largest shift with span < L_d / 2^(k_d + 1) (0.1% margin)
+    u = []
+    for Ld in L:
+        k = 0
+        while k < 30 and math.ldexp(span * 1.001, k + 2) < Ld:
+            k += 1
+        u.append(math.ldexp(Ld, -(32 + k)))
+    return np.asarray(u)
+
+
+def thresholds(u, ce, wmax):
     ef = 8.0 * 5.9604644775390625e-08 * (2.0 * wmax + ce) * math.sqrt(3.0)
     e = 1.01 * math.sqrt(float((u * u).sum())) + ef + 1e-6 * ce
     tin = np.nextafter(np.float32((ce - e) * (ce - e) * (1.0 - 4e-6)), np.float32(0))
@@ -24,9 +35,9 @@ def thresholds(L, ce, wmax):
     return tin, tout, u.astype(np.float32)
 
 
-def quantize(x, L):
-    # pack_q: llrint((x - lo) * 2^32 / L) kept modulo 2^32 (lo = 0 here)
-    q = np.rint(x * (4294967296.0 / L)).astype(np.int64)
+def quantize(x, u):
+    # pack_q: llrint((x - lo) / u) kept modulo 2^32 (lo = 0 here)
+    q = np.rint(x * (1.0 / u)).astype(np.int64)
     return (q & 0xFFFFFFFF).astype(np.uint32)
 
 
@@ -37,7 +48,8 @@ def test_coarse_filter_never_contradicts_fp64(L):
     Ls = np.asarray(L)
     nc = np.floor(Ls / (ce * (1 + 1e-12)))
     w = Ls / nc
-    tin, tout, unit = thresholds(Ls, ce, w.max())
+    u = qunit(Ls, 2.0 * max(w.max(), ce))
+    tin, tout, unit = thresholds(u, ce, w.max())
     n = 400000
     ref = rng.uniform(0, 1, (n, 3)) * Ls             # reference atom of the home cell
     pi = ref + rng.uniform(-1, 1, (n, 3)) * w         # home atom near it
@@ -53,7 +65,7 @@ def test_coarse_filter_never_contradicts_fp64(L):
     r2 = (dx[:, 0] * dx[:, 0] + dx[:, 1] * dx[:, 1]) + dx[:, 2] * dx[:, 2]
     exact = r2 < ce * ce
     # device arithmetic: int32 differences from the reference atom, float32
-    qo, qi, qj = quantize(ref, Ls), quantize(pi, Ls), quantize(pj, Ls)
+    qo, qi, qj = quantize(ref, u), quantize(pi, u), quantize(pj, u)
     rel = lambda q: (q - qo).view(np.int32).astype(np.float32) * unit
     fd = rel(qj) - rel(qi)
     r2c = fd[:, 0] * fd[:, 0] + fd[:, 1] * fd[:, 1] + fd[:, 2] * fd[:, 2]

@@ -104,12 +104,14 @@ WAVE_CASES = {
 }
 
 
+@pytest.mark.parametrize("prec", [D.CV2_PREC_FP64, D.CV2_PREC_FP32_FIXED])
 @pytest.mark.parametrize("name", sorted(WAVE_CASES))
-def test_pair_set_cell_wave_build(name):
-    """Systems that take the wavefront-per-cell build: bit-exact pair sets."""
+def test_pair_set_cell_wave_build(name, prec):
+    """Systems that take the wavefront-per-cell build: bit-exact pair sets,
+    from the 32-bit list (FP64) and from the 16-bit task-slot list (FP32)."""
     s = WAVE_CASES[name]()
-    print(f"[pairs/cell-wave] {name} n={len(s['pos'])}", flush=True)
-    with D.Context.from_sysdef(s) as c:
+    print(f"[pairs/cell-wave] {name} prec={prec} n={len(s['pos'])}", flush=True)
+    with D.Context.from_sysdef(s, precision=prec) as c:
         c.build(True)
         st = c.stats()
         got = sort_rows(c.pairs())
@@ -152,6 +154,26 @@ def test_forces_virial_energy(case, prec):
         assert emax < FP32_TOL_MAX and erms < FP32_TOL_RMS
         assert abs(th["virial"] - vir) < 1e-5 * vsc
         assert abs(th["pe"] - pe) < 1e-5 * esc
+
+
+@pytest.mark.parametrize("name,mid", [("c1_fcc", "1000"), ("ka_4000", "1440")])
+def test_fp32_unstaged_tasks_identical(name, mid, monkeypatch):
+    """Force tasks whose neighbourhood exceeds the LDS staging limit read
+    their neighbours from global memory instead: the same bits (limit 0: every
+    task; mid: about the mean staged count, so some tasks of each kind)."""
+    s = WAVE_CASES[name]()
+    out = {}
+    for lim in ("3072", "0", mid):
+        monkeypatch.setenv("CV2_STAGE_LIMIT", lim)
+        print(f"[stage-limit] {name} limit={lim}", flush=True)
+        with D.Context.from_sysdef(s, precision=D.CV2_PREC_FP32_FIXED) as c:
+            c.compute_forces()
+            th = c.thermo()
+            out[lim] = (c.download()[2], th["virial"], th["pe"])
+    monkeypatch.delenv("CV2_STAGE_LIMIT")
+    for lim in ("0", mid):
+        assert np.array_equal(out[lim][0], out["3072"][0])
+        assert out[lim][1:] == out["3072"][1:]
 
 
 def test_pairs_after_dynamics_bit_exact():
