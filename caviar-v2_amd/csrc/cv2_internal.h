@@ -128,12 +128,12 @@ struct TaskGeom {
 };
 
 __host__ __device__ inline int task_x0(const TaskGeom &g, int t) {
-  return (int)(((long long)t * g.nx) / g.tpr);
+  return t * g.nx / g.tpr;  // nx, tpr <= 1024
 }
 
 // Task (within its row) of x-cell cx.
 __host__ __device__ inline int task_of_x(const TaskGeom &g, int cx) {
-  int t = (int)(((long long)cx * g.tpr) / g.nx);
+  int t = cx * g.tpr / g.nx;
   while (t + 1 < g.tpr && task_x0(g, t + 1) <= cx) ++t;
   return t;
 }

@@ -624,6 +624,8 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
   }
   uint16_t *buf16 = reinterpret_cast<uint16_t *>(buf);
   uint16_t *nbr16 = reinterpret_cast<uint16_t *>(nbr);
+  const uint32_t cin_m1 = __float_as_uint(b.cthr_in) - 1u;
+  const float bmid = 0.5f * (b.cthr_in + b.cthr_out), bhw = b.cthr_out - b.cthr_in;
   // lane-private scratch word for lanes that accept nothing (STAGED)
   const int dummy_at = SLOTS ? kPassCand + 8 + lane : kPassCand + 4 + lane;
   for (int h = h0; h < h1; ++h) {
@@ -659,6 +661,9 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
       cnt += __popcll(m);
     };
     unsigned long long anyband = 0ull;
+    // GHOST_OK: per-lane pre-test of the band, min |r2 - mid| against its
+    // full width (conservative: the band path re-tests exactly)
+    float bm = 1e30f;
 #pragma unroll
     for (int g = 0; g < kSlots; g += 3) {
       if (g * 64 >= ncand) continue;  // uniform; keeps the loops unrolled
@@ -666,12 +671,22 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
       for (int k = g; k < (g + 3 < kSlots ? g + 3 : kSlots); ++k) {
         const float dx = px[k] - hx, dy = py[k] - hy, dz = pz[k] - hz;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const unsigned long long m_in = __builtin_amdgcn_ballot_w64(r2 < (GHOST_OK ? b.cthr_in : tin[GHOST_OK ? 0 : k]));
-        const unsigned long long m_out = __builtin_amdgcn_ballot_w64(r2 < b.cthr_out);
-        anyband |= m_out & ~m_in;  // the home atom itself is in m_in
-        append(k == kself ? m_in & ~selfbit : m_in, ent[k]);
+        if (GHOST_OK) {
+          // 0 < r2 < cthr_in as one unsigned compare of the float bits
+          // (r2 >= +0): the home atom itself (r2 = 0) fails it.  Vector
+          // instructions only; the scalar unit is shared by four SIMDs.
+          const unsigned long long m_in = __builtin_amdgcn_ballot_w64((__float_as_uint(r2) - 1u) < cin_m1);
+          bm = fminf(bm, fabsf(r2 - bmid));
+          append(m_in, ent[k]);
+        } else {
+          const unsigned long long m_in = __builtin_amdgcn_ballot_w64(r2 < tin[GHOST_OK ? 0 : k]);
+          const unsigned long long m_out = __builtin_amdgcn_ballot_w64(r2 < b.cthr_out);
+          anyband |= m_out & ~m_in;  // the home atom itself is in m_in
+          append(k == kself ? m_in & ~selfbit : m_in, ent[k]);
+        }
       }
     }
+    if (GHOST_OK) anyband = __builtin_amdgcn_ballot_w64(bm < bhw);
     // Candidates in the band between the coarse thresholds (rare): the FP64
     // predicate decides, with the ghost-existence rule for images.  Kept
     // out of the loop above so that its global loads never make the common

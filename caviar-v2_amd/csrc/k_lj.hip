@@ -328,6 +328,7 @@ __global__ void __launch_bounds__(kBlock)
 // test and runs without branches: a pair beyond the cutoff adds a zero
 // force, every entry (padding included) adds one magic bias.
 constexpr int kTaskBlock = 256;
+constexpr int kSortBins = 64;  // list lengths in groups of 8 (longer lists share the last bin)
 
 __device__ inline u32x4 load_group8(const uint32_t *__restrict__ nbr, size_t stride, int i, int k) {
   return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(nbr) + (size_t)(k >> 3) * stride + i);
@@ -439,6 +440,8 @@ __global__ void __launch_bounds__(kTaskBlock)
   __shared__ int sj[ENERGY ? kStage : 1];
   __shared__ int4 pc[kTaskPieces];
   __shared__ long long sh[kTaskBlock / 64];
+  __shared__ int lhist[kSortBins + 1];
+  __shared__ unsigned char lorder[kTaskBlock];
   const int task = xcd_block(nblk_real);
   if (task < 0) return;
   if (threadIdx.x < kTaskPieces) pc[threadIdx.x] = tpiece[(size_t)task * kTaskPieces + threadIdx.x];
@@ -478,13 +481,34 @@ __global__ void __launch_bounds__(kTaskBlock)
       }
     }
   }
+  // A wavefront runs until its longest list is done: give each wavefront
+  // home atoms of similar list length (counting sort on the group count,
+  // longest first).  Which thread computes an atom does not change its
+  // result.
+  const int nh = h1 - h0;
+  const bool sorted = nh <= kTaskBlock;
+  if (sorted) {
+    for (int k = threadIdx.x; k <= kSortBins; k += kTaskBlock) lhist[k] = 0;
+    __syncthreads();
+    int bin = 0, rank = 0;
+    if ((int)threadIdx.x < nh) {
+      bin = kSortBins - 1 - min((nnbr[h0 + threadIdx.x] + 7) >> 3, kSortBins - 1);
+      rank = atomicAdd(&lhist[bin + 1], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int k = 1; k <= kSortBins; ++k) lhist[k] += lhist[k - 1];
+    __syncthreads();
+    if ((int)threadIdx.x < nh) lorder[lhist[bin] + rank] = (unsigned char)threadIdx.x;
+  }
   __syncthreads();
   const double u0 = b.qinv[0], u1 = b.qinv[1], u2 = b.qinv[2];
   const PairF32 c0 = pair_f32(lj, 0, ISO ? u0 : 1.0);
   const double fc_rc = sqrt(c0.rc2);  // cutoff in the units of the separations
   const int nt = MULTI ? lj->nt : 1;
   long long vir = 0, pe = 0;
-  for (int i = h0 + (int)threadIdx.x; i < h1; i += kTaskBlock) {
+  for (int t = (int)threadIdx.x; t < nh; t += kTaskBlock) {
+    const int i = h0 + (sorted ? (int)lorder[t] : t);
     const int4 qi = q[i];
     const int ti = MULTI ? (qi.w & 7) : 0;
     const int cp = (nnbr[i] + 7) & ~7;
