@@ -215,6 +215,8 @@ int setup_box(cv2_ctx *ctx) {
   if (ncell != ctx->s.ncell) {
     ctx->s.ncell = (int)ncell;
     HIPCHK(dalloc(&ctx->s.cell_count, ncell + 1));
+    // builds leave the counts cleared for the next one (k_rank_move)
+    HIPCHK(hipMemsetAsync(ctx->s.cell_count, 0, sizeof(int) * (ncell + 1), ctx->stream));
     HIPCHK(dalloc(&ctx->s.cell_start, ncell + 1));
     HIPCHK(dalloc(&ctx->s.gcell_count, ncell + 1));
     HIPCHK(dalloc(&ctx->s.gcell_start, ncell + 1));
@@ -288,7 +290,7 @@ int fetch_energy(cv2_ctx *ctx) {
   return CV2_OK;
 }
 
-int queue_build(cv2_ctx *ctx, bool force) {
+int queue_build(cv2_ctx *ctx, bool force, bool keep_acc) {
   force = force || ctx->need_build;
   ctx->need_build = false;
   if (force) {
@@ -297,7 +299,7 @@ int queue_build(cv2_ctx *ctx, bool force) {
   }
   if (force || (ctx->nl_flags & CV2_NL_REBUILD_EVERY_STEP))
     launch_set_flag(ctx->s.flags, F_REBUILD, 1, ctx->stream);
-  launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo);
+  launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, keep_acc);
   return CV2_OK;
 }
 
@@ -645,7 +647,7 @@ int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt) {
     HIPCHK(hipMemcpy(&before, ctx->s.flags + F_REBUILDS, sizeof(int), hipMemcpyDeviceToHost));
   }
   if (!force) launch_check_disp(ctx->s, ctx->b, ctx->stream);
-  rc = queue_build(ctx, force != 0);
+  rc = queue_build(ctx, force != 0, true);
   if (rc) return rc;
   ctx->forces_valid = false;
   rc = check_flags(ctx);
@@ -659,7 +661,7 @@ int cv2_compute_forces(cv2_ctx *ctx) {
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
   if (ctx->need_build) {
-    rc = queue_build(ctx, true);
+    rc = queue_build(ctx, true, false);
     if (rc) return rc;
   }
   if (ctx->precision != CV2_PREC_FP64) launch_pack_q(ctx->s, ctx->b, ctx->stream);
@@ -690,7 +692,7 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
     span_end(ctx, t, 2);
     t = span_begin(ctx);
     {
-      const int rc = queue_build(ctx, false);
+      const int rc = queue_build(ctx, false, false);
       if (rc) return rc;
     }
     span_end(ctx, t, 1);

@@ -38,7 +38,8 @@ enum Flag : int {
   F_REBUILDS = 5,  // number of builds done on the device
   F_ENTRIES = 6,   // list entries of the last build (low 32 bits)
   F_STAGE = 7,     // a force task's neighbourhood exceeds 16-bit slots
-  F_COUNT = 8
+  F_STATBLK = 8,   // blocks of k_build_stats done (reset by the last one)
+  F_COUNT = 10
 };
 
 struct BoxDev {
@@ -219,7 +220,9 @@ bool build_lj_table(LJDev &L, int ntypes, const double *eps, const double *sigma
 
 // ---- launchers (defined in the .hip files) ----
 void launch_pack_q(const DevState &s, const BoxDev &b, hipStream_t st);
-void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo);
+// keep_acc: carry the accelerations into the new order (false when a force
+// pass follows the build before anything reads them)
+void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, bool keep_acc);
 void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                    int precision, bool energy, hipStream_t st);
 // Task geometry for n atoms on b's cell grid (same result for the build and

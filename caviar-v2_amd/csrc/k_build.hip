@@ -50,9 +50,13 @@ __device__ inline int cell_of(const double4 &p, const BoxDev &b) {
 // atomic add (the in-cell order is fixed afterwards by k_rank).
 __global__ void k_wrap_bin(double4 *__restrict__ pos, int n, BoxDev b, int *__restrict__ cell,
                            int *__restrict__ slot, int *__restrict__ cell_count,
-                           const int *__restrict__ flags) {
+                           int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {  // statistics of this build (k_build_stats)
+    flags[F_MAXNBR] = 0;
+    flags[F_ENTRIES] = 0;
+  }
   const int lane = threadIdx.x & 63;
   const bool valid = i < n;
   int ci = -1;
@@ -147,46 +151,43 @@ __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ 
   perm[start[cell[i]] + slot[i]] = i;
 }
 
-// Order atoms inside each cell by upload index so that the sorted order,
-// and with it every list and sum, is independent of atomic arrival order.
-// One thread per atom: its rank in its cell is the number of cell members
-// with a smaller upload index (indices are unique).
-__global__ void k_rank(const int *__restrict__ cell, const int *__restrict__ start,
-                       const int *__restrict__ perm, const int *__restrict__ tag,
-                       int *__restrict__ order, int n, const int *__restrict__ flags) {
+// Order atoms inside each cell by their position in the previous order so
+// that the sorted order, and with it every list and sum, is independent of
+// atomic arrival order (atoms that stay in a cell keep their relative
+// order).  One thread per atom: its rank in its cell is the number of cell
+// members with a smaller previous index (perm lists them; indices are
+// unique), and it moves its own data to that place in the *_out arrays
+// (acc: only when given), and its position to pos_ref (the build
+// positions).  The cell counts, consumed by the scan, are cleared for the
+// next build.
+__global__ void k_rank_move(const int *__restrict__ cell, const int *__restrict__ start,
+                            const int *__restrict__ perm, int n, const double4 *__restrict__ pos,
+                            double4 *__restrict__ pos_out, double4 *__restrict__ pos_ref,
+                            const double *__restrict__ vel, double *__restrict__ vel_out,
+                            const double *__restrict__ acc, double *__restrict__ acc_out,
+                            const int *__restrict__ tag, int *__restrict__ tag_out,
+                            const int *__restrict__ id, int *__restrict__ id_out,
+                            int *__restrict__ cell_count, int ncell,
+                            const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int c = i; c < ncell; c += gridDim.x * blockDim.x) cell_count[c] = 0;
   if (i >= n) return;
   const int c = cell[i];
   const int a = start[c], e = start[c + 1];
-  const int t = tag[i];
   int rank = 0;
-  for (int k = a; k < e; ++k) rank += tag[perm[k]] < t;
-  order[a + rank] = i;
-}
-
-// Gather every per-atom array into the new order; remember build positions.
-__global__ void k_permute(const int *__restrict__ perm, int n, const double4 *__restrict__ pos,
-                          double4 *__restrict__ pos_out, double4 *__restrict__ pos_ref,
-                          const double *__restrict__ vel, double *__restrict__ vel_out,
-                          const double *__restrict__ acc, double *__restrict__ acc_out,
-                          const int *__restrict__ tag, int *__restrict__ tag_out,
-                          const int *__restrict__ id, int *__restrict__ id_out,
-                          const int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int o = perm[k];
-  const double4 p = pos[o];
+  for (int k = a; k < e; ++k) rank += perm[k] < i;
+  const int k = a + rank;
+  const double4 p = pos[i];
   pos_out[k] = p;
   pos_ref[k] = p;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    vel_out[d * n + k] = vel[d * n + o];
-    acc_out[d * n + k] = acc[d * n + o];
+    vel_out[d * n + k] = vel[d * n + i];
+    if (acc) acc_out[d * n + k] = acc[d * n + i];
   }
-  tag_out[k] = tag[o];
-  id_out[k] = id[o];
+  tag_out[k] = tag[i];
+  id_out[k] = id[i];
 }
 
 // Fixed-point copy of positions for the FP32 pair kernel: (x - lo) * 2^32 / L
@@ -212,7 +213,7 @@ __global__ void k_copy_back(int n, const double4 *__restrict__ pos_alt, double4 
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     vel[d * n + k] = vel_alt[d * n + k];
-    acc[d * n + k] = acc_alt[d * n + k];
+    if (acc) acc[d * n + k] = acc_alt[d * n + k];
   }
   tag[k] = tag_alt[k];
   id[k] = id_alt[k];
@@ -413,7 +414,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // Per-block build statistics (longest list, entries kept), reduced by
-// k_build_done: no same-address atomics from thousands of blocks.
+// k_build_stats: no same-address atomics from thousands of blocks.
 __device__ inline void block_stat(int2 *bstat, int cnt, int kept) {
   __shared__ int2 sh[4];
   for (int o = 32; o > 0; o >>= 1) {
@@ -884,18 +885,19 @@ __global__ void __launch_bounds__(256)
                                     lds, cslot_sh[wv], cnt_sh, tg, tpiece, tstage);
   if (lane == 0) st_sh[wv] = st;
   __syncthreads();
-  if (threadIdx.x == 0) {  // per-block statistics, reduced by k_build_done
+  if (threadIdx.x == 0) {  // per-block statistics, reduced by k_build_stats
     int2 t = st_sh[0];
     for (int w = 1; w < 4; ++w) t = make_int2(max(t.x, st_sh[w].x), t.y + st_sh[w].y);
     bstat[blk] = t;
   }
 }
 
-// Close a build: reduce the per-block statistics (k_build_stats: 256
-// blocks, one atomic per block into flags[F_MAXNBR] / flags[F_ENTRIES],
-// which k_build_begin cleared), then check them and clear the request.
+// Close a build: reduce the per-block statistics of the list kernel (a few
+// blocks, one atomic each into flags[F_MAXNBR] / flags[F_ENTRIES], which
+// k_wrap_bin cleared); the last block to finish checks the capacity and
+// clears the request.
 __global__ void __launch_bounds__(256)
-    k_build_stats(int *__restrict__ flags, const int2 *__restrict__ wstat, int nw) {
+    k_build_stats(int *__restrict__ flags, const int2 *__restrict__ wstat, int nw, int cap) {
   if (flags[F_REBUILD] == 0) return;
   __shared__ int smax[4], ssum[4];
   int m = 0, t = 0;
@@ -918,15 +920,14 @@ __global__ void __launch_bounds__(256)
     t = ssum[0] + ssum[1] + ssum[2] + ssum[3];
     if (m > 0) atomicMax(&flags[F_MAXNBR], m);
     if (t > 0) atomicAdd(&flags[F_ENTRIES], t);
-  }
-}
-
-__global__ void k_build_done(int *__restrict__ flags, int cap) {
-  if (flags[F_REBUILD] == 0) return;
-  if (threadIdx.x == 0) {
-    if (flags[F_MAXNBR] > cap) flags[F_OVERFLOW] = 1;
-    flags[F_REBUILD] = 0;
-    flags[F_REBUILDS] += 1;
+    __threadfence();
+    if (atomicAdd(&flags[F_STATBLK], 1) == (int)gridDim.x - 1) {
+      // every block's atomics are done
+      if (atomicAdd(&flags[F_MAXNBR], 0) > cap) flags[F_OVERFLOW] = 1;
+      flags[F_STATBLK] = 0;
+      flags[F_REBUILD] = 0;
+      flags[F_REBUILDS] += 1;
+    }
   }
 }
 
@@ -967,10 +968,6 @@ __global__ void k_build_begin(int *__restrict__ flags, int *__restrict__ cell_co
   if (c < ncell) {
     cell_count[c] = 0;
     if (gcount) gcount[c] = 0;
-  }
-  if (c == 0) {  // statistics of this build (k_build_stats)
-    flags[F_MAXNBR] = 0;
-    flags[F_ENTRIES] = 0;
   }
 }
 
@@ -1112,7 +1109,7 @@ __global__ void k_ens_copy_back(int n, EnsDev e, const double4 *__restrict__ pos
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     vel[d * n + k] = vel_alt[d * n + k];
-    acc[d * n + k] = acc_alt[d * n + k];
+    if (acc) acc[d * n + k] = acc_alt[d * n + k];
   }
   tag[k] = tag_alt[k];
 }
@@ -1208,24 +1205,28 @@ void launch_pack_q(const DevState &s, const BoxDev &b, hipStream_t st) {
   k_pack_q<<<nblk(ntot, 256), 256, 0, st>>>(s.pos, s.q, ntot, b);
 }
 
-void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) {
+void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, bool keep_acc) {
   const int n = s.n;
   const int nc = s.ncell;
   const bool ghosts = host_halo && s.ng > 0;
-  k_build_begin<<<nblk(nc, 256), 256, 0, st>>>(s.flags, s.cell_count, nc,
-                                                ghosts ? s.gcell_count : nullptr);
+  // cell counts start cleared (allocation, then k_rank_move); the host-halo
+  // ghost counts are cleared here
+  if (ghosts)
+    k_build_begin<<<nblk(nc, 256), 256, 0, st>>>(s.flags, s.cell_count, nc, s.gcell_count);
   if (n > 0) {
     k_wrap_bin<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell, s.slot, s.cell_count, s.flags);
     launch_scan(s.cell_count, s.cell_start, s.tile_sum, nc, s.flags, st);
     k_scatter<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.slot, s.cell_start, s.perm, n, s.flags);
-    k_rank<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.cell_start, s.perm, s.tag, s.order, n, s.flags);
-    k_permute<<<nblk(n, 256), 256, 0, st>>>(s.order, n, s.pos, s.pos_alt, s.pos_ref, s.vel,
-                                            s.vel_alt, s.acc, s.acc_alt, s.tag, s.tag_alt, s.id,
-                                            s.id_alt, s.flags);
+    // accelerations are carried along only when no force pass follows
+    k_rank_move<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.cell_start, s.perm, n, s.pos, s.pos_alt,
+                                              s.pos_ref, s.vel, s.vel_alt,
+                                              keep_acc ? s.acc : nullptr, s.acc_alt, s.tag,
+                                              s.tag_alt, s.id, s.id_alt, s.cell_count, nc,
+                                              s.flags);
     // The host cannot know whether the device ran the build, so the new
     // order is copied back instead of swapping buffer pointers.
     k_copy_back<<<nblk(n, 256), 256, 0, st>>>(n, s.pos_alt, s.pos, s.vel_alt, s.vel, s.acc_alt,
-                                              s.acc, s.tag_alt, s.tag, s.id_alt, s.id,
+                                              keep_acc ? s.acc : nullptr, s.tag_alt, s.tag, s.id_alt, s.id,
                                               (s.write_q || b.coarse) ? s.q : nullptr, b, s.flags);
   }
   if (ghosts) {
@@ -1266,8 +1267,8 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo) 
                                           ghosts ? 1 : 0, s.wstat);
     nw = nblk(n, 256);
   }
-  if (nw > 0) k_build_stats<<<std::min(nblk(nw, 256), 64), 256, 0, st>>>(s.flags, s.wstat, nw);
-  k_build_done<<<1, 64, 0, st>>>(s.flags, s.cap);
+  k_build_stats<<<std::max(1, std::min(nblk(nw, 256), 64)), 256, 0, st>>>(s.flags, s.wstat, nw,
+                                                                           s.cap);
 }
 
 TaskGeom task_geom(int n, const BoxDev &b) {
