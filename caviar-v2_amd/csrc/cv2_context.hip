@@ -46,6 +46,7 @@ struct cv2_ctx {
   double *h_red = nullptr;
   long long *h_fx = nullptr;
   int partial_blocks = 0;
+  bool fuse = true;  // integrator fused into the FP32 task kernel where it runs
   // HIP-event timing of kernel groups (cv2_set_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -128,7 +129,7 @@ void free_state(DevState &s) {
   void *ptrs[] = {s.pos, s.pos_alt, s.pos_ref, s.q, s.vel, s.vel_alt, s.acc, s.acc_alt, s.tag,
                   s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.order, s.box_dev, s.wstat, s.cell_count, s.cell_start,
                   s.gcell_count, s.gcell_start, s.gcell_items, s.tile_sum, s.nbr, s.nnbr, s.flags,
-                  s.partials, s.fx_acc, s.reduce_out, s.tpiece, s.tstage};
+                  s.partials, s.fx_acc, s.reduce_out, s.tpiece, s.tstage, s.q_alt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   s = DevState{};
@@ -320,6 +321,7 @@ int alloc_positions(cv2_ctx *ctx, int64_t ntot) {
   HIPCHK(dalloc(&s.pos, cap));
   HIPCHK(dalloc(&s.pos_ref, cap));
   HIPCHK(dalloc(&s.q, cap));
+  HIPCHK(dalloc(&s.q_alt, cap));
   HIPCHK(dalloc(&s.id, cap));
   HIPCHK(dalloc(&s.cell, cap));
   HIPCHK(dalloc(&s.slot, cap));
@@ -439,6 +441,8 @@ int cv2_create(cv2_ctx **out, int device) {
   ctx->s.stage_limit = kStage;
   if (const char *lim = std::getenv("CV2_STAGE_LIMIT"))
     ctx->s.stage_limit = std::max(0, std::min(kStage, std::atoi(lim)));
+  // CV2_FUSE=0 keeps the integrator in its own kernel (tests compare both)
+  if (const char *fz = std::getenv("CV2_FUSE")) ctx->fuse = std::atoi(fz) != 0;
   if (e != hipSuccess) {
     fprintf(stderr, "cv2_create: %s\n", hipGetErrorString(e));
     cv2_destroy(ctx);
@@ -679,25 +683,43 @@ int cv2_compute_forces(cv2_ctx *ctx) {
 // Shared step loop.  On entry the forces of the current positions are
 // known; each iteration closes the previous step and opens the next one in
 // one integrator pass, runs the (device-conditional) list build, and
-// computes forces (energy and virial on the last step only).
+// computes forces (energy and virial on the last step only).  Where the
+// FP32 task kernel runs, the integrator pass of the next iteration is fused
+// into its epilogue (launch_forces_fused), so that iteration starts with its
+// build.
 int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
   u.writeq = ctx->precision != CV2_PREC_FP64;
   const uint64_t s0 = (uint64_t)ctx->steps;
+  bool opened = false;  // this iteration's integrator pass already ran (fused)
   for (int64_t k = 0; k < nsteps; ++k) {
-    u.finish = k > 0;
-    u.start = true;
-    u.step = s0 + (uint64_t)k - 1;
+    if (!opened) {
+      u.finish = k > 0;
+      u.start = true;
+      u.step = s0 + (uint64_t)k - 1;
+      const int t = span_begin(ctx);
+      launch_update(ctx->s, ctx->b, u, ctx->stream);
+      span_end(ctx, t, 2);
+    }
     int t = span_begin(ctx);
-    launch_update(ctx->s, ctx->b, u, ctx->stream);
-    span_end(ctx, t, 2);
-    t = span_begin(ctx);
     {
       const int rc = queue_build(ctx, false, false);
       if (rc) return rc;
     }
     span_end(ctx, t, 1);
     t = span_begin(ctx);
-    launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
+    opened = false;
+    if (ctx->fuse && k + 1 < nsteps && ctx->precision != CV2_PREC_FP64) {
+      UpdateArgs f = u;
+      f.finish = true;
+      f.start = true;
+      f.step = s0 + (uint64_t)k;  // what iteration k + 1 would close and open
+      if (launch_forces_fused(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, f, ctx->stream)) {
+        std::swap(ctx->s.q, ctx->s.q_alt);
+        opened = true;
+      }
+    }
+    if (!opened)
+      launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
     span_end(ctx, t, 0);
     if (ctx->spans.size() > 4000) {
       HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -109,6 +109,129 @@ __device__ inline int4 pack_q(double x, double y, double z, int type, const doub
   return make_int4(q[0], q[1], q[2], type);
 }
 
+// ---- Philox4x32-10 ----
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+  constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+    const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+
+// Three standard normals for (seed, id, step).
+// Three standard normals for (seed, id, step): Box-Muller on the four
+// Philox words.  F32 evaluates the transform in FP32 with the hardware log,
+// square root and sine/cosine (the FP32+fixed-point form), otherwise in FP64;
+// the uniforms are the same 32-bit words either way.
+template <bool F32>
+__device__ inline void gauss3(uint64_t seed, int id, uint64_t step, double &g0, double &g1,
+                              double &g2) {
+  const U4 r = philox4x32_10(U4{(uint32_t)step, (uint32_t)(step >> 32), (uint32_t)id, 0x4c414e47u},
+                             (uint32_t)seed, (uint32_t)(seed >> 32));
+  if (F32) {
+    constexpr float k2m32f = 2.3283064365386963e-10f;
+    const float u0 = ((float)r.x + 0.5f) * k2m32f, u1 = (float)r.y * k2m32f;  // u0 in (0, 1]
+    const float u2 = ((float)r.z + 0.5f) * k2m32f, u3 = (float)r.w * k2m32f;
+    const float ra = __builtin_sqrtf(-2.0f * __logf(u0)), rb = __builtin_sqrtf(-2.0f * __logf(u2));
+    // v_sin_f32 / v_cos_f32 take the angle in turns: sin(2 pi u)
+    g0 = (double)(ra * __builtin_amdgcn_cosf(u1));
+    g1 = (double)(ra * __builtin_amdgcn_sinf(u1));
+    g2 = (double)(rb * __builtin_amdgcn_cosf(u3));
+  } else {
+    constexpr double k2m32 = 2.3283064365386963e-10;
+    const double u0 = ((double)r.x + 0.5) * k2m32, u1 = (double)r.y * k2m32;
+    const double u2 = ((double)r.z + 0.5) * k2m32, u3 = (double)r.w * k2m32;
+    const double ra = sqrt(-2.0 * log(u0)), rb = sqrt(-2.0 * log(u2));
+    double sa, ca, sb, cb;
+    sincospi(2.0 * u1, &sa, &ca);
+    sincospi(2.0 * u3, &sb, &cb);
+    g0 = ra * ca;
+    g1 = ra * sa;
+    g2 = rb * cb;
+    (void)sb;
+  }
+}
+
+struct StepParams {
+  double dt;
+  double la, lb, lc;  // Langevin a, b, c (md_simulator.cpp:398-405)
+  uint64_t seed;
+  uint64_t step;      // step index s of the finishing half; s+1 starts
+  double thresh_sq;
+  double lo[3], qscale[3];
+};
+
+
+// The integrator for one atom (k_update, and the epilogue of the fused FP32
+// force kernel), on loaded values: FINISH applies the velocity update that
+// closes step p.step (after its forces a), START the updates that open step
+// p.step + 1 (velocities and positions).  F32NOISE draws the Langevin noise
+// with the FP32 transform (the FP32+fixed-point form).
+template <bool LANGEVIN, bool FINISH, bool START, bool F32NOISE>
+__device__ inline void integrate_values(double4 &x, double v[3], int id, const double a[3],
+                                        const StepParams &p) {
+  // explicit FMAs: every kernel that inlines this rounds the same way
+  const double hdt = 0.5 * p.dt;
+  if (FINISH) {
+    if (LANGEVIN) {
+      double e[3];
+      gauss3<F32NOISE>(p.seed, id, p.step, e[0], e[1], e[2]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = fma(a[d], hdt, fma(e[d], p.lb, v[d] * p.la));  // md_simulator.cpp:860
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = fma(a[d], hdt, v[d]);
+    }
+  }
+  if (START) {
+    if (LANGEVIN) {
+      double e[3];
+      gauss3<F32NOISE>(p.seed, id, p.step + 1, e[0], e[1], e[2]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = v[d] + fma(e[d], p.lb, a[d] * hdt);  // :778
+      x.x = fma(v[0], p.lc, x.x);                                          // :835
+      x.y = fma(v[1], p.lc, x.y);
+      x.z = fma(v[2], p.lc, x.z);
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = fma(a[d], hdt, v[d]);
+      x.x = fma(v[0], p.dt, x.x);
+      x.y = fma(v[1], p.dt, x.y);
+      x.z = fma(v[2], p.dt, x.z);
+    }
+  }
+}
+
+// Stores of one integrated atom: velocities; with START the positions,
+// fixed-point coordinates (WRITEQ) and the rebuild check of
+// Neighborlist::update_is_needed (neighborlist.cpp:100-148) against the
+// build position r, raising F_REBUILD.
+template <bool START, bool WRITEQ>
+__device__ inline void integrate_store(int i, int n, const double4 &x, const double v[3],
+                                       const double4 &r, double4 *__restrict__ pos,
+                                       double *__restrict__ vel, int4 *__restrict__ q,
+                                       const StepParams &p, int *__restrict__ flags) {
+  if (START) {
+    pos[i] = x;
+    if (WRITEQ) q[i] = pack_q(x.x, x.y, x.z, (int)x.w, p.lo, p.qscale);
+    const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
+    if (dx * dx + dy * dy + dz * dz > p.thresh_sq) atomicOr(&flags[F_REBUILD], 1);
+  }
+  vel[i] = v[0];
+  vel[n + i] = v[1];
+  vel[2 * n + i] = v[2];
+}
+
 // Chooses the fixed-point units of b (bc and L set): span is the largest
 // separation along one axis that any kernel forms from q.  Sets qscale,
 // qinv, cunit and qiso.
@@ -174,6 +297,7 @@ struct DevState {
   int cap;      // ELL slots per atom
   double4 *pos, *pos_alt, *pos_ref;
   int4 *q;
+  int4 *q_alt;                  // second fixed-point buffer (fused FP32 step)
   double *vel, *vel_alt;  // 3 * n each (SoA: x block, y block, z block)
   double *acc, *acc_alt;
   int *tag, *tag_alt, *id, *id_alt;
@@ -238,6 +362,15 @@ struct UpdateArgs {
   uint64_t seed, step;
 };
 void launch_update(DevState &s, const BoxDev &b, const UpdateArgs &u, hipStream_t st);
+StepParams step_params(const BoxDev &b, const UpdateArgs &u);
+// FP32 forces on a wave-build box (slot lists) with the integrator fused
+// into the epilogue: forces of the current positions, then u's finish +
+// start for every owned atom (accelerations are not stored); the
+// fixed-point coordinates of the new positions go to s.q_alt, and the host
+// swaps s.q and s.q_alt after the launch.  Returns false (nothing launched)
+// where the fused form does not apply.
+bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
+                         const UpdateArgs &u, hipStream_t st);
 void launch_check_disp(DevState &s, const BoxDev &b, hipStream_t st);
 void launch_thermo(DevState &s, const LJDev *lj_dev, hipStream_t st);
 void launch_set_flag(int *flags, int which, int value, hipStream_t st);

@@ -29,55 +29,8 @@ constexpr int kBlock = 256;
 
 inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
-// ---- Philox4x32-10 ----
-struct U4 {
-  uint32_t x, y, z, w;
-};
-
-__device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
-  constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-  constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-    const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
-    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
-    k0 += W0;
-    k1 += W1;
-  }
-  return c;
-}
-
-// Three standard normals for (seed, id, step).
-__device__ inline void gauss3(uint64_t seed, int id, uint64_t step, double &g0, double &g1,
-                              double &g2) {
-  const U4 r = philox4x32_10(U4{(uint32_t)step, (uint32_t)(step >> 32), (uint32_t)id, 0x4c414e47u},
-                             (uint32_t)seed, (uint32_t)(seed >> 32));
-  constexpr double k2m32 = 2.3283064365386963e-10;
-  const double u0 = ((double)r.x + 0.5) * k2m32, u1 = (double)r.y * k2m32;
-  const double u2 = ((double)r.z + 0.5) * k2m32, u3 = (double)r.w * k2m32;
-  const double ra = sqrt(-2.0 * log(u0)), rb = sqrt(-2.0 * log(u2));
-  double sa, ca, sb, cb;
-  sincospi(2.0 * u1, &sa, &ca);
-  sincospi(2.0 * u3, &sb, &cb);
-  g0 = ra * ca;
-  g1 = ra * sa;
-  g2 = rb * cb;
-  (void)sb;
-}
-
-struct StepParams {
-  double dt;
-  double la, lb, lc;  // Langevin a, b, c (md_simulator.cpp:398-405)
-  uint64_t seed;
-  uint64_t step;      // step index s of the finishing half; s+1 starts
-  double thresh_sq;
-  double lo[3], qscale[3];
-};
-
-// LANGEVIN selects the integrator; FINISH applies the velocity update that
-// closes step s, START the updates that open step s+1; WRITEQ refreshes the
-// fixed-point coordinates.
+// One pass of the integrator over the owned atoms (integrate_values,
+// integrate_store).
 template <bool LANGEVIN, bool FINISH, bool START, bool WRITEQ>
 __global__ void __launch_bounds__(kBlock)
     k_update(double4 *__restrict__ pos, const double4 *__restrict__ pos_ref,
@@ -85,46 +38,15 @@ __global__ void __launch_bounds__(kBlock)
              int4 *__restrict__ q, int n, StepParams p, int *__restrict__ flags) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  double v[3] = {vel[i], vel[n + i], vel[2 * n + i]};
   const double a[3] = {acc[i], acc[n + i], acc[2 * n + i]};
-  const double hdt = 0.5 * p.dt;
-  if (FINISH) {
-    if (LANGEVIN) {
-      double e[3];
-      gauss3(p.seed, id[i], p.step, e[0], e[1], e[2]);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) v[d] = v[d] * p.la + e[d] * p.lb + a[d] * 0.5 * p.dt;  // :860
-    } else {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) v[d] += a[d] * hdt;
-    }
-  }
+  double v[3] = {vel[i], vel[n + i], vel[2 * n + i]};
+  double4 x = make_double4(0.0, 0.0, 0.0, 0.0), r = x;
   if (START) {
-    double4 x = pos[i];
-    if (LANGEVIN) {
-      double e[3];
-      gauss3(p.seed, id[i], p.step + 1, e[0], e[1], e[2]);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) v[d] += a[d] * 0.5 * p.dt + e[d] * p.lb;  // :778
-      x.x += v[0] * p.lc;                                                   // :835
-      x.y += v[1] * p.lc;
-      x.z += v[2] * p.lc;
-    } else {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) v[d] += a[d] * hdt;
-      x.x += v[0] * p.dt;
-      x.y += v[1] * p.dt;
-      x.z += v[2] * p.dt;
-    }
-    pos[i] = x;
-    if (WRITEQ) q[i] = pack_q(x.x, x.y, x.z, (int)x.w, p.lo, p.qscale);
-    const double4 r = pos_ref[i];
-    const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
-    if (dx * dx + dy * dy + dz * dz > p.thresh_sq) atomicOr(&flags[F_REBUILD], 1);
+    x = pos[i];
+    r = pos_ref[i];
   }
-  vel[i] = v[0];
-  vel[n + i] = v[1];
-  vel[2 * n + i] = v[2];
+  integrate_values<LANGEVIN, FINISH, START, WRITEQ>(x, v, LANGEVIN ? id[i] : 0, a, p);
+  integrate_store<START, WRITEQ>(i, n, x, v, r, pos, vel, q, p, flags);
 }
 
 // Displacement check alone (Neighborlist::update_is_needed), owned atoms
@@ -196,22 +118,17 @@ __global__ void __launch_bounds__(kBlock)
                  int n, EnsDev e, double dt, double thresh_sq, int *__restrict__ flags) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const double hdt = 0.5 * dt;
   double v[3] = {vel[i], vel[n + i], vel[2 * n + i]};
   const double a[3] = {acc[i], acc[n + i], acc[2 * n + i]};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) v[d] += a[d] * hdt;
-  if (FINISH && START) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) v[d] += a[d] * hdt;
-  }
+  StepParams p{};
+  p.dt = dt;
+  double4 x = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (START) x = pos[i];
+  // the same arithmetic as the single-box integrator (velocity Verlet)
+  integrate_values<false, FINISH, START, true>(x, v, 0, a, p);
   if (START) {
     const int sy = e.sys[i];
     const BoxDev &b = e.box[sy];
-    double4 x = pos[i];
-    x.x += v[0] * dt;
-    x.y += v[1] * dt;
-    x.z += v[2] * dt;
     pos[i] = x;
     q[i] = pack_q(x.x, x.y, x.z, (int)x.w, b.lo, b.qscale);
     const double4 r = pos_ref[i];
@@ -273,8 +190,7 @@ void update_q(DevState &s, const StepParams &p, bool writeq, hipStream_t st) {
 
 }  // namespace
 
-void launch_update(DevState &s, const BoxDev &b, const UpdateArgs &u, hipStream_t st) {
-  if (s.n == 0 || (!u.finish && !u.start)) return;
+StepParams step_params(const BoxDev &b, const UpdateArgs &u) {
   StepParams p{};
   p.dt = u.dt;
   p.la = u.la;
@@ -287,6 +203,12 @@ void launch_update(DevState &s, const BoxDev &b, const UpdateArgs &u, hipStream_
     p.lo[d] = b.lo[d];
     p.qscale[d] = b.qscale[d];
   }
+  return p;
+}
+
+void launch_update(DevState &s, const BoxDev &b, const UpdateArgs &u, hipStream_t st) {
+  if (s.n == 0 || (!u.finish && !u.start)) return;
+  const StepParams p = step_params(b, u);
 #define CV2_UPD(L, F, S)                             \
   if (u.langevin == L && u.finish == F && u.start == S) { \
     update_q<L, F, S>(s, p, u.writeq, st);           \

@@ -426,16 +426,30 @@ __device__ inline void task_atom(const int4 *__restrict__ q, const uint32_t *__r
   }
 }
 
-template <bool MULTI, bool ENERGY, bool ISO>
+// Integrator arrays of the fused form (FUSE): the epilogue applies finish +
+// start of step p.step to each home atom instead of storing its acceleration.
+struct FuseArgs {
+  double4 *pos;
+  const double4 *pos_ref;
+  double *vel;
+  const int *id;
+  int4 *q_out;
+  StepParams p;
+  int langevin;
+};
+
+template <bool MULTI, bool ENERGY, bool ISO, bool FUSE>
 __global__ void __launch_bounds__(kTaskBlock)
     k_lj_f32_task(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
                   const int *__restrict__ nnbr, int stride, int n, const int *__restrict__ cstart,
                   TaskGeom tg, const int4 *__restrict__ tpiece, const int *__restrict__ tstage,
                   int stage_limit, const LJDev *__restrict__ lj, BoxDev b,
                   double *__restrict__ acc, long long *__restrict__ partials,
-                  int *__restrict__ flags, int nblk_real) {
+                  int *__restrict__ flags, int nblk_real, FuseArgs fz) {
   static_assert(!(MULTI && ISO), "species tables are in length units");
-  __shared__ int sx[kStage], sy[kStage], sz[kStage];
+  static_assert(!(FUSE && ENERGY), "the energy pass is not fused");
+  __shared__ __attribute__((aligned(16))) int sx[kStage];
+  __shared__ int sy[kStage], sz[kStage];
   __shared__ unsigned char stype[MULTI ? kStage : 1];
   __shared__ int sj[ENERGY ? kStage : 1];
   __shared__ int4 pc[kTaskPieces];
@@ -450,6 +464,21 @@ __global__ void __launch_bounds__(kTaskBlock)
   const int t = task % tg.tpr, row = task / tg.tpr;
   const int h0 = cstart[row * tg.nx + task_x0(tg, t)];
   const int h1 = cstart[row * tg.nx + task_x0(tg, t + 1)];
+  // FUSE, one atom per thread (at most kTaskBlock home atoms): the
+  // integrator's loads for atom h0 + thread (atom order) are issued first,
+  // so that staging and the force loop hide their latency
+  double4 fx_pos = make_double4(0.0, 0.0, 0.0, 0.0), fx_ref = fx_pos;
+  double fx_vel[3] = {0.0, 0.0, 0.0};
+  int fx_id = 0;
+  if (FUSE && h1 - h0 <= kTaskBlock && (int)threadIdx.x < h1 - h0) {
+    const int ia = h0 + (int)threadIdx.x;
+    fx_pos = fz.pos[ia];
+    fx_ref = fz.pos_ref[ia];
+    fx_vel[0] = fz.vel[ia];
+    fx_vel[1] = fz.vel[n + ia];
+    fx_vel[2] = fz.vel[2 * n + ia];
+    if (fz.langevin) fx_id = fz.id[ia];
+  }
   int4 sent = make_int4(0, 0, 0, 0);
   if (h0 < h1) {
     sent = q[h0];
@@ -507,6 +536,20 @@ __global__ void __launch_bounds__(kTaskBlock)
   const double fc_rc = sqrt(c0.rc2);  // cutoff in the units of the separations
   const int nt = MULTI ? lj->nt : 1;
   long long vir = 0, pe = 0;
+  // FUSE: accelerations of the (at most kTaskBlock) home atoms, in the
+  // staging area once every neighbour has been read
+  double *at_sh = reinterpret_cast<double *>(sx);
+  static_assert(3 * kTaskBlock * sizeof(double) <= sizeof(int) * kStage, "staging area too small");
+  double my_a[3] = {0.0, 0.0, 0.0};
+  int my_t = -1;
+  auto fuse_step = [&](int ia, double4 x, double v[3], const double4 &r, int idv, const double *aa) {
+    if (fz.langevin)
+      integrate_values<true, true, true, true>(x, v, idv, aa, fz.p);
+    else
+      integrate_values<false, true, true, true>(x, v, idv, aa, fz.p);
+    integrate_store<true, true>(ia, n, x, v, r, fz.pos, fz.vel, fz.q_out, fz.p, flags);
+  };
+
   for (int t = (int)threadIdx.x; t < nh; t += kTaskBlock) {
     const int i = h0 + (sorted ? (int)lorder[t] : t);
     const int4 qi = q[i];
@@ -524,12 +567,40 @@ __global__ void __launch_bounds__(kTaskBlock)
     const int terms = (ENERGY || MULTI) ? a.terms : cp;
     const bool ovf = (ENERGY || MULTI) ? a.ovf : !((double)fcmax * fc_rc < 0.9 * 2251799813685248.0);
     const double mi = lj->mass_inv[ti] * (1.0 / kForceScale);
-    acc[i] = fx_value(a.sx, terms) * mi;
-    acc[n + i] = fx_value(a.sy, terms) * mi;
-    acc[2 * n + i] = fx_value(a.sz, terms) * mi;
+    const double ai[3] = {fx_value(a.sx, terms) * mi, fx_value(a.sy, terms) * mi,
+                          fx_value(a.sz, terms) * mi};
+    if (FUSE && sorted) {
+      // one atom per thread: handed back in atom order through LDS (below)
+      my_a[0] = ai[0];
+      my_a[1] = ai[1];
+      my_a[2] = ai[2];
+      my_t = i - h0;
+    } else if (FUSE) {
+      double v[3] = {fz.vel[i], fz.vel[n + i], fz.vel[2 * n + i]};
+      fuse_step(i, fz.pos[i], v, fz.pos_ref[i], fz.langevin ? fz.id[i] : 0, ai);
+    } else {
+      acc[i] = ai[0];
+      acc[n + i] = ai[1];
+      acc[2 * n + i] = ai[2];
+    }
     if (ovf) atomicOr(&flags[F_FXOVF], 1);
     vir += a.vir;
     pe += a.pe;
+  }
+  if (FUSE && sorted) {
+    // the integrator in atom order: coalesced position / velocity traffic
+    __syncthreads();  // every neighbour read of the staging area is done
+    if (my_t >= 0) {
+      at_sh[3 * my_t] = my_a[0];
+      at_sh[3 * my_t + 1] = my_a[1];
+      at_sh[3 * my_t + 2] = my_a[2];
+    }
+    __syncthreads();
+    const int t = (int)threadIdx.x;
+    if (t < nh) {
+      const double ai[3] = {at_sh[3 * t], at_sh[3 * t + 1], at_sh[3 * t + 2]};
+      fuse_step(h0 + t, fx_pos, fx_vel, fx_ref, fx_id, ai);
+    }
   }
   if (ENERGY) {
     const long long v = block_sum_i64(vir, sh);
@@ -632,9 +703,9 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
     if (s.slots) {
       const int treal = s.tg.ntask, tgrid = ((treal + 7) / 8) * 8;
 #define CV2_LJT(I)                                                                             \
-  k_lj_f32_task<MULTI, ENERGY, I><<<tgrid, kTaskBlock, 0, st>>>(                               \
+  k_lj_f32_task<MULTI, ENERGY, I, false><<<tgrid, kTaskBlock, 0, st>>>(                        \
       s.q, s.nbr, s.nnbr, s.stride, s.n, s.cell_start, s.tg, s.tpiece, s.tstage, s.stage_limit, \
-      lj_dev, b, s.acc, part, s.flags, treal)
+      lj_dev, b, s.acc, part, s.flags, treal, FuseArgs{})
       if (!MULTI && b.qiso)
         CV2_LJT(!MULTI);
       else
@@ -682,6 +753,26 @@ void launch_forces_ens(DevState &s, const EnsDev &e, const LJDev *lj_dev, int bl
   CV2_LJE(128)
   CV2_LJE(256)
 #undef CV2_LJE
+}
+
+bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
+                         const UpdateArgs &u, hipStream_t st) {
+  if (!s.slots || s.n == 0 || s.ng > 0 || !s.q_alt || !u.finish || !u.start) return false;
+  FuseArgs fz{s.pos, s.pos_ref, s.vel, s.id, s.q_alt, step_params(b, u), u.langevin ? 1 : 0};
+  long long *part = reinterpret_cast<long long *>(s.partials);
+  const int treal = s.tg.ntask, tgrid = ((treal + 7) / 8) * 8;
+#define CV2_LJF(M, I)                                                                          \
+  k_lj_f32_task<M, false, I, true><<<tgrid, kTaskBlock, 0, st>>>(                              \
+      s.q, s.nbr, s.nnbr, s.stride, s.n, s.cell_start, s.tg, s.tpiece, s.tstage, s.stage_limit, \
+      lj_dev, b, s.acc, part, s.flags, treal, fz)
+  if (lj.nt > 1)
+    CV2_LJF(true, false);
+  else if (b.qiso)
+    CV2_LJF(false, true);
+  else
+    CV2_LJF(false, false);
+#undef CV2_LJF
+  return true;
 }
 
 void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,

@@ -176,6 +176,31 @@ def test_fp32_unstaged_tasks_identical(name, mid, monkeypatch):
         assert out[lim][1:] == out["3072"][1:]
 
 
+@pytest.mark.parametrize("langevin", [False, True])
+def test_fused_integrator_identical(langevin, monkeypatch):
+    """The FP32 task kernel with the integrator in its epilogue gives the
+    same bits as separate force and integrator kernels, across rebuilds."""
+    s = perturbed(lj_fluid(10, temperature=1.44, seed=12), 0.05, 12)
+    out = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("CV2_FUSE", fuse)
+        print(f"[fused] langevin={langevin} fuse={fuse}", flush=True)
+        with D.Context.from_sysdef(s, precision=D.CV2_PREC_FP32_FIXED) as c:
+            c.compute_forces()
+            if langevin:
+                c.run_langevin(120, 0.005, 1.0, 1.44, 31)
+            else:
+                c.run_vv(120, 0.005)
+            th = c.thermo()
+            out[fuse] = (c.download(), th, c.stats()["rebuilds"])
+    monkeypatch.delenv("CV2_FUSE")
+    (p1, v1, a1), th1, r1 = out["1"]
+    (p0, v0, a0), th0, r0 = out["0"]
+    assert r1 == r0 and r1 > 1
+    assert np.array_equal(p1, p0) and np.array_equal(v1, v0) and np.array_equal(a1, a0)
+    assert th1 == th0
+
+
 def test_pairs_after_dynamics_bit_exact():
     """Pair set at a later rebuild, as a function of the device's positions."""
     s = perturbed(lj_fluid(8, temperature=1.44, seed=9), 0.05, 9)
