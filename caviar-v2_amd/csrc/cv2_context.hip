@@ -188,20 +188,9 @@ int setup_box(cv2_ctx *ctx) {
   // (below ce + skin < 2 ce); the coarse filter forms them between a home
   // cell's atoms and those of the adjacent cells (below 2 w).
   set_qscale(b, fast ? 2.0 * std::max(wmax, ce) : 2.0 * ce);
-  // Coarse filter bounds.  Rounding to the fixed-point unit u moves a
-  // distance by less than sqrt(sum u_d^2); the float arithmetic on
-  // coordinates relative to a home atom (|x| < 2 w) adds at most
-  // e_f = 8 * 2^-24 * (2 w + ce) per axis.
   {
-    double u2 = 0.0;
-    for (int d = 0; d < 3; ++d) u2 += b.qinv[d] * b.qinv[d];
-    const double ef = 8.0 * 5.9604644775390625e-08 * (2.0 * wmax + ce) * std::sqrt(3.0);
-    const double e = 1.01 * std::sqrt(u2) + ef + 1e-6 * ce;
-    b.cthr_in = std::nextafter((float)((ce - e) * (ce - e) * (1.0 - 4e-6)), 0.0f);
-    b.cthr_out = std::nextafter((float)((ce + e) * (ce + e) * (1.0 + 4e-6)), 1e30f);
-    bool ok = fast && ce > 2.0 * e;
-    for (int d = 0; d < 3; ++d) ok = ok && 0.5 * b.L[d] > ce + 2.0 * e;
-    if (fast && !ok) return fail(ctx, CV2_ERR_BAD_ARG, "box too large for the coarse list filter");
+    const bool ok = !fast || coarse_filter_setup(b, ce, wmax);
+    if (!ok) return fail(ctx, CV2_ERR_BAD_ARG, "box too large for the coarse list filter");
     b.coarse = fast ? 1 : 0;
     b.ghost_ok = ctx->ghost_cut >= ce ? 1 : 0;
   }
@@ -392,6 +381,21 @@ bool build_lj_table(LJDev &L, int ntypes, const double *eps, const double *sigma
     }
   }
   return true;
+}
+
+bool coarse_filter_setup(BoxDev &b, double ce, double wmax) {
+  // Rounding to the fixed-point unit u moves a distance by less than
+  // sqrt(sum u_d^2); the float arithmetic on coordinates relative to a home
+  // atom (|x| < 2 w) adds at most e_f = 8 * 2^-24 * (2 w + ce) per axis.
+  double u2 = 0.0;
+  for (int d = 0; d < 3; ++d) u2 += b.qinv[d] * b.qinv[d];
+  const double ef = 8.0 * 5.9604644775390625e-08 * (2.0 * wmax + ce) * std::sqrt(3.0);
+  const double e = 1.01 * std::sqrt(u2) + ef + 1e-6 * ce;
+  b.cthr_in = std::nextafter((float)((ce - e) * (ce - e) * (1.0 - 4e-6)), 0.0f);
+  b.cthr_out = std::nextafter((float)((ce + e) * (ce + e) * (1.0 + 4e-6)), 1e30f);
+  bool ok = ce > 2.0 * e;
+  for (int d = 0; d < 3; ++d) ok = ok && 0.5 * b.L[d] > ce + 2.0 * e;
+  return ok;
 }
 
 void set_qscale(BoxDev &b, double span) {

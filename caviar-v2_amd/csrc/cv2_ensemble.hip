@@ -132,13 +132,16 @@ void free_all(cv2_ens *ens) {
   ens->forces_valid = false;
 }
 
-// Box constants of one periodic system: cells at least half the list
-// cutoff wide (the per-atom build scans +-2 cells).
+// Box constants of one periodic system.  With at least 5 cells of width >=
+// ce per axis the system takes the wave-per-cell build (coarse filter on
+// the fixed-point coordinates); otherwise cells at least half the list cutoff
+// wide for the per-atom build (+-2 cells).
 BoxDev system_box(const double lo[3], const double hi[3], double ce, double rc) {
   BoxDev b{};
   b.ce_sq = ce * ce;
   b.thresh_sq = 0.25 * (ce - rc) * (ce - rc);
   b.ghost_cut = ce;  // every image within the list cutoff exists
+  bool fast = true;
   for (int d = 0; d < 3; ++d) {
     b.lo[d] = lo[d];
     b.hi[d] = hi[d];
@@ -146,13 +149,29 @@ BoxDev system_box(const double lo[3], const double hi[3], double ce, double rc) 
     b.halfL[d] = 0.5 * b.L[d];
     b.bc[d] = 1;
     b.glo[d] = lo[d];
-    int nc = (int)std::floor(b.L[d] / (0.5 * ce * (1.0 + 1e-12)));
+    fast = fast && std::floor(b.L[d] / (ce * (1.0 + 1e-12))) >= 5.0;
+  }
+  double wmax = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    int nc = (int)std::floor(b.L[d] / ((fast ? 1.0 : 0.5) * ce * (1.0 + 1e-12)));
     nc = std::max(1, std::min(nc, 1024));
     b.nc[d] = nc;
     b.cinv[d] = nc / b.L[d];
+    wmax = std::max(wmax, b.L[d] / nc);
   }
-  set_qscale(b, 2.0 * ce);  // list pairs stay below ce + skin < 2 ce
-  b.coarse = 0;
+  // list pairs stay below ce + skin < 2 ce; the coarse filter forms
+  // separations up to two cell widths
+  set_qscale(b, fast ? 2.0 * std::max(wmax, ce) : 2.0 * ce);
+  b.coarse = fast && coarse_filter_setup(b, ce, wmax) ? 1 : 0;
+  if (fast && !b.coarse) {  // the filter cannot decide: per-atom build
+    for (int d = 0; d < 3; ++d) {
+      int nc = (int)std::floor(b.L[d] / (0.5 * ce * (1.0 + 1e-12)));
+      nc = std::max(1, std::min(nc, 1024));
+      b.nc[d] = nc;
+      b.cinv[d] = nc / b.L[d];
+    }
+    set_qscale(b, 2.0 * ce);
+  }
   b.ghost_ok = 1;
   return b;
 }
@@ -395,7 +414,9 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
   EHIP(ralloc(&st.cell_start, ncell + 1));
   EHIP(ralloc(&st.tile_sum, ncell / 1024 + 2));
   EHIP(ralloc(&st.flags, F_COUNT));
-  EHIP(ralloc(&st.wstat, (n + 63) / 64 + 8));  // per-block statistics, blocks of >= 64
+  // per-block statistics: per-atom build blocks (>= 64 atoms), then
+  // wave-per-cell build blocks (4 cells)
+  EHIP(ralloc(&st.wstat, (n + 63) / 64 + (ncell + 3) / 4 + 16));
   EHIP(ralloc(&ens->d_sys, n));
   EHIP(ralloc(&ens->d_cell_sys, ncell));
   EHIP(ralloc(&ens->d_cell_off, nsys));

@@ -237,6 +237,11 @@ __device__ inline void integrate_store(int i, int n, const double4 &x, const dou
 // qinv, cunit and qiso.
 void set_qscale(BoxDev &b, double span);
 
+// Coarse-filter thresholds of the wave-per-cell build (cthr_in, cthr_out)
+// for list cutoff ce and cell width at most wmax, from the fixed-point unit
+// (set_qscale first); false when the filter cannot decide (box too large).
+bool coarse_filter_setup(BoxDev &b, double ce, double wmax);
+
 // Force tasks of the FP32 path on wave-build boxes.  A task is a run of
 // consecutive cells [x0, x1) of one x-row of the cell grid (its atoms are
 // contiguous in the sorted order).  Its neighbourhood -- the x-cells

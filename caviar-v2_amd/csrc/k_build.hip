@@ -1125,11 +1125,56 @@ __global__ void k_ens_nlist(const double4 *__restrict__ pos, int n, EnsDev e,
   int cnt = 0, kept = 0;
   if (i < n) {
     const int sy = e.sys[i];
-    if (e.sflag[sy])
+    if (e.sflag[sy] && !e.box[sy].coarse)  // wave-build systems: k_ens_nlist_cells
       nlist_atom(pos, n, e.box[sy], cstart + e.cell_off[sy], nullptr, nullptr, nbr, nnbr, stride,
                  cap, 0, i, cnt, kept);
   }
   block_stat(bstat, cnt, kept);
+}
+
+// Wave-per-cell Verlet list of the flagged systems that have the coarse
+// filter (k_nlist_cells with each system's box, cell grid and fixed-point
+// unit; every image has a ghost, ghost cutoff = list cutoff).  One
+// wavefront per cell of the whole ensemble; cells of other systems return.
+__global__ void __launch_bounds__(256)
+    k_ens_nlist_cells(const double4 *__restrict__ pos, const int4 *__restrict__ q, EnsDev e,
+                      int ncell, const int *__restrict__ cstart, uint32_t *__restrict__ nbr,
+                      int *__restrict__ nnbr, int stride, int cap, const int *__restrict__ flags,
+                      int nblk_real, int2 *__restrict__ bstat) {
+  if (flags[F_REBUILD] == 0) return;
+  const int per = gridDim.x >> 3;
+  const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (blk >= nblk_real) return;
+  __shared__ __attribute__((aligned(16))) uint32_t lds[4][kPassCand + 4 + 64];
+  __shared__ int cnt_sh[4][kMaxHomeCell];
+  __shared__ int2 st_sh[4];
+  const int wv = (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int c = __builtin_amdgcn_readfirstlane(blk * 4 + wv);
+  int2 st = make_int2(0, 0);
+  if (c < ncell) {
+    const int sy = __builtin_amdgcn_readfirstlane(e.cell_sys[c]);
+    const BoxDev *bd = e.box + sy;
+    if (e.sflag[sy] && bd->coarse) {
+      CellWaveBox cb{};
+      for (int d = 0; d < 3; ++d) {
+        cb.nc[d] = bd->nc[d];
+        cb.cunit[d] = bd->cunit[d];
+      }
+      cb.cthr_in = bd->cthr_in;
+      cb.cthr_out = bd->cthr_out;
+      const int off = e.cell_off[sy];
+      st = cell_wave<true, false>(pos, q, cb, bd, cstart + off, nbr, nnbr, stride, cap, c - off, wv,
+                                  lane, lds, nullptr, cnt_sh, TaskGeom{}, nullptr, nullptr);
+    }
+  }
+  if (lane == 0) st_sh[wv] = st;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int2 t = st_sh[0];
+    for (int w = 1; w < 4; ++w) t = make_int2(max(t.x, st_sh[w].x), t.y + st_sh[w].y);
+    bstat[blk] = t;
+  }
 }
 
 // Close an ensemble build: longest list and overflow from the per-block
@@ -1192,7 +1237,10 @@ void launch_build_ens(DevState &s, const EnsDev &e, int ncell, int block, bool p
   const int nb = nblk(n, block);
   k_ens_nlist<<<nb, block, 0, st>>>(s.pos, n, e, s.cell_start, s.nbr, s.nnbr, s.stride, s.cap,
                                      s.flags, s.wstat);
-  k_ens_done<<<1, 1024, 0, st>>>(s.flags, s.wstat, nb, s.cap, e);
+  const int nbc = nblk(ncell, 4), gridc = (nbc + 7) / 8 * 8;
+  k_ens_nlist_cells<<<gridc, 256, 0, st>>>(s.pos, s.q, e, ncell, s.cell_start, s.nbr, s.nnbr,
+                                           s.stride, s.cap, s.flags, nbc, s.wstat + nb);
+  k_ens_done<<<1, 1024, 0, st>>>(s.flags, s.wstat, nb + nbc, s.cap, e);
 }
 
 void launch_set_flag(int *flags, int which, int value, hipStream_t st) {
