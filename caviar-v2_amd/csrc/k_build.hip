@@ -602,8 +602,15 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
   float px[kSlots], py[kSlots], pz[kSlots];
   float tin[GHOST_OK ? 1 : kSlots];  // per-slot inner threshold when images may lack ghosts
   // the value each candidate appends: its entry, or (SLOTS) its task slot,
-  // the entry then being re-read from cand on the rare exact path
-  uint32_t ent[kSlots];
+  // two 16-bit slots to a register (slot k in half k % 2), the entry then
+  // being re-read from cand on the rare exact path
+  constexpr int kEntRegs = SLOTS ? (kSlots + 1) / 2 : kSlots;
+  uint32_t ent[kEntRegs];
+#pragma unroll
+  for (int k = 0; k < kEntRegs; ++k) ent[k] = 0u;
+  auto ent_of = [&](int k) -> uint32_t {
+    return SLOTS ? ((k & 1) ? ent[k >> 1] >> 16 : ent[k >> 1] & 0xffffu) : ent[k];
+  };
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) {
     const int t = k * 64 + lane;
@@ -611,7 +618,6 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
     py[k] = 0.0f;
     pz[k] = 0.0f;
     if (!GHOST_OK) tin[k] = b.cthr_in;
-    ent[k] = 0u;
     if (t < ncand) {
       const uint32_t e = cand[t];
       const int3 c = load_q3(q, e & kIndexMask);
@@ -620,7 +626,10 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
       pz[k] = q_rel(c.z, o.z, b.cunit[2]);
       // images whose ghost may not exist always take the exact path
       if (!GHOST_OK && (e >> 26) != kNoImage) tin[k] = -1.0f;
-      ent[k] = SLOTS ? (uint32_t)cslot[t] : e;
+      if (SLOTS)
+        ent[k >> 1] |= (uint32_t)cslot[t] << ((k & 1) * 16);
+      else
+        ent[k] = e;
     }
   }
   uint16_t *buf16 = reinterpret_cast<uint16_t *>(buf);
@@ -678,12 +687,12 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
           // instructions only; the scalar unit is shared by four SIMDs.
           const unsigned long long m_in = __builtin_amdgcn_ballot_w64((__float_as_uint(r2) - 1u) < cin_m1);
           bm = fminf(bm, fabsf(r2 - bmid));
-          append(m_in, ent[k]);
+          append(m_in, ent_of(k));
         } else {
           const unsigned long long m_in = __builtin_amdgcn_ballot_w64(r2 < tin[GHOST_OK ? 0 : k]);
           const unsigned long long m_out = __builtin_amdgcn_ballot_w64(r2 < b.cthr_out);
           anyband |= m_out & ~m_in;  // the home atom itself is in m_in
-          append(k == kself ? m_in & ~selfbit : m_in, ent[k]);
+          append(k == kself ? m_in & ~selfbit : m_in, ent_of(k));
         }
       }
     }
@@ -704,7 +713,7 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
         const bool band = (r2 < b.cthr_out) & !(r2 < (GHOST_OK ? b.cthr_in : tin[GHOST_OK ? 0 : k]));
         bool take = false;
         if (band) {
-          uint32_t e = SLOTS ? cand[k * 64 + lane] : ent[k];
+          uint32_t e = SLOTS ? cand[k * 64 + lane] : ent_of(k);
           __asm__ volatile("" : "+v"(e));  // decode here, not hoisted
           // x, y, z only: a loaded but unused w would leave a load pending
           // that the next reuse of its register has to wait for
@@ -723,7 +732,7 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
           }
           take = list_pair(pix, piy, piz, gc[0], gc[1], gc[2], B.ce_sq) & exists;
         }
-        append(__builtin_amdgcn_ballot_w64(take), ent[k]);
+        append(__builtin_amdgcn_ballot_w64(take), ent_of(k));
       }
     }
     if (STAGED) {
@@ -776,18 +785,27 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
   const int4 *tp = SLOTS ? tpiece + (size_t)((cy + ny * cz) * tg.tpr + task_of_x(tg, cx)) * kTaskPieces
                          : nullptr;
   const int pad_slot = SLOTS ? tstage[(cy + ny * cz) * tg.tpr + task_of_x(tg, cx)] : 0;
-  // segment table: flattened position of each row range
-  int total = 0, self_base = 0, self_j0 = 0;
-  for (int sg = 0; sg < kSegs; ++sg) {
-    int j0, len;
-    uint32_t img;
-    row_range(cstart, b, cx, cy, cz, sg % kWaveRows, sg / kWaveRows, j0, len, img);
-    if (sg == kWaveRows / 2) {  // dy = dz = 0, direct x: contains the home cell
-      self_base = total;
-      self_j0 = j0;
-    }
-    total += len;
+  // segment table, one row range per lane (lanes < kSegs): first entry,
+  // length, image bits, (SLOTS) slot offset and flattened start.  The
+  // loads of all ranges are issued together as vector loads; the passes
+  // read the table back with readlane.
+  int sj0 = 0, slen = 0, sdelta = 0;
+  uint32_t simg = 0u;
+  if (lane < kSegs) {
+    const int r = lane < kWaveRows ? lane : lane - kWaveRows, part = lane < kWaveRows ? 0 : 1;
+    row_range(cstart, b, cx, cy, cz, r, part, sj0, slen, simg);
+    if (SLOTS) sdelta = seg_slot_delta(tp, nx, cx, r, part);
   }
+  int sat = slen;  // inclusive prefix sum over lanes (kSegs <= 32)
+#pragma unroll
+  for (int d = 1; d < 32; d <<= 1) {
+    const int v = __shfl_up(sat, d);
+    sat += lane >= d ? v : 0;
+  }
+  const int total = __builtin_amdgcn_readlane(sat, kSegs - 1);
+  sat -= slen;
+  const int self_base = __builtin_amdgcn_readlane(sat, kWaveRows / 2);  // dy = dz = 0, direct x
+  const int self_j0 = __builtin_amdgcn_readlane(sj0, kWaveRows / 2);
   const int3 o = load_q3(q, h0);  // coordinates are relative to the first home atom
   int wmax = 0, went = 0;
   const bool staged = total <= kPassCand;
@@ -804,12 +822,13 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
     }
     for (int c0 = 0; c0 < total; c0 += kPassCand) {
       // gather this pass's candidate entries into LDS
-      int at = 0;
       for (int sg = 0; sg < kSegs; ++sg) {
-        int j0, len;
-        uint32_t img;
-        row_range(cstart, b, cx, cy, cz, sg % kWaveRows, sg / kWaveRows, j0, len, img);
-        const int delta = SLOTS && len > 0 ? seg_slot_delta(tp, nx, cx, sg % kWaveRows, sg / kWaveRows) : 0;
+        const int len = __builtin_amdgcn_readlane(slen, sg);
+        const int at = __builtin_amdgcn_readlane(sat, sg);
+        if (len == 0 || at + len <= c0 || at >= c0 + kPassCand) continue;
+        const int j0 = __builtin_amdgcn_readlane(sj0, sg);
+        const uint32_t img = (uint32_t)__builtin_amdgcn_readlane((int)simg, sg);
+        const int delta = SLOTS ? __builtin_amdgcn_readlane(sdelta, sg) : 0;
         for (int t = lane; t < len; t += 64) {
           const int p = at + t - c0;
           if (p >= 0 && p < kPassCand) {
@@ -817,7 +836,6 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
             if (SLOTS) cslot[p] = (uint16_t)(j0 + t + delta);
           }
         }
-        at += len;
       }
       const int ncand = min(kPassCand, total - c0);
       if (staged)
@@ -856,7 +874,7 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
 }
 
 template <bool GHOST_OK, bool SLOTS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
     k_nlist_cells(const double4 *__restrict__ pos, const int4 *__restrict__ q, CellWaveBox b,
                   const BoxDev *__restrict__ bdev, const int *__restrict__ cstart,
                   uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
@@ -1136,7 +1154,7 @@ __global__ void k_ens_nlist(const double4 *__restrict__ pos, int n, EnsDev e,
 // filter (k_nlist_cells with each system's box, cell grid and fixed-point
 // unit; every image has a ghost, ghost cutoff = list cutoff).  One
 // wavefront per cell of the whole ensemble; cells of other systems return.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
     k_ens_nlist_cells(const double4 *__restrict__ pos, const int4 *__restrict__ q, EnsDev e,
                       int ncell, const int *__restrict__ cstart, uint32_t *__restrict__ nbr,
                       int *__restrict__ nnbr, int stride, int cap, const int *__restrict__ flags,
