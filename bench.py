@@ -39,13 +39,22 @@ def workload(ncell):
     return lj_fluid(ncell, rho=0.8442, temperature=1.0, seed=1003, integrator=1, gamma=1.0)
 
 
+FORCE_SAMPLE = 8   # HIP events around every 8th force launch of the timed steps
+PHASE_STEPS = 200  # instrumented steps (events around every kernel group) after the timed run
+
+
 def run_gpu(s, precision, steps, warmup, seed=7):
+    """Timed Langevin run; the force launches of the timed steps are sampled
+    with HIP events (every FORCE_SAMPLE-th, on the context's stream), and a
+    short run with events around every kernel group follows for the phase
+    breakdown (events cost a few microseconds each, so that run is not the
+    timed one)."""
     n = len(s["pos"])
     with D.Context.from_sysdef(s, precision=precision) as c:
         c.compute_forces()
         c.run_langevin(warmup, s["dt"], s["gamma"], s["kbt"], seed)
         c.synchronize()
-        c.set_timing(True)
+        c.set_timing(True, period=FORCE_SAMPLE)
         st0 = c.stats()
         t0 = time.perf_counter()
         c.run_langevin(steps, s["dt"], s["gamma"], s["kbt"], seed)
@@ -54,9 +63,14 @@ def run_gpu(s, precision, steps, warmup, seed=7):
         tm = c.timing()
         st = c.stats()
         th = c.thermo(n_r_df=0)
+        c.set_timing(True)
+        c.run_langevin(PHASE_STEPS, s["dt"], s["gamma"], s["kbt"], seed + 1)
+        ph = c.timing()
+        c.set_timing(False)
         pos = c.download()[0]
     el = t1 - t0
     return dict(value=n * steps / el, ms_per_step=1e3 * el / steps, timing=tm,
+                phases={k: ph[k] / PHASE_STEPS for k in ("force_ms", "build_ms", "integrate_ms")},
                 rebuilds=st["rebuilds"] - st0["rebuilds"], mean_list=st["list_entries"] / n,
                 max_list=st["max_neighbors"], capacity=st["capacity"],
                 temperature=2 * th["ke"] / (3 * n), pos=pos)
@@ -256,7 +270,7 @@ def main():
                                              "rebuilds_in_timed_steps": r64["rebuilds"]},
         "speedup_fp32fx_vs_fp64": None if r64 is None else r32["value"] / r64["value"],
         "force_error_vs_fp64": ferr,
-        "phase_ms_per_step": {k: tm[k] / a.steps for k in ("force_ms", "build_ms", "integrate_ms")},
+        "phase_ms_per_step": r32["phases"],
         "temperature_end": r32["temperature"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(),

@@ -47,8 +47,11 @@ struct cv2_ctx {
   long long *h_fx = nullptr;
   int partial_blocks = 0;
   bool fuse = true;  // integrator fused into the FP32 task kernel where it runs
-  // HIP-event timing of kernel groups (cv2_set_timing)
+  // HIP-event timing of kernel groups (cv2_set_timing): every group, or
+  // (period > 1) the force launch of every period-th force call only
   bool timing = false;
+  int timing_period = 1;
+  int64_t force_seq = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   struct Span {
@@ -71,9 +74,11 @@ hipEvent_t next_event(cv2_ctx *c) {
   return c->ev_pool[c->ev_used++];
 }
 
-// Open a timed span on the context stream (no-op when timing is off).
-int span_begin(cv2_ctx *c) {
+// Open a timed span of the given kind (0 force, 1 build, 2 integrate) on the
+// context stream (no-op when timing is off or the span is not sampled).
+int span_begin(cv2_ctx *c, int kind) {
   if (!c->timing) return -1;
+  if (c->timing_period > 1 && (kind != 0 || c->force_seq++ % c->timing_period != 0)) return -1;
   hipEvent_t a = next_event(c), b = next_event(c);
   if (!a || !b) return -1;
   (void)hipEventRecord(a, c->stream);
@@ -127,7 +132,7 @@ hipError_t dalloc(T **p, size_t count) {
 
 void free_state(DevState &s) {
   void *ptrs[] = {s.pos, s.pos_alt, s.pos_ref, s.q, s.vel, s.vel_alt, s.acc, s.acc_alt, s.tag,
-                  s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.order, s.box_dev, s.wstat, s.cell_count, s.cell_start,
+                  s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.order, s.box_dev, s.wstat, s.cell_count, s.cell_start, s.cell_local,
                   s.gcell_count, s.gcell_start, s.gcell_items, s.tile_sum, s.nbr, s.nnbr, s.flags,
                   s.partials, s.fx_acc, s.reduce_out, s.tpiece, s.tstage, s.q_alt};
   for (void *p : ptrs)
@@ -204,10 +209,16 @@ int setup_box(cv2_ctx *ctx) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
   if (ncell != ctx->s.ncell) {
     ctx->s.ncell = (int)ncell;
+    for (int **p : {&ctx->s.cell_count, &ctx->s.cell_start, &ctx->s.cell_local, &ctx->s.gcell_count,
+                    &ctx->s.gcell_start, &ctx->s.tile_sum}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
     HIPCHK(dalloc(&ctx->s.cell_count, ncell + 1));
     // builds leave the counts cleared for the next one (k_rank_move)
     HIPCHK(hipMemsetAsync(ctx->s.cell_count, 0, sizeof(int) * (ncell + 1), ctx->stream));
     HIPCHK(dalloc(&ctx->s.cell_start, ncell + 1));
+    HIPCHK(dalloc(&ctx->s.cell_local, ncell + 1));
     HIPCHK(dalloc(&ctx->s.gcell_count, ncell + 1));
     HIPCHK(dalloc(&ctx->s.gcell_start, ncell + 1));
     HIPCHK(dalloc(&ctx->s.tile_sum, ncell / 1024 + 2));
@@ -673,7 +684,7 @@ int cv2_compute_forces(cv2_ctx *ctx) {
     if (rc) return rc;
   }
   if (ctx->precision != CV2_PREC_FP64) launch_pack_q(ctx->s, ctx->b, ctx->stream);
-  const int t = span_begin(ctx);
+  const int t = span_begin(ctx, 0);
   launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, true, ctx->stream);
   span_end(ctx, t, 0);
   rc = check_flags(ctx);
@@ -700,17 +711,17 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
       u.finish = k > 0;
       u.start = true;
       u.step = s0 + (uint64_t)k - 1;
-      const int t = span_begin(ctx);
+      const int t = span_begin(ctx, 2);
       launch_update(ctx->s, ctx->b, u, ctx->stream);
       span_end(ctx, t, 2);
     }
-    int t = span_begin(ctx);
+    int t = span_begin(ctx, 1);
     {
       const int rc = queue_build(ctx, false, false);
       if (rc) return rc;
     }
     span_end(ctx, t, 1);
-    t = span_begin(ctx);
+    t = span_begin(ctx, 0);
     opened = false;
     if (ctx->fuse && k + 1 < nsteps && ctx->precision != CV2_PREC_FP64) {
       UpdateArgs f = u;
@@ -734,7 +745,7 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
     u.finish = true;
     u.start = false;
     u.step = s0 + (uint64_t)nsteps - 1;
-    const int t = span_begin(ctx);
+    const int t = span_begin(ctx, 2);
     launch_update(ctx->s, ctx->b, u, ctx->stream);
     span_end(ctx, t, 2);
   }
@@ -929,8 +940,11 @@ int cv2_set_timing(cv2_ctx *ctx, int enable) {
   if (!ctx) return CV2_ERR_BAD_ARG;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   spans_collect(ctx);
+  if (enable < 0) return fail(ctx, CV2_ERR_BAD_ARG, "timing period must be >= 0");
   ctx->timing = enable != 0;
   if (enable) {
+    ctx->timing_period = enable;
+    ctx->force_seq = 0;
     ctx->t_ms[0] = ctx->t_ms[1] = ctx->t_ms[2] = 0.0;
     ctx->force_calls = 0;
   }

@@ -39,6 +39,7 @@ enum Flag : int {
   F_ENTRIES = 6,   // list entries of the last build (low 32 bits)
   F_STAGE = 7,     // a force task's neighbourhood exceeds 16-bit slots
   F_STATBLK = 8,   // blocks of k_build_stats done (reset by the last one)
+  F_BINBLK = 9,    // blocks of k_wrap_bin done (reset by the last one)
   F_COUNT = 10
 };
 
@@ -311,6 +312,7 @@ struct DevState {
   int64_t wstat_cap;
   int *cell, *slot, *perm, *order;  // per owned atom during a build
   int *cell_count, *cell_start; // ncell (+1)
+  int *cell_local;               // tile-local cell starts during a build (ncell)
   int *tile_sum;                // per scan tile (ncell / 1024 + 1)
   int *gcell_count, *gcell_start, *gcell_items; // host-halo ghost cell list
   uint32_t *nbr;                // u32 entries, or u16 slots when `slots`

@@ -142,13 +142,89 @@ __global__ void __launch_bounds__(kScanTile)
     start[ncell] = off + tile_sum[blockIdx.x];
 }
 
-__global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
-                          const int *__restrict__ start, int *__restrict__ perm, int n,
-                          const int *__restrict__ flags) {
+// Cell starts of a build in one pass: each work-group scans a tile of
+// kScanTile cells into tile-local starts (local) and stores the tile total;
+// the last work-group to finish turns the tile totals into tile offsets (in
+// place) and stores the grand total in start[ncell].  A start is then
+// local[c] + tile_sum[c / kScanTile] (CellStarts); k_scatter writes them out.
+__global__ void __launch_bounds__(kScanTile)
+    k_scan_local(const int *__restrict__ count, int *__restrict__ local, int *tile_sum,
+                 int *__restrict__ start, int ncell, int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int sh[kScanTile];
+  __shared__ int last_sh;
+  const int c = blockIdx.x * kScanTile + threadIdx.x;
+  const int v = c < ncell ? count[c] : 0;
+  const int ex = block_exclusive_scan(v, sh);
+  if (c < ncell) local[c] = ex;
+  if (threadIdx.x == kScanTile - 1) tile_sum[blockIdx.x] = ex + v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last_sh = atomicAdd(&flags[F_BINBLK], 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last_sh) return;
+  __threadfence();
+  const int ntiles = gridDim.x;
+  int carry = 0;
+  for (int t0 = 0; t0 < ntiles; t0 += kScanTile) {
+    const int t = t0 + threadIdx.x;
+    const int tv = t < ntiles ? tile_sum[t] : 0;
+    __syncthreads();  // sh of the previous chunk has been read
+    const int tex = block_exclusive_scan(tv, sh);
+    if (t < ntiles) tile_sum[t] = carry + tex;
+    carry += sh[kScanTile - 1];
+  }
+  if (threadIdx.x == 0) {
+    start[ncell] = carry;
+    flags[F_BINBLK] = 0;
+  }
+}
+
+// Cell starts of a build from the tile-local starts and tile offsets of
+// k_scan_local (start[ncell], the total, already written by it).
+struct CellStarts {
+  const int *local, *tile_off, *start;
+  int ncell;
+  __device__ int operator()(int c) const {
+    return c < ncell ? local[c] + tile_off[c / kScanTile] : start[ncell];
+  }
+};
+
+__device__ void task_table_entry(int task, const CellStarts &cstart, const TaskGeom &g,
+                                 int4 *__restrict__ tpiece, int *__restrict__ tstage,
+                                 int *__restrict__ flags);
+
+// Scatter atom indices to their cell slots (work-groups [0, nbs)); the
+// next nfix work-groups store the cell starts, and the rest fill the
+// force-task piece table.
+__global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
+                          CellStarts cs, int *start, int *__restrict__ perm, int n,
+                          int *__restrict__ flags, int nbs, int nfix, TaskGeom g,
+                          int4 *__restrict__ tpiece, int *__restrict__ tstage) {
+  if (flags[F_REBUILD] == 0) return;
+  const int b = blockIdx.x;
+  if (b >= nbs + nfix) {
+    const int task = (b - nbs - nfix) * blockDim.x + threadIdx.x;
+    if (task < g.ntask) task_table_entry(task, cs, g, tpiece, tstage, flags);
+    return;
+  }
+  if (b >= nbs) {
+    const int c = (b - nbs) * blockDim.x + threadIdx.x;
+    if (c < cs.ncell) start[c] = cs(c);
+    return;
+  }
+  const int i = b * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  perm[start[cell[i]] + slot[i]] = i;
+  perm[cs(cell[i]) + slot[i]] = i;
+}
+
+// Fixed-point copy of positions for the FP32 pair kernel: (x - lo) * 2^32 / L
+// reduced modulo 2^32, so that int32 differences are minimum-image
+// separations on periodic axes.
+__device__ inline int4 qpack(const double4 &p, const BoxDev &b) {
+  return pack_q(p.x, p.y, p.z, (int)p.w, b.lo, b.qscale);
 }
 
 // Order atoms inside each cell by their position in the previous order so
@@ -157,9 +233,9 @@ __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ 
 // order).  One thread per atom: its rank in its cell is the number of cell
 // members with a smaller previous index (perm lists them; indices are
 // unique), and it moves its own data to that place in the *_out arrays
-// (acc: only when given), and its position to pos_ref (the build
-// positions).  The cell counts, consumed by the scan, are cleared for the
-// next build.
+// (acc: only when given), its position to pos_ref (the build positions)
+// and (q given) its fixed-point coordinates to q.  The cell counts,
+// consumed by the scan, are cleared for the next build.
 __global__ void k_rank_move(const int *__restrict__ cell, const int *__restrict__ start,
                             const int *__restrict__ perm, int n, const double4 *__restrict__ pos,
                             double4 *__restrict__ pos_out, double4 *__restrict__ pos_ref,
@@ -167,8 +243,8 @@ __global__ void k_rank_move(const int *__restrict__ cell, const int *__restrict_
                             const double *__restrict__ acc, double *__restrict__ acc_out,
                             const int *__restrict__ tag, int *__restrict__ tag_out,
                             const int *__restrict__ id, int *__restrict__ id_out,
-                            int *__restrict__ cell_count, int ncell,
-                            const int *__restrict__ flags) {
+                            int *__restrict__ cell_count, int ncell, int4 *__restrict__ q,
+                            BoxDev b, const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   for (int c = i; c < ncell; c += gridDim.x * blockDim.x) cell_count[c] = 0;
@@ -181,6 +257,7 @@ __global__ void k_rank_move(const int *__restrict__ cell, const int *__restrict_
   const double4 p = pos[i];
   pos_out[k] = p;
   pos_ref[k] = p;
+  if (q) q[k] = qpack(p, b);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     vel_out[d * n + k] = vel[d * n + i];
@@ -190,33 +267,37 @@ __global__ void k_rank_move(const int *__restrict__ cell, const int *__restrict_
   id_out[k] = id[i];
 }
 
-// Fixed-point copy of positions for the FP32 pair kernel: (x - lo) * 2^32 / L
-// reduced modulo 2^32, so that int32 differences are minimum-image
-// separations on periodic axes.
-__device__ inline int4 qpack(const double4 &p, const BoxDev &b) {
-  return pack_q(p.x, p.y, p.z, (int)p.w, b.lo, b.qscale);
-}
+// The arrays of the new order, copied back to the primary buffers.
+struct CopyBack {
+  const double4 *pos_alt;
+  double4 *pos;
+  const double *vel_alt;
+  double *vel;
+  const double *acc_alt;  // null: accelerations not carried
+  double *acc;
+  const int *tag_alt;
+  int *tag;
+  const int *id_alt;
+  int *id;
+  int n;
+};
 
-__global__ void k_copy_back(int n, const double4 *__restrict__ pos_alt, double4 *__restrict__ pos,
-                            const double *__restrict__ vel_alt, double *__restrict__ vel,
-                            const double *__restrict__ acc_alt, double *__restrict__ acc,
-                            const int *__restrict__ tag_alt, int *__restrict__ tag,
-                            const int *__restrict__ id_alt, int *__restrict__ id,
-                            int4 *__restrict__ q, BoxDev b,
-                            const int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const double4 p = pos_alt[k];
-  pos[k] = p;
-  if (q) q[k] = qpack(p, b);
+__device__ inline void copy_back_atom(int k, const CopyBack &c) {
+  const int n = c.n;
+  c.pos[k] = c.pos_alt[k];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    vel[d * n + k] = vel_alt[d * n + k];
-    if (acc) acc[d * n + k] = acc_alt[d * n + k];
+    c.vel[d * n + k] = c.vel_alt[d * n + k];
+    if (c.acc_alt) c.acc[d * n + k] = c.acc_alt[d * n + k];
   }
-  tag[k] = tag_alt[k];
-  id[k] = id_alt[k];
+  c.tag[k] = c.tag_alt[k];
+  c.id[k] = c.id_alt[k];
+}
+
+__global__ void k_copy_back(CopyBack c, const int *__restrict__ flags) {
+  if (flags[F_REBUILD] == 0) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < c.n) copy_back_atom(k, c);
 }
 
 // Host-halo ghosts keep host order; they get their own CSR cell list and
@@ -879,10 +960,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
                   const BoxDev *__restrict__ bdev, const int *__restrict__ cstart,
                   uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
                   const int *__restrict__ flags, int nblk_real, int2 *__restrict__ bstat,
-                  TaskGeom tg, const int4 *__restrict__ tpiece, const int *__restrict__ tstage) {
+                  TaskGeom tg, const int4 *__restrict__ tpiece, const int *__restrict__ tstage,
+                  int ncopy, CopyBack cpb) {
   if (flags[F_REBUILD] == 0) return;
+  const int lgrid = (int)gridDim.x - ncopy;  // list work-groups, a multiple of 8
+  if ((int)blockIdx.x >= lgrid) {
+    // the last ncopy work-groups copy the new order back to the primary
+    // buffers (this kernel reads only q and the new-order positions)
+    for (int k = ((int)blockIdx.x - lgrid) * blockDim.x + threadIdx.x; k < cpb.n;
+         k += ncopy * blockDim.x)
+      copy_back_atom(k, cpb);
+    return;
+  }
   // consecutive cells (which share most candidates) on the same XCD
-  const int per = gridDim.x >> 3;
+  const int per = lgrid >> 3;
   const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if (blk >= nblk_real) return;
   // candidates of a pass, then (staged) one home atom's entries + 3 pads;
@@ -953,11 +1044,9 @@ __global__ void __launch_bounds__(256)
 // of a build are known): first atom, slot base and x-cells of each piece,
 // and the staged total S; slot S is the task's sentinel, so S must fit 16
 // bits.
-__global__ void k_task_table(const int *__restrict__ cstart, TaskGeom g, int4 *__restrict__ tpiece,
-                             int *__restrict__ tstage, int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
-  const int task = blockIdx.x * blockDim.x + threadIdx.x;
-  if (task >= g.ntask) return;
+__device__ void task_table_entry(int task, const CellStarts &cstart, const TaskGeom &g,
+                                 int4 *__restrict__ tpiece, int *__restrict__ tstage,
+                                 int *__restrict__ flags) {
   const int t = task % g.tpr, row = task / g.tpr;
   const int cy = row % g.ny, cz = row / g.ny;
   const int x0 = task_x0(g, t), x1 = task_x0(g, t + 1);
@@ -969,8 +1058,8 @@ __global__ void k_task_table(const int *__restrict__ cstart, TaskGeom g, int4 *_
     for (int p = 0; p < 2; ++p) {
       int a0, a1;
       task_piece_cells(g.nx, x0, x1, p, a0, a1);
-      const int j0 = cstart[rb + a0];
-      const int len = a1 > a0 ? cstart[rb + a1] - j0 : 0;
+      const int j0 = cstart(rb + a0);
+      const int len = a1 > a0 ? cstart(rb + a1) - j0 : 0;
       tpiece[(size_t)task * kTaskPieces + 2 * r + p] = make_int4(j0, base, a0, a1);
       base += len;
     }
@@ -1275,25 +1364,38 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
   const int n = s.n;
   const int nc = s.ncell;
   const bool ghosts = host_halo && s.ng > 0;
-  // cell counts start cleared (allocation, then k_rank_move); the host-halo
-  // ghost counts are cleared here
+  // Every kernel of the chain returns at once unless a rebuild is pending,
+  // so the chain is kept short: one scan kernel, the cell starts and the
+  // task table written by k_scatter, the fixed-point packing in k_rank_move
+  // and (without host ghosts) the copy back in the list kernel.
+  // Cell counts start cleared (allocation, then k_rank_move); the
+  // host-halo ghost counts are cleared here.
   if (ghosts)
     k_build_begin<<<nblk(nc, 256), 256, 0, st>>>(s.flags, s.cell_count, nc, s.gcell_count);
+  const bool coarse = n > 0 && b.coarse;
+  const bool copy_in_list = coarse && !ghosts;
+  const CopyBack cpb{s.pos_alt,  s.pos,     s.vel_alt, s.vel, keep_acc ? s.acc_alt : nullptr,
+                     keep_acc ? s.acc : nullptr, s.tag_alt, s.tag, s.id_alt,  s.id,  n};
   if (n > 0) {
     k_wrap_bin<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell, s.slot, s.cell_count, s.flags);
-    launch_scan(s.cell_count, s.cell_start, s.tile_sum, nc, s.flags, st);
-    k_scatter<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.slot, s.cell_start, s.perm, n, s.flags);
+    k_scan_local<<<nblk(nc, kScanTile), kScanTile, 0, st>>>(s.cell_count, s.cell_local, s.tile_sum,
+                                                           s.cell_start, nc, s.flags);
+    const CellStarts cs{s.cell_local, s.tile_sum, s.cell_start, nc};
+    const int nbs = nblk(n, 256), nfix = nblk(nc, 256);
+    const bool tasks = coarse && s.slots;
+    TaskGeom tg = s.tg;
+    if (!tasks) tg.ntask = 0;
+    k_scatter<<<nbs + nfix + nblk(tg.ntask, 256), 256, 0, st>>>(
+        s.cell, s.slot, cs, s.cell_start, s.perm, n, s.flags, nbs, nfix, tg, s.tpiece, s.tstage);
     // accelerations are carried along only when no force pass follows
     k_rank_move<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.cell_start, s.perm, n, s.pos, s.pos_alt,
                                               s.pos_ref, s.vel, s.vel_alt,
                                               keep_acc ? s.acc : nullptr, s.acc_alt, s.tag,
                                               s.tag_alt, s.id, s.id_alt, s.cell_count, nc,
-                                              s.flags);
+                                              (s.write_q || b.coarse) ? s.q : nullptr, b, s.flags);
     // The host cannot know whether the device ran the build, so the new
     // order is copied back instead of swapping buffer pointers.
-    k_copy_back<<<nblk(n, 256), 256, 0, st>>>(n, s.pos_alt, s.pos, s.vel_alt, s.vel, s.acc_alt,
-                                              keep_acc ? s.acc : nullptr, s.tag_alt, s.tag, s.id_alt, s.id,
-                                              (s.write_q || b.coarse) ? s.q : nullptr, b, s.flags);
+    if (!copy_in_list) k_copy_back<<<nblk(n, 256), 256, 0, st>>>(cpb, s.flags);
   }
   if (ghosts) {
     k_ghost_bin<<<nblk(s.ng, 256), 256, 0, st>>>(s.pos, s.pos_ref, n, s.ng, b, s.cell + n,
@@ -1303,8 +1405,8 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
                                                       s.gcell_items, s.ng, s.flags);
     k_ghost_sort_cells<<<nblk(nc, 256), 256, 0, st>>>(s.gcell_items, s.gcell_start, nc, s.flags);
   }
-  int nw = 0;  // per-wavefront statistics written by the list kernel
-  if (n > 0 && b.coarse) {
+  int nw = 0;  // per-work-group statistics written by the list kernel
+  if (coarse) {
     CellWaveBox cb{};
     for (int d = 0; d < 3; ++d) {
       cb.nc[d] = b.nc[d];
@@ -1313,14 +1415,15 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
     cb.cthr_in = b.cthr_in;
     cb.cthr_out = b.cthr_out;
     const int real = nblk(nc, 4), grid = (real + 7) / 8 * 8;
-    if (s.slots)
-      k_task_table<<<nblk(s.tg.ntask, 256), 256, 0, st>>>(s.cell_start, s.tg, s.tpiece, s.tstage,
-                                                          s.flags);
+    const int ncopy = copy_in_list ? (nblk(n, 256) + 7) / 8 * 8 : 0;
+    // the list kernel reads the new-order positions where they were moved
+    const double4 *pos_new = copy_in_list ? s.pos_alt : s.pos;
 #define CV2_NLC(G, S)                                                                            \
   if (b.ghost_ok == G && s.slots == S)                                                           \
-    k_nlist_cells<G, S><<<grid, 256, 0, st>>>(s.pos, s.q, cb, s.box_dev, s.cell_start, s.nbr,    \
-                                              s.nnbr, s.stride, s.cap, s.flags, real, s.wstat,   \
-                                              s.tg, s.tpiece, s.tstage);
+    k_nlist_cells<G, S><<<ncopy + grid, 256, 0, st>>>(pos_new, s.q, cb, s.box_dev, s.cell_start, \
+                                                      s.nbr, s.nnbr, s.stride, s.cap, s.flags,   \
+                                                      real, s.wstat, s.tg, s.tpiece, s.tstage,   \
+                                                      ncopy, cpb);
     CV2_NLC(true, true)
     CV2_NLC(true, false)
     CV2_NLC(false, true)

@@ -218,8 +218,10 @@ class Context:
                     ncell=tuple(s.ncell), precision=s.precision,
                     build_kind=('cell_wave' if s.build_kind == 1 else 'per_atom'))
 
-    def set_timing(self, enable=True):
-        self._chk(self.L.cv2_set_timing(self.h, int(bool(enable))))
+    def set_timing(self, enable=True, period=1):
+        """HIP-event timing: every kernel group (period 1), or only the force
+        launch of every period-th force call."""
+        self._chk(self.L.cv2_set_timing(self.h, int(period) if enable else 0))
 
     def timing(self):
         t = Timing()

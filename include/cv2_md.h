@@ -152,7 +152,10 @@ typedef struct {
   double integrate_ms;  /* integrator kernels */
 } cv2_timing;
 
-/* enable != 0 starts (and zeroes) timing; 0 stops it. */
+/* enable != 0 starts (and zeroes) timing; 0 stops it.  enable = 1 times
+ * every kernel group; enable = k > 1 times only the force launch of every
+ * k-th force call (build_ms and integrate_ms stay 0), so that the events
+ * barely disturb the run they measure. */
 int cv2_set_timing(cv2_ctx *ctx, int enable);
 int cv2_get_timing(cv2_ctx *ctx, cv2_timing *out);
 
