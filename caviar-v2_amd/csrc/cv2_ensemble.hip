@@ -149,7 +149,10 @@ BoxDev system_box(const double lo[3], const double hi[3], double ce, double rc) 
     b.halfL[d] = 0.5 * b.L[d];
     b.bc[d] = 1;
     b.glo[d] = lo[d];
-    fast = fast && std::floor(b.L[d] / (ce * (1.0 + 1e-12))) >= 5.0;
+    // 3 cells per axis keep the 27 stencil cells distinct; axes with 3 or 4
+    // have no shift in the fixed-point unit, and the wave build forms its
+    // relative coordinates there from unwrapped values (q_rel_wide)
+    fast = fast && std::floor(b.L[d] / (ce * (1.0 + 1e-12))) >= 3.0;
   }
   double wmax = 0.0;
   for (int d = 0; d < 3; ++d) {

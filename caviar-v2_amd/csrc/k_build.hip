@@ -606,6 +606,23 @@ __device__ inline float q_rel(int q, int o, float unit) {
   return (float)(int)((unsigned)q - (unsigned)o) * unit;
 }
 
+// Coordinates relative to the reference atom po from the FP64 positions, for
+// boxes with 3 or 4 cells on some axis: there a candidate can lie more than
+// half a box from the reference atom, beyond what an int32 difference of q
+// resolves, so the separation of the candidate's image v (position p - v L)
+// is formed in FP64 and rounded once.  Home atoms take the same path, so an
+// atom's own coordinates are identical as candidate and as home atom.
+__device__ inline float3 pos_rel(const double4 *__restrict__ pos, int j, uint32_t img,
+                                 const double4 &po, const BoxDev &B) {
+  const double *pp = reinterpret_cast<const double *>(pos + j);
+  const double x = pp[0], y = pp[1], z = pp[2];  // w is not needed
+  const int vx = (int)((img >> 26) & 3u) - 1, vy = (int)((img >> 28) & 3u) - 1,
+            vz = (int)(img >> 30) - 1;
+  return make_float3((float)((vx ? x - vx * B.L[0] : x) - po.x),
+                     (float)((vy ? y - vy * B.L[1] : y) - po.y),
+                     (float)((vz ? z - vz * B.L[2] : z) - po.z));
+}
+
 // x, y, z of q[j] only (w is not needed, and a loaded but unused w would
 // leave a load outstanding for the next reuse of its register to wait for)
 __device__ inline int3 load_q3(const int4 *q, int j) {
@@ -627,6 +644,7 @@ struct CellWaveBox {
   int nc[3];
   float cunit[3];
   float cthr_in, cthr_out;
+  int wide;  // some axis has 3 or 4 cells: relative coordinates from the FP64 positions
 };
 
 // Range of sorted atoms of stencil row r (dy = r % 3 - 1, dz = r / 3 - 1) of
@@ -676,7 +694,8 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
                                const CellWaveBox &b, const BoxDev *__restrict__ bdev,
                                const uint32_t *cand, const uint16_t *cslot, int ncand, int c0,
                                int self_base, int self_j0, int pad_slot, int h0, int h1,
-                               const int3 o, uint32_t *__restrict__ nbr, int *__restrict__ nnbr,
+                               const int3 o, const double4 &po, uint32_t *__restrict__ nbr,
+                               int *__restrict__ nnbr,
                                size_t stride, int cap, uint32_t *buf, int *cnt_io, float hxl,
                                float hyl, float hzl, int lane, bool first, int &kept_l, int &wmax,
                                int &went) {
@@ -705,6 +724,12 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
       px[k] = q_rel(c.x, o.x, b.cunit[0]);
       py[k] = q_rel(c.y, o.y, b.cunit[1]);
       pz[k] = q_rel(c.z, o.z, b.cunit[2]);
+      if (b.wide) {  // uniform
+        const float3 r = pos_rel(pos, e & kIndexMask, e, po, *bdev);
+        px[k] = r.x;
+        py[k] = r.y;
+        pz[k] = r.z;
+      }
       // images whose ghost may not exist always take the exact path
       if (!GHOST_OK && (e >> 26) != kNoImage) tin[k] = -1.0f;
       if (SLOTS)
@@ -888,6 +913,11 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
   const int self_base = __builtin_amdgcn_readlane(sat, kWaveRows / 2);  // dy = dz = 0, direct x
   const int self_j0 = __builtin_amdgcn_readlane(sj0, kWaveRows / 2);
   const int3 o = load_q3(q, h0);  // coordinates are relative to the first home atom
+  double4 po = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (b.wide) {
+    const double *pp = reinterpret_cast<const double *>(pos + h0);
+    po = make_double4(pp[0], pp[1], pp[2], 0.0);
+  }
   int wmax = 0, went = 0;
   const bool staged = total <= kPassCand;
   for (int h = h0; h < h1; h += kMaxHomeCell) {
@@ -900,6 +930,12 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
       hxl = q_rel(c.x, o.x, b.cunit[0]);
       hyl = q_rel(c.y, o.y, b.cunit[1]);
       hzl = q_rel(c.z, o.z, b.cunit[2]);
+      if (b.wide) {
+        const float3 r = pos_rel(pos, h + lane, kNoImage << 26, po, *bdev);
+        hxl = r.x;
+        hyl = r.y;
+        hzl = r.z;
+      }
     }
     for (int c0 = 0; c0 < total; c0 += kPassCand) {
       // gather this pass's candidate entries into LDS
@@ -921,14 +957,14 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
       const int ncand = min(kPassCand, total - c0);
       if (staged)
         cell_wave_pass<true, GHOST_OK, SLOTS>(pos, q, b, bdev, cand, cslot, ncand, c0, self_base,
-                                              self_j0, pad_slot, h, he, o, nbr, nnbr,
+                                              self_j0, pad_slot, h, he, o, po, nbr, nnbr,
                                               (size_t)stride, cap,
                                               SLOTS ? reinterpret_cast<uint32_t *>(cslot) : cand,
                                               cnt_sh[wv], hxl, hyl, hzl, lane, c0 == 0, kept_l,
                                               wmax, went);
       else
         cell_wave_pass<false, GHOST_OK, SLOTS>(pos, q, b, bdev, cand, cslot, ncand, c0, self_base,
-                                               self_j0, pad_slot, h, he, o, nbr, nnbr,
+                                               self_j0, pad_slot, h, he, o, po, nbr, nnbr,
                                                (size_t)stride, cap, nullptr, cnt_sh[wv], hxl,
                                                hyl, hzl, lane, c0 == 0, kept_l, wmax, went);
     }
@@ -1267,6 +1303,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
       for (int d = 0; d < 3; ++d) {
         cb.nc[d] = bd->nc[d];
         cb.cunit[d] = bd->cunit[d];
+        if (bd->nc[d] < 5) cb.wide |= 1 << d;
       }
       cb.cthr_in = bd->cthr_in;
       cb.cthr_out = bd->cthr_out;
