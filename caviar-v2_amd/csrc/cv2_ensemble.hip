@@ -39,6 +39,7 @@ struct cv2_ens {
   int ncell = 0;
   double thresh_sq = 0.0;
   bool iso = true;  // every system has one fixed-point unit on all axes
+  bool per_atom = false;  // some system takes the per-atom list build
   // device state and tables
   DevState s{};
   int *d_sys = nullptr, *d_cell_sys = nullptr, *d_cell_off = nullptr, *d_atom_off = nullptr;
@@ -149,10 +150,12 @@ BoxDev system_box(const double lo[3], const double hi[3], double ce, double rc) 
     b.halfL[d] = 0.5 * b.L[d];
     b.bc[d] = 1;
     b.glo[d] = lo[d];
-    // 3 cells per axis keep the 27 stencil cells distinct; axes with 3 or 4
-    // have no shift in the fixed-point unit, and the wave build forms its
-    // relative coordinates there from unwrapped values (q_rel_wide)
-    fast = fast && std::floor(b.L[d] / (ce * (1.0 + 1e-12))) >= 3.0;
+    // Two cells per axis suffice: with 2, the stencil meets a cell twice,
+    // under two images L apart, and as L > 2 ce at most one image of an atom
+    // lies within ce of a home atom.  Boxes with fewer than 5 cells on some
+    // axis form the wave build's relative coordinates from the FP64
+    // positions (pos_rel).
+    fast = fast && std::floor(b.L[d] / (ce * (1.0 + 1e-12))) >= 2.0;
   }
   double wmax = 0.0;
   for (int d = 0; d < 3; ++d) {
@@ -196,7 +199,8 @@ int prime(cv2_ens *ens) {
   const EnsDev E = ens->ens();
   launch_fill_int(ens->d_sflag, ens->nsys, 1, ens->stream);
   launch_set_flag(ens->s.flags, F_REBUILD, 1, ens->stream);
-  launch_build_ens(ens->s, E, ens->ncell, ens->tune.build_block, false, ens->stream);
+  launch_build_ens(ens->s, E, ens->ncell, ens->tune.build_block, false, ens->per_atom,
+                   ens->stream);
   launch_forces_ens(ens->s, E, ens->lj_dev, ens->tune.force_block, ens->iso, false, ens->eatom, ens->stream);
   ens->forces_valid = true;
   return check_flags(ens);
@@ -236,7 +240,8 @@ int ensure_graph(cv2_ens *ens, double dt) {
   const EnsDev E = ens->ens();
   EHIP(hipStreamBeginCapture(ens->stream, hipStreamCaptureModeThreadLocal));
   launch_update_ens(ens->s, E, true, true, dt, ens->thresh_sq, ens->stream);
-  launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->stream);
+  launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->per_atom,
+                   ens->stream);
   launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream);
   EHIP(hipStreamEndCapture(ens->stream, &ens->graph));
   EHIP(hipGraphInstantiate(&ens->gexec, ens->graph, nullptr, nullptr, 0));
@@ -259,7 +264,8 @@ int run(cv2_ens *ens, int64_t nsteps, double dt) {
   // step 1 opens eagerly; steps 2..n replay the captured step; the final
   // half kick closes the last step
   launch_update_ens(ens->s, E, false, true, dt, ens->thresh_sq, ens->stream);
-  launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->stream);
+  launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->per_atom,
+                   ens->stream);
   launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream);
   if (nsteps > 1) {
     if (ens->use_graph) {
@@ -269,7 +275,8 @@ int run(cv2_ens *ens, int64_t nsteps, double dt) {
     } else {
       for (int64_t k = 1; k < nsteps; ++k) {
         launch_update_ens(ens->s, E, true, true, dt, ens->thresh_sq, ens->stream);
-        launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->stream);
+        launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0,
+                         ens->per_atom, ens->stream);
         launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream);
       }
     }
@@ -354,7 +361,7 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
   std::vector<int> coff(nsys, 0);
   int64_t ncell = 0;
   double max_expect = 0.0;
-  bool iso = true;
+  bool iso = true, per_atom = false;
   for (int s = 0; s < nsys; ++s) {
     if (natoms[s] < 1) return efail(ens, CV2_ERR_BAD_ARG, "every system needs atoms");
     for (int d = 0; d < 3; ++d)
@@ -362,6 +369,7 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
         return efail(ens, CV2_ERR_BAD_ARG, "periodic box must be wider than twice the list cutoff");
     boxes[s] = system_box(lo + 3 * s, hi + 3 * s, ce, ens->nl_cutoff);
     iso = iso && boxes[s].qiso;
+    per_atom = per_atom || !boxes[s].coarse;
     aoff[s + 1] = aoff[s] + natoms[s];
     coff[s] = (int)ncell;
     ncell += (int64_t)boxes[s].nc[0] * boxes[s].nc[1] * boxes[s].nc[2];
@@ -375,6 +383,7 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
   ens->nsys = nsys;
   ens->n = n;
   ens->iso = iso;
+  ens->per_atom = per_atom;
   ens->natoms.assign(natoms, natoms + nsys);
   ens->atom_off = aoff;
   ens->lo.assign(lo, lo + 3 * nsys);

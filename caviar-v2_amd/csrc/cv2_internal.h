@@ -387,7 +387,7 @@ void launch_set_flag(int *flags, int which, int value, hipStream_t st);
 // fixed-point virial and energy when energy is set), velocity Verlet with
 // the per-system rebuild test, per-system (ke, pe, virial).
 void launch_build_ens(DevState &s, const EnsDev &e, int ncell, int block, bool policy_all,
-                      hipStream_t st);
+                      bool per_atom, hipStream_t st);
 void launch_forces_ens(DevState &s, const EnsDev &e, const LJDev *lj_dev, int block, bool iso, bool energy,
                        long long *eatom, hipStream_t st);
 void launch_update_ens(DevState &s, const EnsDev &e, bool finish, bool start, double dt,

@@ -1363,7 +1363,7 @@ void launch_fill_int(int *p, int n, int value, hipStream_t st) {
 }
 
 void launch_build_ens(DevState &s, const EnsDev &e, int ncell, int block, bool policy_all,
-                      hipStream_t st) {
+                      bool per_atom, hipStream_t st) {
   const int n = s.n;
   if (n == 0) return;
   if (policy_all) k_ens_policy<<<nblk(e.nsys, 256), 256, 0, st>>>(e, s.flags);
@@ -1378,9 +1378,12 @@ void launch_build_ens(DevState &s, const EnsDev &e, int ncell, int block, bool p
                                                s.flags);
   k_ens_copy_back<<<nblk(n, 256), 256, 0, st>>>(n, e, s.pos_alt, s.pos, s.vel_alt, s.vel,
                                                  s.acc_alt, s.acc, s.tag_alt, s.tag, s.q, s.flags);
-  const int nb = nblk(n, block);
-  k_ens_nlist<<<nb, block, 0, st>>>(s.pos, n, e, s.cell_start, s.nbr, s.nnbr, s.stride, s.cap,
-                                     s.flags, s.wstat);
+  // per-atom build only where some system has it (boxes the coarse filter
+  // cannot serve)
+  const int nb = per_atom ? nblk(n, block) : 0;
+  if (per_atom)
+    k_ens_nlist<<<nb, block, 0, st>>>(s.pos, n, e, s.cell_start, s.nbr, s.nnbr, s.stride, s.cap,
+                                       s.flags, s.wstat);
   const int nbc = nblk(ncell, 4), gridc = (nbc + 7) / 8 * 8;
   k_ens_nlist_cells<<<gridc, 256, 0, st>>>(s.pos, s.q, e, ncell, s.cell_start, s.nbr, s.nnbr,
                                            s.stride, s.cap, s.flags, nbc, s.wstat + nb);

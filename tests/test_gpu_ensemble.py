@@ -65,6 +65,44 @@ def test_ensemble_pairs_bit_exact_and_forces():
         assert st["graph_nodes"] > 0
 
 
+def lattice_system(L, rho, seed, temperature=1.5):
+    """Jittered simple-cubic fill of an (Lx, Ly, Lz) periodic box."""
+    rng = np.random.default_rng(seed)
+    L = np.asarray(L, float)
+    a = (1.0 / rho) ** (1.0 / 3.0)
+    k = np.maximum(1, np.round(L / a)).astype(int)
+    g = np.stack(np.meshgrid(*[np.arange(m) for m in k], indexing="ij"), -1).reshape(-1, 3)
+    pos = np.mod((g + 0.5) * (L / k) + rng.normal(0.0, 0.04, g.shape), L)
+    vel = rng.normal(0.0, np.sqrt(temperature), pos.shape)
+    return dict(lo=np.zeros(3), hi=L, pos=pos, vel=vel - vel.mean(0))
+
+
+def test_small_boxes_pairs_bit_exact():
+    """Boxes with 2, 3 and 4 list-cutoff-wide cells on some axis (the wave
+    build meets a cell under two images when an axis has 2; relative
+    coordinates come from the FP64 positions below 5 cells), next to a
+    larger one: pair sets bit-exact against the oracle before and after
+    dynamics, forces within the FP32+fixed-point tolerance."""
+    boxes = [(6.0, 6.0, 6.0), (8.6, 8.6, 8.6), (11.5, 11.5, 11.5), (6.2, 9.1, 14.6), (15.0, 15.0, 15.0)]
+    systems = [lattice_system(L, 0.8, 90 + i) for i, L in enumerate(boxes)]
+    print(f"[ensemble] small boxes {boxes}", flush=True)
+    with D.Ensemble.from_systems(systems, rc=RC, nl_cutoff=RC, nl_coef=NL_COEF) as e:
+        for rnd in range(2):
+            if rnd:
+                e.run_vv(200, DT)
+                e.build()
+            pos, _, acc = e.download()
+            for s, (sy, p, a) in enumerate(zip(systems, split(e, pos), split(e, acc))):
+                sd = sysdef(dict(sy, pos=p))
+                got = sort_rows(e.pairs(s))
+                exp = oracle_pairs(p, sd)
+                assert len(got) == len(exp), (rnd, s)
+                assert np.array_equal(got, exp), (rnd, s)
+                _p, ref, _vir, _pe = oracle_forces(p, sd)
+                emax, erms = force_error(a, ref)
+                assert emax < 2e-5 and erms < 5e-6, (rnd, s, emax, erms)
+
+
 def test_ensemble_matches_single_box_context():
     systems = small_ensemble(seed=22, nsys=4)
     with D.Ensemble.from_systems(systems, rc=RC, nl_cutoff=RC, nl_coef=NL_COEF) as e:
