@@ -199,7 +199,9 @@ COPY_BYTES = 256 << 20
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    # the workload's own length: 5000 Langevin steps from the FCC start (the
+    # first ~600 melt the lattice and run slower on both forms' kernels)
+    ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--ncell", type=int, default=80, help="FCC cells per edge (80 -> 2,048,000 atoms)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)

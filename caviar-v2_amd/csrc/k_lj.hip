@@ -438,8 +438,10 @@ struct FuseArgs {
   int langevin;
 };
 
+// FUSE: five wavefronts per SIMD (96 registers); the LDS stage allows five
+// blocks per CU
 template <bool MULTI, bool ENERGY, bool ISO, bool FUSE>
-__global__ void __launch_bounds__(kTaskBlock)
+__global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu(FUSE ? 5 : 1)))
     k_lj_f32_task(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
                   const int *__restrict__ nnbr, int stride, int n, const int *__restrict__ cstart,
                   TaskGeom tg, const int4 *__restrict__ tpiece, const int *__restrict__ tstage,
@@ -464,21 +466,6 @@ __global__ void __launch_bounds__(kTaskBlock)
   const int t = task % tg.tpr, row = task / tg.tpr;
   const int h0 = cstart[row * tg.nx + task_x0(tg, t)];
   const int h1 = cstart[row * tg.nx + task_x0(tg, t + 1)];
-  // FUSE, one atom per thread (at most kTaskBlock home atoms): the
-  // integrator's loads for atom h0 + thread (atom order) are issued first,
-  // so that staging and the force loop hide their latency
-  double4 fx_pos = make_double4(0.0, 0.0, 0.0, 0.0), fx_ref = fx_pos;
-  double fx_vel[3] = {0.0, 0.0, 0.0};
-  int fx_id = 0;
-  if (FUSE && h1 - h0 <= kTaskBlock && (int)threadIdx.x < h1 - h0) {
-    const int ia = h0 + (int)threadIdx.x;
-    fx_pos = fz.pos[ia];
-    fx_ref = fz.pos_ref[ia];
-    fx_vel[0] = fz.vel[ia];
-    fx_vel[1] = fz.vel[n + ia];
-    fx_vel[2] = fz.vel[2 * n + ia];
-    if (fz.langevin) fx_id = fz.id[ia];
-  }
   int4 sent = make_int4(0, 0, 0, 0);
   if (h0 < h1) {
     sent = q[h0];
@@ -596,10 +583,14 @@ __global__ void __launch_bounds__(kTaskBlock)
       at_sh[3 * my_t + 2] = my_a[2];
     }
     __syncthreads();
+    // (the integrator's loads are issued here rather than before the force
+    // loop: held through it they cost a wavefront per SIMD of occupancy)
     const int t = (int)threadIdx.x;
     if (t < nh) {
+      const int ia = h0 + t;
+      double v[3] = {fz.vel[ia], fz.vel[n + ia], fz.vel[2 * n + ia]};
       const double ai[3] = {at_sh[3 * t], at_sh[3 * t + 1], at_sh[3 * t + 2]};
-      fuse_step(h0 + t, fx_pos, fx_vel, fx_ref, fx_id, ai);
+      fuse_step(ia, fz.pos[ia], v, fz.pos_ref[ia], fz.langevin ? fz.id[ia] : 0, ai);
     }
   }
   if (ENERGY) {
