@@ -19,8 +19,9 @@ def oracle_pairs(pos, s, cell=True):
     i = np.repeat(np.arange(n), np.diff(offs))
     j = nbr.astype(np.int64)
     g = j >= n
-    jj = np.where(g, owner[np.minimum(j - n, len(owner) - 1)] if len(owner) else j, j)
-    cc = np.where(g, code[np.minimum(j - n, len(code) - 1)] if len(code) else 13, 13)
+    gi = np.clip(j - n, 0, max(len(owner) - 1, 0))  # ghost index (owned entries: unused)
+    jj = np.where(g, owner[gi] if len(owner) else j, j)
+    cc = np.where(g, code[gi] if len(code) else 13, 13)
     return sort_rows(np.stack([i, jj, cc], 1))
 
 

@@ -1,0 +1,74 @@
+"""The 256,000-atom FP64 configuration on the device (40^3 FCC cells,
+rho* = 0.8442, T* = 1.44, rc 2.5, skin 0.3, velocity Verlet dt = 0.005,
+10^4 steps).
+
+- Pair sets are bit-exact against the oracle at the device's positions of
+  each of the first rebuilds, and after the 10^4 steps (a build forced at
+  the final positions).  Trajectories of the two diverge, so each check is a
+  function of the positions the device built from, as in the reference's
+  own predicate.
+- The total energy is sampled every 1000 steps; its per-atom excursion stays
+  within the bound the 4000-atom run keeps.
+
+The oracle's cell-list build takes about 3 s per check at this size, so the
+rebuilds checked are the first four and the last.
+"""
+import numpy as np
+import pytest
+
+from cv2md import device as D
+from cv2md.systems import lj_fluid
+from helpers import oracle_pairs, sort_rows
+
+pytestmark = pytest.mark.gpu
+
+DT = 0.005
+STEPS = 10000
+
+
+def check_pairs(c, s, tag):
+    pos = c.download()[0]
+    got = sort_rows(c.pairs())
+    exp = oracle_pairs(pos, dict(s, pos=pos))
+    print(f"[c2] {tag}: {len(got)} pairs", flush=True)
+    assert len(got) == len(exp), tag
+    assert np.array_equal(got, exp), tag
+
+
+def energy(c):
+    t = c.thermo()
+    return t["ke"] + t["pe"]
+
+
+def test_256k_pairs_at_rebuilds_and_energy():
+    s = lj_fluid(40, temperature=1.44, seed=1002)
+    n = len(s["pos"])
+    assert n == 256000
+    with D.Context.from_sysdef(s, precision=D.CV2_PREC_FP64) as c:
+        c.compute_forces()
+        check_pairs(c, s, "setup")
+        e = [energy(c)]
+        step, checked = 0, 0
+        # velocity Verlet drifts positions before the build of a step, so
+        # after a step that rebuilt the positions are the build positions
+        while checked < 4:
+            r0 = c.stats()["rebuilds"]
+            c.run_vv(1, DT)
+            step += 1
+            if c.stats()["rebuilds"] != r0:
+                check_pairs(c, s, f"rebuild at step {step}")
+                checked += 1
+            assert step < 300, "fewer rebuilds than expected"
+        while step < STEPS:
+            k = min(1000, STEPS - step)
+            c.run_vv(k, DT)
+            step += k
+            e.append(energy(c))
+        rebuilds = c.stats()["rebuilds"]
+        c.build(True)
+        check_pairs(c, s, f"after {STEPS} steps ({rebuilds} rebuilds)")
+    e = np.asarray(e)
+    de = np.abs(e - e[0]).max() / n
+    print(f"[c2] E0 {e[0]:.4f}, max |E - E0| per atom {de:.3e}", flush=True)
+    assert rebuilds > 100
+    assert de < 2e-3
