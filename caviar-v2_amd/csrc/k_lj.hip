@@ -379,7 +379,7 @@ __device__ inline void pair_f32_flat(const int4 qi, const int4 qj, double u0, do
 template <bool STAGED, bool MULTI>
 __device__ inline int4 task_coord(const int *sx, const int *sy, const int *sz,
                                   const unsigned char *stype, const int4 *__restrict__ q,
-                                  const int4 *pc, int S, const int4 sent, int sl) {
+                                  const int4 *pc, int S, const int4 &sent, int sl) {
   if (STAGED) return make_int4(sx[sl], sy[sl], sz[sl], MULTI ? (int)stype[sl] : 0);
   const bool is_sent = sl == S;
   const int4 v = q[slot_to_j(pc, is_sent ? 0 : sl)];  // selects, not a branch
@@ -395,7 +395,7 @@ __device__ inline void task_atom(const int4 *__restrict__ q, const uint32_t *__r
                                  const LJDev *__restrict__ lj, const PairF32 &c0, double u0,
                                  double u1, double u2, const int *sx, const int *sy,
                                  const int *sz, const unsigned char *stype, const int *sj,
-                                 const int4 *pc, int S, const int4 sent, AccF32 &a,
+                                 const int4 *pc, int S, const int4 &sent, AccF32 &a,
                                  float &fcmax) {
   if (cp == 0) return;
   // the next group is loaded while this one is processed
@@ -466,11 +466,16 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
   const int t = task % tg.tpr, row = task / tg.tpr;
   const int h0 = cstart[row * tg.nx + task_x0(tg, t)];
   const int h1 = cstart[row * tg.nx + task_x0(tg, t + 1)];
-  int4 sent = make_int4(0, 0, 0, 0);
-  if (h0 < h1) {
-    sent = q[h0];
-    sent.y = (int)((unsigned)sent.y + 0x80000000u);
-    sent.z = (int)((unsigned)sent.z + 0x80000000u);
+  // the sentinel, in LDS (read by the staging and the unstaged path only)
+  __shared__ int4 sent;
+  if (threadIdx.x == 0) {
+    int4 v = make_int4(0, 0, 0, 0);
+    if (h0 < h1) {
+      v = q[h0];
+      v.y = (int)((unsigned)v.y + 0x80000000u);
+      v.z = (int)((unsigned)v.z + 0x80000000u);
+    }
+    sent = v;
   }
   __syncthreads();
   if (staged) {
