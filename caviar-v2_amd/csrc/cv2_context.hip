@@ -41,6 +41,12 @@ struct cv2_ctx {
   bool need_build = true;  // configuration changed since the last forced build
   bool atoms = false;
   int64_t steps = 0;
+  // MPI halo packing scratch (cv2_halo_pack / cv2_halo_unpack)
+  int *halo_code = nullptr, *halo_src = nullptr, *halo_tiles = nullptr, *halo_toff = nullptr;
+  int *halo_dn = nullptr;
+  long long *halo_base = nullptr;
+  double *halo_buf = nullptr;
+  int64_t halo_n = 0, halo_buf_cap = 0;
   double virial = 0.0, pe = 0.0;
   int *h_flags = nullptr;
   double *h_red = nullptr;
@@ -432,6 +438,52 @@ void set_qscale(BoxDev &b, double span) {
   b.qiso = b.qinv[0] == b.qinv[1] && b.qinv[1] == b.qinv[2];
 }
 
+namespace {
+
+// MPI halo packing helpers (cv2_halo_pack / cv2_halo_unpack).
+HaloGeomDev halo_geom(const cv2_halo_geom *g) {
+  HaloGeomDev h{};
+  for (int a = 0; a < 3; ++a) {
+    h.gi[a] = g->grid_index[a];
+    h.np[a] = g->nprocs[a];
+    h.bc[a] = g->bc[a];
+    h.lo[a] = g->lo_local[a];
+    h.hi[a] = g->hi_local[a];
+    h.size[a] = g->size_global[a];
+  }
+  h.cut = g->ghost_cutoff;
+  h.with_vel = g->with_velocity ? 1 : 0;
+  return h;
+}
+
+// Device scratch of the halo packing for n owned atoms.
+int halo_scratch(cv2_ctx *ctx, int64_t n) {
+  if (n <= ctx->halo_n && ctx->halo_code) return CV2_OK;
+  for (void *p : {(void *)ctx->halo_code, (void *)ctx->halo_src, (void *)ctx->halo_tiles,
+                  (void *)ctx->halo_toff})
+    if (p) (void)hipFree(p);
+  const int64_t nt = halo_tiles((int)n);
+  HIPCHK(dalloc(&ctx->halo_code, n));
+  HIPCHK(dalloc(&ctx->halo_src, n));
+  HIPCHK(dalloc(&ctx->halo_tiles, nt * 27));
+  HIPCHK(dalloc(&ctx->halo_toff, nt * 27));
+  if (!ctx->halo_dn) HIPCHK(dalloc(&ctx->halo_dn, 27));
+  if (!ctx->halo_base) HIPCHK(dalloc(&ctx->halo_base, 3 * 64));
+  ctx->halo_n = n;
+  return CV2_OK;
+}
+
+int halo_buffer(cv2_ctx *ctx, int64_t doubles) {
+  if (doubles <= ctx->halo_buf_cap) return CV2_OK;
+  if (ctx->halo_buf) (void)hipFree(ctx->halo_buf);
+  ctx->halo_buf = nullptr;
+  HIPCHK(dalloc(&ctx->halo_buf, doubles));
+  ctx->halo_buf_cap = doubles;
+  return CV2_OK;
+}
+
+}  // namespace
+
 }  // namespace cv2
 
 extern "C" {
@@ -472,6 +524,10 @@ int cv2_destroy(cv2_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_state(ctx->s);
+  for (void *p : {(void *)ctx->halo_code, (void *)ctx->halo_src, (void *)ctx->halo_tiles,
+                  (void *)ctx->halo_toff, (void *)ctx->halo_dn, (void *)ctx->halo_base,
+                  (void *)ctx->halo_buf})
+    if (p) (void)hipFree(p);
   if (ctx->lj_dev) (void)hipFree(ctx->lj_dev);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
@@ -653,6 +709,120 @@ int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *t
   if (s.write_q) launch_pack_q(s, ctx->b, ctx->stream);
   launch_check_disp(s, ctx->b, ctx->stream);  // ghosts compared by index, :139-144
   HIPCHK(hipStreamSynchronize(ctx->stream));  // p4/idv are stack buffers
+  return CV2_OK;
+}
+
+
+int cv2_halo_pack(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[27], int64_t cap,
+                  double *out, int64_t *need) {
+  if (!ctx || !g || !counts || !need || cap < 0 || (cap > 0 && !out))
+    return fail(ctx, CV2_ERR_BAD_ARG, "bad halo pack arguments");
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload atoms before packing a halo");
+  for (int a = 0; a < 3; ++a)
+    if (g->nprocs[a] < 1 || g->grid_index[a] < 0 || g->grid_index[a] >= g->nprocs[a] ||
+        !(g->hi_local[a] > g->lo_local[a]))
+      return fail(ctx, CV2_ERR_BAD_ARG, "bad process grid or local box");
+  (void)hipSetDevice(ctx->device);
+  const int n = (int)ctx->n;
+  int rc = halo_scratch(ctx, std::max(n, 1));
+  if (rc) return rc;
+  const HaloGeomDev h = halo_geom(g);
+  const int nt = halo_tiles(n);
+  std::vector<int> tc((size_t)nt * 27);
+  launch_halo_classify(ctx->s, h, ctx->halo_code, ctx->halo_src, ctx->halo_tiles, ctx->stream);
+  if (nt > 0)
+    HIPCHK(hipMemcpyAsync(tc.data(), ctx->halo_tiles, sizeof(int) * tc.size(), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // per-direction totals, tile offsets and packet bases (exclusive scans)
+  std::vector<int> toff(tc.size());
+  int dn[27];
+  long long base[27];
+  const int64_t total = g->with_velocity ? 8 : 5;
+  int64_t at = 0;
+  for (int d = 0; d < 27; ++d) {
+    int64_t c = 0;
+    for (int t = 0; t < nt; ++t) {
+      toff[(size_t)t * 27 + d] = (int)c;
+      c += tc[(size_t)t * 27 + d];
+    }
+    counts[d] = c;
+    dn[d] = (int)c;
+    base[d] = at;
+    at += total * c;
+  }
+  *need = at;
+  if (at == 0 || at > cap) return CV2_OK;
+  rc = halo_buffer(ctx, at);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(ctx->halo_toff, toff.data(), sizeof(int) * toff.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->halo_dn, dn, sizeof(dn), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->halo_base, base, sizeof(base), hipMemcpyHostToDevice, ctx->stream));
+  launch_halo_write(ctx->s, h, ctx->halo_code, ctx->halo_src, ctx->halo_toff, ctx->halo_base,
+                    ctx->halo_dn, ctx->halo_buf, ctx->stream);
+  HIPCHK(hipMemcpyAsync(out, ctx->halo_buf, sizeof(double) * at, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return CV2_OK;
+}
+
+int cv2_halo_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const double *packets,
+                    int32_t with_velocity) {
+  if (!ctx || npackets < 0 || npackets > 64 || (npackets > 0 && !counts))
+    return fail(ctx, CV2_ERR_BAD_ARG, "bad halo unpack arguments (at most 64 packets)");
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload owned atoms before ghosts");
+  (void)hipSetDevice(ctx->device);
+  const int64_t total = with_velocity ? 8 : 5;
+  long long first[64], pbase[64], pn[64];
+  int64_t ng = 0, at = 0;
+  for (int p = 0; p < npackets; ++p) {
+    if (counts[p] < 0) return fail(ctx, CV2_ERR_BAD_ARG, "negative packet count");
+    first[p] = ng;
+    pbase[p] = at;
+    pn[p] = counts[p];
+    // types are validated here, as cv2_set_ghosts does
+    for (int64_t c = 0; c < counts[p]; ++c) {
+      const double t = packets[at + counts[p] + c];
+      if (!(t >= 0.0 && t < ctx->lj.nt && t == (double)(int)t))
+        return fail(ctx, CV2_ERR_BAD_ARG, "ghost type out of range");
+    }
+    ng += counts[p];
+    at += total * counts[p];
+  }
+  if (ng > 0 && !packets) return fail(ctx, CV2_ERR_BAD_ARG, "missing packets");
+  const bool count_changed = ng != ctx->ng || !ctx->host_halo;
+  if (!ctx->host_halo) {
+    ctx->host_halo = true;
+    int rc = setup_box(ctx);
+    if (rc) return rc;
+  }
+  int rc = alloc_positions(ctx, ctx->n + ng);
+  if (rc) return rc;
+  rc = halo_scratch(ctx, std::max<int64_t>(ctx->n, 1));
+  if (rc) return rc;
+  rc = halo_buffer(ctx, std::max<int64_t>(at, 1));
+  if (rc) return rc;
+  DevState &s = ctx->s;
+  if (at > 0)
+    HIPCHK(hipMemcpyAsync(ctx->halo_buf, packets, sizeof(double) * at, hipMemcpyHostToDevice,
+                          ctx->stream));
+  if (npackets > 0) {
+    long long tab[3 * 64];
+    for (int p = 0; p < npackets; ++p) {
+      tab[p] = first[p];
+      tab[64 + p] = pbase[p];
+      tab[128 + p] = pn[p];
+    }
+    HIPCHK(hipMemcpyAsync(ctx->halo_base, tab, sizeof(tab), hipMemcpyHostToDevice, ctx->stream));
+  }
+  ctx->ng = ng;
+  s.ng = (int)ng;
+  launch_halo_unpack(s, ctx->halo_buf, npackets, ctx->halo_base, ctx->halo_base + 64,
+                     ctx->halo_base + 128, (int)ng, ctx->stream);
+  if (count_changed) ctx->need_build = true;  // neighborlist.cpp:118-121
+  if (s.write_q) launch_pack_q(s, ctx->b, ctx->stream);
+  launch_check_disp(s, ctx->b, ctx->stream);  // ghosts compared by index, :139-144
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // tab is a stack buffer
   return CV2_OK;
 }
 

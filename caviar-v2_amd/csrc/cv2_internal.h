@@ -396,4 +396,20 @@ void launch_thermo_ens(const DevState &s, const EnsDev &e, const LJDev *lj_dev,
                        const long long *eatom, double *out, hipStream_t st);
 void launch_fill_int(int *p, int n, int value, hipStream_t st);
 
+// MPI halo packets (k_halo.hip): the domain's place in the process grid.
+struct HaloGeomDev {
+  int gi[3], np[3], bc[3];
+  double lo[3], hi[3], size[3];
+  double cut;
+  int with_vel;
+};
+int halo_tiles(int n);
+void launch_halo_classify(const DevState &s, const HaloGeomDev &g, int *code_by_tag,
+                          int *src_by_tag, int *tile_counts, hipStream_t st);
+void launch_halo_write(const DevState &s, const HaloGeomDev &g, const int *code_by_tag,
+                       const int *src_by_tag, const int *tile_off, const long long *dir_base,
+                       const int *dir_n, double *out, hipStream_t st);
+void launch_halo_unpack(DevState &s, const double *in, int npk, const long long *pk_first,
+                        const long long *pk_base, const long long *pk_n, int ng, hipStream_t st);
+
 }  // namespace cv2

@@ -36,6 +36,12 @@ class Timing(C.Structure):
                 ("integrate_ms", C.c_double)]
 
 
+class HaloGeom(C.Structure):
+    _fields_ = [("grid_index", C.c_int32 * 3), ("nprocs", C.c_int32 * 3), ("lo_local", C.c_double * 3),
+                ("hi_local", C.c_double * 3), ("size_global", C.c_double * 3), ("bc", C.c_int32 * 3),
+                ("ghost_cutoff", C.c_double), ("with_velocity", C.c_int32)]
+
+
 class EnsTuning(C.Structure):
     _fields_ = [("force_block", C.c_int32), ("build_block", C.c_int32), ("rebuild_all", C.c_int32),
                 ("ms_per_step", C.c_double)]
@@ -74,6 +80,8 @@ def lib():
             "cv2_set_precision": ([vp, C.c_int], C.c_int),
             "cv2_upload_atoms": ([vp, C.c_int64, _d, _d, _i32, _i64], C.c_int),
             "cv2_set_ghosts": ([vp, C.c_int64, _d, _i32, _i64], C.c_int),
+            "cv2_halo_pack": ([vp, C.POINTER(HaloGeom), _i64, C.c_int64, _d, _i64], C.c_int),
+            "cv2_halo_unpack": ([vp, C.c_int32, _i64, _d, C.c_int32], C.c_int),
             "cv2_build_neighbors": ([vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
             "cv2_compute_forces": ([vp], C.c_int),
             "cv2_run_vv": ([vp, C.c_int64, C.c_double], C.c_int),
@@ -174,6 +182,32 @@ class Context:
         return c
 
     # ---- work
+    def halo_pack(self, grid_index, nprocs, lo_local, hi_local, size_global, bc, ghost_cutoff,
+                  with_velocity=False):
+        """MPI halo packets of the owned atoms in the reference's layout:
+        (counts[27], packets concatenated in direction order d = i*9+j*3+k)."""
+        g = HaloGeom()
+        for a in range(3):
+            g.grid_index[a] = int(grid_index[a]); g.nprocs[a] = int(nprocs[a]); g.bc[a] = int(bc[a])
+            g.lo_local[a] = float(lo_local[a]); g.hi_local[a] = float(hi_local[a])
+            g.size_global[a] = float(size_global[a])
+        g.ghost_cutoff = float(ghost_cutoff)
+        g.with_velocity = int(bool(with_velocity))
+        counts = np.zeros(27, np.int64)
+        need = C.c_int64()
+        self._chk(self.L.cv2_halo_pack(self.h, C.byref(g), _p(counts, _i64), 0, None, C.byref(need)))
+        out = np.empty(max(need.value, 1))
+        self._chk(self.L.cv2_halo_pack(self.h, C.byref(g), _p(counts, _i64), need.value, _p(out, _d),
+                                       C.byref(need)))
+        return counts, out[:need.value]
+
+    def halo_unpack(self, counts, packets, with_velocity=False):
+        """Ghosts from received packets, appended in the order given."""
+        cnt = np.ascontiguousarray(counts, np.int64)
+        pk = np.ascontiguousarray(packets, np.float64)
+        self._chk(self.L.cv2_halo_unpack(self.h, len(cnt), _p(cnt, _i64), _p(pk, _d) if pk.size else None,
+                                         int(bool(with_velocity))))
+
     def build(self, force=True):
         r = C.c_int(0)
         self._chk(self.L.cv2_build_neighbors(self.h, int(force), C.byref(r)))

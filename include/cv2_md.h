@@ -101,6 +101,43 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
 int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *type,
                    const int64_t *id);
 
+/* MPI halo packets: the device side of Atom_data::exchange_ghost_mpi_shared_atoms
+ * (exchange_ghost_mpi_shared_atoms.cpp), so that a domain's share exchanges its
+ * halo in the reference's own packet format without staging the owned atoms
+ * on the host.  The domain's place in the process grid, as Domain holds it
+ * (grid_index_*, nprocs_*, lower_local / upper_local, size_global,
+ * boundary_condition) plus the ghost cutoff and make_ghost_velocity. */
+typedef struct {
+  int32_t grid_index[3];
+  int32_t nprocs[3];
+  double lo_local[3], hi_local[3];
+  double size_global[3];
+  int32_t bc[3];
+  double ghost_cutoff;
+  int32_t with_velocity; /* 8 doubles per atom (with velocities), else 5 */
+} cv2_halo_geom;
+
+/* The packing half (:113-325): every owned atom within ghost_cutoff of a face
+ * is sent to each of the up to seven directions (x, y, z, xy, xz, yz, xyz) its
+ * face flags span; direction d = i*9 + j*3 + k with i = x_val + 1, etc.
+ * counts[27] receives the atoms per direction (counts[13] = 0).  The packets
+ * are written to out concatenated in direction order, each
+ * [N ids][N types][3N positions][3N velocities] as doubles, atoms in upload
+ * order, positions shifted by size_global where they cross a periodic grid
+ * edge.  *need receives the doubles required; out is written only when cap
+ * suffices (call with cap 0 to size it). */
+int cv2_halo_pack(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[27], int64_t cap,
+                  double *out, int64_t *need);
+
+/* The unpacking half (:405-470): the ghosts become the atoms of npackets
+ * received packets, appended in the order given (the reference appends the
+ * received packets in direction order, then the domain's own periodic
+ * images), packet p holding counts[p] atoms in the layout above.  Equivalent
+ * to cv2_set_ghosts with the packets' positions, types and ids; ghost
+ * velocities are not used by the force path. */
+int cv2_halo_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const double *packets,
+                    int32_t with_velocity);
+
 /* Neighbour-list build.  Neighborlist::build(bool) (neighborlist.cpp:77-98):
  * wraps positions into the box (exchange_owned_single_md_domain.cpp:29-57),
  * re-sorts atoms by cell and rebuilds the list when force != 0 or when

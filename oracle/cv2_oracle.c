@@ -62,6 +62,81 @@ static void emit_ghost(int64_t *k, int64_t cap, double *gpos, int64_t *owner, in
   ++*k;
 }
 
+/* exchange_ghost_mpi_shared_atoms.cpp:113-203 (classification), :222-241
+ * (layout), :258-325 (fill). */
+int64_t orc_halo_pack(int64_t n, const double *pos, const double *vel, const int64_t *id,
+                      const int32_t *type, const int32_t gi[3], const int32_t np[3],
+                      const double lo[3], const double hi[3], const double size_global[3],
+                      const int32_t bc[3], double g, int32_t with_vel, int64_t counts[27],
+                      int64_t cap, double *out) {
+  const int64_t total = with_vel ? 8 : 5;
+  int32_t *dval = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)(n > 0 ? n : 1));
+  for (int d = 0; d < 27; ++d) counts[d] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int v[3];
+    for (int a = 0; a < 3; ++a) {
+      const double x = pos[3 * i + a];
+      v[a] = 0;
+      if (x < lo[a] + g) {
+        v[a] = -1;
+        if (gi[a] == 0) v[a] *= bc[a];
+      } else if (x > hi[a] - g) {
+        v[a] = +1;
+        if (gi[a] == np[a] - 1) v[a] *= bc[a];
+      }
+      dval[3 * i + a] = v[a];
+    }
+    /* x, y, z, xy, xz, yz, xyz: each combination of the nonzero components */
+    for (int m = 1; m < 8; ++m) {
+      int ok = 1, c[3];
+      for (int a = 0; a < 3; ++a) {
+        const int use = (m >> a) & 1;
+        if (use && v[a] == 0) ok = 0;
+        c[a] = use ? v[a] + 1 : 1;
+      }
+      if (ok) counts[c[0] * 9 + c[1] * 3 + c[2]]++;
+    }
+  }
+  int64_t need = 0, base[27];
+  for (int d = 0; d < 27; ++d) {
+    base[d] = need;
+    need += total * counts[d];
+  }
+  if (need <= cap) {
+    int64_t fill[27] = {0};
+    for (int64_t i = 0; i < n; ++i) {
+      const int *v = dval + 3 * i;
+      for (int m = 1; m < 8; ++m) {
+        int ok = 1, c[3];
+        for (int a = 0; a < 3; ++a) {
+          const int use = (m >> a) & 1;
+          if (use && v[a] == 0) ok = 0;
+          c[a] = use ? v[a] + 1 : 1;
+        }
+        if (!ok) continue;
+        const int d = c[0] * 9 + c[1] * 3 + c[2];
+        const int64_t N = counts[d], k = fill[d]++;
+        double *pk = out + base[d];
+        double p[3];
+        for (int a = 0; a < 3; ++a) {
+          p[a] = pos[3 * i + a];
+          if (bc[a] == 1) {
+            if (gi[a] == 0 && c[a] == 0) p[a] += size_global[a];
+            if (gi[a] == np[a] - 1 && c[a] == 2) p[a] -= size_global[a];
+          }
+        }
+        pk[k] = (double)id[i];
+        pk[N + k] = (double)type[i];
+        for (int a = 0; a < 3; ++a) pk[2 * N + 3 * k + a] = p[a];
+        if (with_vel)
+          for (int a = 0; a < 3; ++a) pk[5 * N + 3 * k + a] = vel[3 * i + a];
+      }
+    }
+  }
+  free(dval);
+  return need;
+}
+
 int64_t orc_ghosts(int64_t n, const double *pos, const orc_box *b, double g, int64_t cap,
                    double *gpos, int64_t *owner, int32_t *code) {
   const double llow[3] = {b->lo[0] + g, b->lo[1] + g, b->lo[2] + g};

@@ -47,6 +47,22 @@ int orc_wrap(int64_t n, double *pos, const orc_box *b, int32_t *msd);
 int64_t orc_ghosts(int64_t n, const double *pos, const orc_box *b, double ghost_cutoff,
                    int64_t cap, double *gpos, int64_t *owner, int32_t *code);
 
+/* Halo packets of the MPI ghost exchange, exchange_ghost_mpi_shared_atoms.cpp:
+ * direction of each owned atom in index order (:113-203: below lower_local +
+ * g gives -1, above upper_local - g gives +1, zeroed at a non-periodic grid
+ * edge; an atom goes to each of the x, y, z, xy, xz, yz, xyz directions its
+ * nonzero components span), packet layout (:222-241: N ids, N types, 3N
+ * positions, 3N velocities when with_vel) and the periodic shift of
+ * positions sent across the global box (:266-305).  Direction d = i*9 + j*3
+ * + k with i = x_val + 1 etc.; counts[27] receives the atoms per direction
+ * and out the packets concatenated in d order.  Returns the doubles needed
+ * (writes at most cap). */
+int64_t orc_halo_pack(int64_t n, const double *pos, const double *vel, const int64_t *id,
+                      const int32_t *type, const int32_t grid_index[3], const int32_t nprocs[3],
+                      const double lo_local[3], const double hi_local[3],
+                      const double size_global[3], const int32_t bc[3], double ghost_cutoff,
+                      int32_t with_vel, int64_t counts[27], int64_t cap, double *out);
+
 /* neighborlist.cpp:247-297 (O(N^2) default).  CSR output: offs[n+1],
  * nbr[...] with ghost g stored as n+g.  Returns the total number of entries
  * (writes at most cap). */

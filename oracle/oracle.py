@@ -75,6 +75,9 @@ def oracle_lib():
         L.orc_wrap.restype = C.c_int
         L.orc_ghosts.argtypes = [C.c_int64, _d, C.POINTER(Box), C.c_double, C.c_int64, _d, _i64, _i32]
         L.orc_ghosts.restype = C.c_int64
+        L.orc_halo_pack.argtypes = [C.c_int64, _d, _d, _i64, _i32, _i32, _i32, _d, _d, _d, _i32,
+                                    C.c_double, C.c_int32, _i64, C.c_int64, _d]
+        L.orc_halo_pack.restype = C.c_int64
         L.orc_nlist_brute.argtypes = [C.c_int64, _d, C.c_int64, _d, C.c_double, C.c_double,
                                       C.c_int64, _i64, _u32]
         L.orc_nlist_brute.restype = C.c_int64
@@ -124,6 +127,29 @@ def ghosts(pos, box, ghost_cutoff):
         if k <= cap:
             return g[:k].copy(), o[:k].copy(), c[:k].copy()
         cap = k
+
+
+def halo_pack(pos, vel, id_, type_, grid_index, nprocs, lo_local, hi_local, size_global, bc,
+              ghost_cutoff, with_vel=False):
+    """MPI halo packets in the reference's layout: (counts[27], packets
+    concatenated in direction order d = i*9 + j*3 + k)."""
+    p = _f64(pos)
+    v = _f64(vel) if vel is not None else np.zeros_like(p)
+    n = len(p)
+    ids = np.ascontiguousarray(id_, np.int64)
+    ty = np.ascontiguousarray(type_, np.int32)
+    a32 = lambda x: np.ascontiguousarray(x, np.int32)
+    gi, npr, bcs = a32(grid_index), a32(nprocs), a32(bc)
+    lo, hi, sg = _f64(lo_local), _f64(hi_local), _f64(size_global)
+    counts = np.zeros(27, np.int64)
+    L = oracle_lib()
+    args = (n, _p(p, _d), _p(v, _d), _p(ids, _i64), _p(ty, _i32), _p(gi, _i32), _p(npr, _i32),
+            _p(lo, _d), _p(hi, _d), _p(sg, _d), _p(bcs, _i32), ghost_cutoff, int(with_vel),
+            _p(counts, _i64))
+    need = L.orc_halo_pack(*args, 0, None)
+    out = np.empty(max(need, 1))
+    L.orc_halo_pack(*args, need, _p(out, _d))
+    return counts, out[:need]
 
 
 def _csr(fn, n, *args):

@@ -169,3 +169,75 @@ def test_cell_list_equals_brute_force_as_sets():
 def test_random_velocity_fixture(golden):
     d = golden("random_velocity")
     assert np.array_equal(O.ref_random_velocity(int(d["n"]), int(d["seed"]), float(d["amplitude"])), d["vel"])
+
+
+def _packets(counts, out, with_vel=False):
+    """Split the concatenated halo packets: {direction: (id, type, pos, vel)}."""
+    total = 8 if with_vel else 5
+    res, at = {}, 0
+    for d in range(27):
+        n = int(counts[d])
+        if n:
+            pk = out[at:at + total * n]
+            res[d] = (pk[:n], pk[n:2 * n], pk[2 * n:5 * n].reshape(n, 3),
+                      pk[5 * n:8 * n].reshape(n, 3) if with_vel else None)
+        at += total * n
+    return res
+
+
+def test_halo_pack_one_domain_equals_single_domain_ghosts():
+    """With a 1x1x1 process grid every direction is the domain itself, and
+    the union of the 26 packets must be the single-domain ghost set of
+    exchange_ghost_single_md_domain (pinned against the compiled reference):
+    the same images, ids and positions bit for bit."""
+    s = lj_fluid(5, seed=17)
+    box = O.make_box(s["lo"], s["hi"])
+    p, _ = O.wrap(s["pos"] + np.random.default_rng(3).normal(0, 0.05, s["pos"].shape), box)
+    n = len(p)
+    ids = np.arange(n) + 100
+    L = s["hi"] - s["lo"]
+    counts, out = O.halo_pack(p, None, ids, np.zeros(n, np.int32), (0, 0, 0), (1, 1, 1), s["lo"], s["hi"],
+                              L, (1, 1, 1), 2.8)
+    assert counts[13] == 0  # (1, 1, 1) is never a direction
+    got = np.concatenate([np.column_stack([i, x]) for i, _t, x, _v in _packets(counts, out).values()])
+    gpos, owner, _code = O.ghosts(p, box, 2.8)
+    exp = np.column_stack([ids[owner], gpos])
+    key = lambda a: a[np.lexsort(a.T[::-1])]
+    assert len(got) == len(exp)
+    assert np.array_equal(key(got), key(exp))
+
+
+def test_halo_pack_known_answer_two_domains():
+    """2x1x1 grid, box [0, 10)^3 split at x = 5, ghost cutoff 1.  Domain 0
+    sends atoms below x = 1 to -x (shifted by +10: it wraps to domain 1's
+    far face) and atoms above x = 4 to +x (unshifted), in index order, with
+    the y/z directions and their combinations (periodic, one process per
+    axis) alongside; velocities follow the positions."""
+    pos = np.array([[0.5, 5.0, 5.0],    # -x
+                    [4.5, 5.0, 5.0],    # +x
+                    [2.5, 0.5, 5.0],    # -y
+                    [0.2, 9.7, 5.0],    # -x, +y and -x+y
+                    [2.5, 5.0, 5.0],    # interior
+                    [4.9, 0.1, 9.9]])   # +x, -y, +z and all combinations
+    vel = np.arange(18, dtype=float).reshape(6, 3)
+    ids = np.array([7, 8, 9, 10, 11, 12])
+    ty = np.array([0, 1, 0, 1, 0, 1], np.int32)
+    counts, out = O.halo_pack(pos, vel, ids, ty, (0, 0, 0), (2, 1, 1), (0, 0, 0), (5, 10, 10),
+                              (10, 10, 10), (1, 1, 1), 1.0, with_vel=True)
+    pk = _packets(counts, out, True)
+    D = lambda i, j, k: i * 9 + j * 3 + k
+    exp_sets = {D(0, 1, 1): [7, 10], D(2, 1, 1): [8, 12], D(1, 0, 1): [9, 12], D(1, 2, 1): [10],
+                D(1, 1, 2): [12], D(0, 2, 1): [10], D(2, 0, 1): [12], D(2, 1, 2): [12],
+                D(1, 0, 2): [12], D(2, 0, 2): [12]}
+    assert sorted(pk) == sorted(exp_sets)
+    for d, want in exp_sets.items():
+        assert list(pk[d][0].astype(int)) == want, d
+    i0, t0, x0, v0 = pk[D(0, 1, 1)]
+    assert np.array_equal(x0, [[10.5, 5.0, 5.0], [10.2, 9.7, 5.0]])
+    assert np.array_equal(t0, [0, 1]) and np.array_equal(v0, vel[[0, 3]])
+    i2, _t2, x2, _v2 = pk[D(2, 1, 1)]
+    assert np.array_equal(x2, pos[[1, 5]])  # sent to +x inside the global box: no shift
+    _i, _t, xy, _v = pk[D(1, 2, 1)]
+    assert np.array_equal(xy, [[0.2, 9.7 - 10.0, 5.0]])  # +y wraps (one process along y)
+    _i, _t, xyz, _v = pk[D(2, 0, 2)]
+    assert np.allclose(xyz, [[4.9, 10.1, -0.1]]) and xyz[0, 1] == 0.1 + 10.0 and xyz[0, 2] == 9.9 - 10.0
