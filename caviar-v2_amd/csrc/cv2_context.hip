@@ -568,12 +568,25 @@ int cv2_set_lj(cv2_ctx *ctx, int ntypes, const double *eps, const double *sigma,
   if (cutoff < 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "Force field cutoff must be non-negative.");
   {
     std::string err;
+    const double jt = ctx->lj.jump_tol;  // set independently (cv2_set_jump_tol)
     if (!build_lj_table(ctx->lj, ntypes, eps, sigma, cutoff_list, cutoff, mass, err))
       return fail(ctx, CV2_ERR_BAD_ARG, err);
+    ctx->lj.jump_tol = jt;
   }
   ctx->lj_set = true;
   ctx->need_build = true;
   ctx->forces_valid = false;
+  hipError_t e = hipMemcpyAsync(ctx->lj_dev, &ctx->lj, sizeof(LJDev), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(ctx, CV2_ERR_HIP, hipGetErrorString(e));
+  return CV2_OK;
+}
+
+int cv2_set_jump_tol(cv2_ctx *ctx, double jump_tol) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  ctx->lj.jump_tol = jump_tol > 0.0 ? jump_tol : 0.0;
+  ctx->forces_valid = false;
+  if (!ctx->lj_set) return CV2_OK;  // uploaded with the LJ table
   hipError_t e = hipMemcpyAsync(ctx->lj_dev, &ctx->lj, sizeof(LJDev), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return fail(ctx, CV2_ERR_HIP, hipGetErrorString(e));

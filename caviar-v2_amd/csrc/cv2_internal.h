@@ -83,6 +83,10 @@ struct LJDev {
   float e12f[kMaxTypes * kMaxTypes];  // 12 * 4 eps * 2^32
   float fshf[kMaxTypes * kMaxTypes];  // 4 eps * fshift * 2^32
   double r2ovf[kMaxTypes * kMaxTypes];  // r^2 below which |F| may exceed 2^19
+  // jump_fix (lj.cpp:459-468): a listed pair closer than sqrt(jump_tol) takes
+  // the force of r^2 = jump_tol along its own direction; 0 = off.  Served by
+  // the species-table kernels only (MULTI), which runs that select.
+  double jump_tol;
 };
 
 // Fixed-point scales of the FP32 path: forces (and accelerations) in units of

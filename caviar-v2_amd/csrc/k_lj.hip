@@ -92,6 +92,7 @@ __global__ void __launch_bounds__(kBlock)
     const int nt = MULTI ? lj->nt : 1;
     // single-type parameters stay in registers for the whole loop
     const double rc2_0 = lj->rc2[0], sig2_0 = lj->sig2[0], eps4_0 = lj->eps4[0], fsh_0 = lj->fshift[0];
+    const double jt = MULTI ? lj->jump_tol : 0.0;  // jump_fix (species-table kernels only)
     const int cp = (nnbr[i] + 3) & ~3;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int id_i = (ENERGY && HALO) ? id[i] : 0;
@@ -112,10 +113,11 @@ __global__ void __launch_bounds__(kBlock)
           gz -= (double)((int)(img >> 4) - 1) * b.L[2];
         }
         const double dx = gx - pi.x, dy = gy - pi.y, dz = gz - pi.z;
-        const double r2 = dx * dx + dy * dy + dz * dz;
+        double r2 = dx * dx + dy * dy + dz * dz;
         const int tp = MULTI ? ti * nt + (int)pj[u].w : 0;
         const double rc2 = MULTI ? lj->rc2[tp] : rc2_0;
         if (r2 <= rc2 && r2 > 0.0) {
+          if (MULTI && r2 < jt) r2 = jt;  // lj.cpp:459-468
           // 1/r^2: hardware reciprocal and two Newton steps (about 1 ulp)
           double r2i = __builtin_amdgcn_rcp(r2);
           r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
@@ -165,6 +167,7 @@ __global__ void __launch_bounds__(kBlock)
 // s6 = (sigma^2 / r2)^3 = (sig2' r2i')^3, sig2' = sigma^2 / g^2.
 struct PairF32 {
   double rc2, sig2, r2ovf;  // in units of g^2
+  double jt;                // jump_fix clamp of r^2 (0: off), units of g^2
   float e6, e12, fsh;
   float eps4;
   double ecut, ecut2;       // energy: ecut in units of 1/g^2
@@ -177,6 +180,7 @@ __device__ inline PairF32 pair_f32_table(const LJDev *__restrict__ lj, int tp) {
   c.rc2 = lj->rc2[tp];
   c.sig2 = lj->sig2[tp];
   c.r2ovf = lj->r2ovf[tp];
+  c.jt = lj->jump_tol;
   c.e6 = lj->e6f[tp];
   c.e12 = lj->e12f[tp];
   c.fsh = lj->fshf[tp];
@@ -193,6 +197,7 @@ __device__ inline PairF32 pair_f32(const LJDev *__restrict__ lj, int tp, double 
   c.rc2 = lj->rc2[tp] * g2i;
   c.sig2 = lj->sig2[tp] * g2i;
   c.r2ovf = lj->r2ovf[tp] * g2i;
+  c.jt = lj->jump_tol * g2i;
   const double e4s = lj->eps4[tp] * kForceScale / g;
   c.e6 = (float)(6.0 * e4s);
   c.e12 = (float)(12.0 * e4s);
@@ -216,7 +221,7 @@ struct AccF32 {
   bool ovf = false;
 };
 
-template <bool ISO, bool ENERGY>
+template <bool ISO, bool ENERGY, bool JUMP = false>
 __device__ inline void pair_f32_entry(const int4 qi, const int4 qj, const double u0, const double u1,
                                       const double u2, const PairF32 &c, bool once, AccF32 &a) {
   double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y), dz = (double)(qj.z - qi.z);
@@ -225,8 +230,9 @@ __device__ inline void pair_f32_entry(const int4 qi, const int4 qj, const double
     dy *= u1;
     dz *= u2;
   }
-  const double r2 = dx * dx + dy * dy + dz * dz;
+  double r2 = dx * dx + dy * dy + dz * dz;
   if (r2 <= c.rc2 && r2 > 0.0) {
+    if (JUMP && r2 < c.jt) r2 = c.jt;  // jump_fix, lj.cpp:459-468
     double r2i = __builtin_amdgcn_rcp(r2);
     r2i = fma(r2i, fma(-r2, r2i, 1.0), r2i);
     const double s2 = c.sig2 * r2i;
@@ -285,8 +291,8 @@ __global__ void __launch_bounds__(kBlock)
         const int j = (int)(e[v] & kIndexMask);
         const bool once = ENERGY && (HALO && j >= n ? (id_i < id[j]) : (i < j));
         if (MULTI)
-          pair_f32_entry<false, ENERGY>(qi, qj[v], u0, u1, u2,
-                                        pair_f32_table(lj, ti * nt + (qj[v].w & 7)), once, a);
+          pair_f32_entry<false, ENERGY, true>(qi, qj[v], u0, u1, u2,
+                                              pair_f32_table(lj, ti * nt + (qj[v].w & 7)), once, a);
         else
           pair_f32_entry<ISO, ENERGY>(qi, qj[v], u0, u1, u2, c0, once, a);
       }
@@ -418,7 +424,7 @@ __device__ inline void task_atom(const int4 *__restrict__ q, const uint32_t *__r
         int j = 0;
         if (ENERGY) j = sl[v] == S ? 0x7fffffff : (STAGED ? sj[sl[v]] : slot_to_j(pc, sl[v]));
         const PairF32 c = MULTI ? pair_f32_table(lj, ti * nt + (qj[v].w & 7)) : c0;
-        pair_f32_entry<ISO, ENERGY>(qi, qj[v], u0, u1, u2, c, ENERGY && i < j, a);
+        pair_f32_entry<ISO, ENERGY, MULTI>(qi, qj[v], u0, u1, u2, c, ENERGY && i < j, a);
       } else {
         pair_f32_flat<ISO>(qi, qj[v], u0, u1, u2, c0, a.sx, a.sy, a.sz, fcmax);
       }
@@ -761,7 +767,7 @@ bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, cons
   k_lj_f32_task<M, false, I, true><<<tgrid, kTaskBlock, 0, st>>>(                              \
       s.q, s.nbr, s.nnbr, s.stride, s.n, s.cell_start, s.tg, s.tpiece, s.tstage, s.stage_limit, \
       lj_dev, b, s.acc, part, s.flags, treal, fz)
-  if (lj.nt > 1)
+  if (lj.nt > 1 || lj.jump_tol > 0.0)
     CV2_LJF(true, false);
   else if (b.qiso)
     CV2_LJF(false, true);
@@ -774,7 +780,8 @@ bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, cons
 void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                    int precision, bool energy, hipStream_t st) {
   if (s.n == 0) return;
-  const bool multi = lj.nt > 1;
+  // jump_fix runs on the species-table kernels
+  const bool multi = lj.nt > 1 || lj.jump_tol > 0.0;
   const bool halo = s.ng > 0;
 #define CV2_LJ(M, E, H)                                      \
   if (multi == M && energy == E && halo == H) {              \

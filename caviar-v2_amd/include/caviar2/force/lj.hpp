@@ -37,6 +37,8 @@ class Lj : public Force_field {
   // Dense ntypes x ntypes tables as the device uses them (missing entries 0).
   void tables(int ntypes, std::vector<double> &eps, std::vector<double> &sigma,
               std::vector<double> &cutoff_list, bool &has_cutoff_list) const;
+  // The jump_fix clamp of r^2 the device applies (0 when jump_fix is off).
+  double device_jump_tol() const { return jump_fix ? jump_tol : 0.0; }
 
  private:
   std::vector<std::vector<double>> epsilon, sigma, cutoff_list, lambda_s, lambda_e;

@@ -113,6 +113,7 @@ void configure(DeviceSession &s, Precision precision, int device) {
   s.check(cv2_set_lj(s.ctx, nt, eps.data(), sig.data(), has_cl ? cl.data() : nullptr, lj->cutoff,
                      ad.atom_type_params.mass.data()),
           "cv2_set_lj");
+  s.check(cv2_set_jump_tol(s.ctx, lj->device_jump_tol()), "cv2_set_jump_tol");
   Neighborlist &nl = cv->neighborlist;
   s.check(cv2_set_neighbor(s.ctx, nl.cutoff, nl.cutoff_extra_coef,
                            nl.rebuild_test ? CV2_NL_REBUILD_EVERY_STEP : 0u),
@@ -398,7 +399,6 @@ void Lj::set_cutoff(double v) {
 void Lj::set_jump_tol(double v) {
   jump_fix = true;
   jump_tol = v;
-  caviar_->log.warning("Lj::set_jump_tol: jump_fix is not applied on the device path");
 }
 void Lj::ignore_intra_molecule_flag(bool f) {
   ignore_intra_molecule = f;

@@ -77,6 +77,13 @@ int cv2_set_box(cv2_ctx *ctx, const double lo[3], const double hi[3], const int 
 int cv2_set_lj(cv2_ctx *ctx, int ntypes, const double *eps, const double *sigma,
                const double *cutoff_list, double cutoff, const double *mass);
 
+/* jump_fix.  force_field::Lj::set_jump_tol (lj.hpp; applied at
+ * lj.cpp:459-468): a listed pair closer than sqrt(jump_tol) takes the force,
+ * virial and energy of r^2 = jump_tol along its own direction, which caps the
+ * force of overlapping atoms (a relaxation from random positions).
+ * jump_tol <= 0 turns it off.  Kept across cv2_set_lj. */
+int cv2_set_jump_tol(cv2_ctx *ctx, double jump_tol);
+
 /* Verlet-list parameters.  Neighborlist::set_cutoff /
  * set_cutoff_extra_coef / set_rebuild_test (neighborlist.hpp:46-54,
  * neighborlist.cpp:202-234).  The list holds pairs closer than

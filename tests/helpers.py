@@ -38,7 +38,8 @@ def oracle_forces(pos, s):
     offs, nbr = O.nlist_cell(p, gpos, box, s["nl_cutoff"], s["nl_coef"])
     t = np.asarray(s["type"], np.int32)
     acc, vir, pe = O.lj(p, t, np.arange(len(p)), gpos, t[owner], owner, offs, nbr, s["eps"],
-                        s["sigma"], s["lj_cutoff"], s["mass"], s.get("cutoff_list"))
+                        s["sigma"], s["lj_cutoff"], s["mass"], s.get("cutoff_list"),
+                        s.get("jump_tol", 0.0))
     return p, acc, vir, pe
 
 

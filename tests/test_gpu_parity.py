@@ -279,3 +279,32 @@ def test_host_halo_matches_periodic():
     _, a_ref, vir, pe = oracle_forces(s["pos"], s)
     assert np.abs(acc - a_ref).max() < 1e-12 * max(np.abs(a_ref).max(), 1.0)
     assert abs(th["virial"] - vir) < 1e-12 * abs(vir)
+
+
+@pytest.mark.parametrize("prec", [D.CV2_PREC_FP64, D.CV2_PREC_FP32_FIXED])
+@pytest.mark.parametrize("mix", [False, True])
+def test_jump_fix_forces(prec, mix):
+    """jump_fix on overlapping random positions (a relaxation start): forces
+    and virial against the oracle with the same clamp, one species (the
+    dispatch then takes the species-table kernels) and the KA mixture."""
+    rng = np.random.default_rng(41)
+    if mix:
+        s = ka_mixture(3000, seed=42)
+    else:
+        s = lj_fluid(8, seed=43)
+    L = s["hi"] - s["lo"]
+    s = dict(s, pos=rng.uniform(0.0, 1.0, s["pos"].shape) * L + s["lo"], jump_tol=0.64)
+    _, a_ref, vir, _pe = oracle_forces(s["pos"], s)
+    with D.Context.from_sysdef(s, precision=prec) as c:
+        c.set_jump_tol(s["jump_tol"])
+        c.compute_forces()
+        acc = c.download()[2]
+        th = c.thermo()
+    emax, erms = force_error(acc, a_ref)
+    print(f"[jump] mix={mix} prec={prec}: max {emax:.2e} rms {erms:.2e}, |a| max {np.abs(a_ref).max():.1f}")
+    if prec == D.CV2_PREC_FP64:
+        assert emax < FP64_TOL
+        assert abs(th["virial"] - vir) < 1e-10 * abs(vir)
+    else:
+        assert emax < FP32_TOL_MAX and erms < FP32_TOL_RMS
+        assert abs(th["virial"] - vir) < 1e-5 * abs(vir)

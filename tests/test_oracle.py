@@ -241,3 +241,20 @@ def test_halo_pack_known_answer_two_domains():
     assert np.array_equal(xy, [[0.2, 9.7 - 10.0, 5.0]])  # +y wraps (one process along y)
     _i, _t, xyz, _v = pk[D(2, 0, 2)]
     assert np.allclose(xyz, [[4.9, 10.1, -0.1]]) and xyz[0, 1] == 0.1 + 10.0 and xyz[0, 2] == 9.9 - 10.0
+
+
+def test_jump_fix_dimer_closed_form():
+    """jump_fix (lj.cpp:459-468): a pair closer than sqrt(jump_tol) takes the
+    force coefficient of r^2 = jump_tol, along its own separation; a pair
+    beyond it is untouched."""
+    eps, sig, rc, jt = 1.0, 1.0, 2.5, 0.81
+    for r, clamped in ((0.7, True), (0.95, False)):
+        pos = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+        acc, vir, pe = O.lj(pos, [0, 0], [0, 1], None, None, None, [0, 1, 1], [1], [[eps]], [[sig]], rc,
+                            [1.0], None, jt)
+        r2 = jt if clamped else r * r
+        c2 = rc * rc
+        coef = 4 * eps * (-12 * (sig ** 2 / r2) ** 6 / r2 + 6 * (sig ** 2 / r2) ** 3 / r2
+                          + 12 * (sig ** 2 / c2) ** 6 / c2 - 6 * (sig ** 2 / c2) ** 3 / c2)
+        assert np.isclose(acc[0, 0], coef * r, rtol=1e-13) and np.isclose(acc[1, 0], -coef * r, rtol=1e-13)
+        assert np.isclose(vir, -coef * r2, rtol=1e-13)
