@@ -504,8 +504,12 @@ void Md_simulator::langevin_kbt(double v) {
 void Md_simulator::set_integrator_type(Integrator_t t) {
   integrator_type = t;
   if (t == Integrator_t::Unknown) caviar_->log.error_all("invalid or non-implemented integrator");
-  if (t != Integrator_t::Velocity_verlet && t != Integrator_t::Velocity_verlet_langevin)
-    caviar_->log.error_all("only Velocity_verlet and Velocity_verlet_langevin run on the device path");
+  // Leap_frog (md_simulator.cpp:871-980) is the kick-drift-kick form
+  // v += a dt/2; x += v dt; forces; v += a dt/2 -- the form the device's
+  // velocity Verlet runs -- so both take cv2_run_vv.
+  if (t != Integrator_t::Velocity_verlet && t != Integrator_t::Leap_frog &&
+      t != Integrator_t::Velocity_verlet_langevin)
+    caviar_->log.error_all("only Velocity_verlet, Leap_frog and Velocity_verlet_langevin run on the device path");
 }
 
 void Md_simulator::verify_settings() {
