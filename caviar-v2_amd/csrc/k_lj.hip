@@ -353,7 +353,7 @@ __device__ inline int slot_to_j(const int4 *pc, int sl) {
 // one FP64 FMA.  fcmax tracks the largest |forceCoef'| of a pair within the
 // cutoff: |fc' d| <= |fc'| rc on every axis, so fcmax * rc < 2^51 keeps each
 // product in the range of the rounding FMA (checked once per atom).
-template <bool ISO>
+template <bool ISO, bool SELF = false>
 __device__ inline void pair_f32_flat(const int4 qi, const int4 qj, double u0, double u1, double u2,
                                      const PairF32 &c, unsigned long long &sx,
                                      unsigned long long &sy, unsigned long long &sz,
@@ -371,7 +371,8 @@ __device__ inline void pair_f32_flat(const int4 qi, const int4 qj, double u0, do
   const float s6 = (float)(s2 * s2 * s2);
   const float r2if = (float)r2i;
   const float fc = fmaf(s6 * fmaf(-c.e12, s6, c.e6), r2if, c.fsh);
-  float fsel = r2 <= c.rc2 ? fc : 0.0f;
+  // SELF: the list pads with the atom itself (r^2 = 0), which adds nothing
+  float fsel = (r2 <= c.rc2 && (!SELF || r2 > 0.0)) ? fc : 0.0f;
   __asm__ volatile("" : "+v"(fsel));  // select in FP32, before the conversion
   fcmax = fmaxf(fcmax, fabsf(fsel));
   const double fcs = (double)fsel;
@@ -651,15 +652,35 @@ __global__ void __launch_bounds__(BLOCK)
   const PairF32 c = pair_f32(lj, 0, ISO ? u0 : 1.0);
   const int cp = (nnbr[i] + 3) & ~3;
   AccF32 a;
-  for (int k = 0; k < cp; k += 4) {
-    const u32x4 g = load_group(nbr, (size_t)stride, i, k);
-    const uint32_t ent[4] = {g.x, g.y, g.z, g.w};
-    int4 qj[4];
+  if (ENERGY) {
+    for (int k = 0; k < cp; k += 4) {
+      const u32x4 g = load_group(nbr, (size_t)stride, i, k);
+      const uint32_t ent[4] = {g.x, g.y, g.z, g.w};
+      int4 qj[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) qj[v] = q[ent[v] & kIndexMask];
+      for (int v = 0; v < 4; ++v) qj[v] = q[ent[v] & kIndexMask];
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-      pair_f32_entry<ISO, ENERGY>(qi, qj[v], u0, u1, u2, c, i < (int)(ent[v] & kIndexMask), a);
+      for (int v = 0; v < 4; ++v)
+        pair_f32_entry<ISO, ENERGY>(qi, qj[v], u0, u1, u2, c, i < (int)(ent[v] & kIndexMask), a);
+    }
+  } else if (cp > 0) {
+    // branchless entries (every entry adds one rounding bias), the next
+    // group's entries loaded while this one runs
+    float fcmax = 0.0f;
+    u32x4 gn = load_group(nbr, (size_t)stride, i, 0);
+    for (int k = 0; k < cp; k += 4) {
+      const u32x4 g = gn;
+      gn = load_group(nbr, (size_t)stride, i, min(k + 4, cp - 4));
+      __builtin_amdgcn_sched_barrier(0);
+      const uint32_t ent[4] = {g.x, g.y, g.z, g.w};
+      int4 qj[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) qj[v] = q[ent[v] & kIndexMask];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) pair_f32_flat<ISO, true>(qi, qj[v], u0, u1, u2, c, a.sx, a.sy, a.sz, fcmax);
+    }
+    a.terms = cp;
+    a.ovf = !((double)fcmax * sqrt(c.rc2) < 0.9 * 2251799813685248.0);
   }
   const double mi = lj->mass_inv[0] * (1.0 / kForceScale);
   acc[i] = fx_value(a.sx, a.terms) * mi;
