@@ -114,7 +114,7 @@ void drop_graph(cv2_ens *ens) {
 void free_all(cv2_ens *ens) {
   drop_graph(ens);
   DevState &s = ens->s;
-  void *ptrs[] = {s.pos, s.pos_alt, s.pos_ref, s.q, s.vel, s.vel_alt, s.acc, s.acc_alt, s.tag,
+  void *ptrs[] = {s.pos, s.pos_alt, s.pos_ref, s.q, s.q_alt, s.vel, s.vel_alt, s.acc, s.acc_alt, s.tag,
                   s.tag_alt, s.cell, s.slot, s.perm, s.order, s.cell_count, s.cell_start,
                   s.tile_sum, s.nbr, s.nnbr, s.flags, s.wstat};
   for (void *p : ptrs)
@@ -231,6 +231,17 @@ int load_state(cv2_ens *ens, const double *pos, const double *vel) {
   return prime(ens);
 }
 
+// One step that closes the previous step and opens the next: the list build
+// (whose first kernel takes over the coordinates the previous force pass
+// wrote to q_alt) and the force pass with the integrator in its epilogue.
+void middle_step(cv2_ens *ens, const EnsDev &E, double dt) {
+  const cv2_ens_tuning &t = ens->tune;
+  launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->per_atom,
+                   ens->stream, ens->s.q_alt);
+  launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream,
+                    dt, ens->thresh_sq);
+}
+
 int ensure_graph(cv2_ens *ens, double dt) {
   const cv2_ens_tuning &t = ens->tune;
   if (ens->gexec && ens->graph_dt == dt && ens->graph_tune.force_block == t.force_block &&
@@ -239,10 +250,7 @@ int ensure_graph(cv2_ens *ens, double dt) {
   drop_graph(ens);
   const EnsDev E = ens->ens();
   EHIP(hipStreamBeginCapture(ens->stream, hipStreamCaptureModeThreadLocal));
-  launch_update_ens(ens->s, E, true, true, dt, ens->thresh_sq, ens->stream);
-  launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->per_atom,
-                   ens->stream);
-  launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream);
+  middle_step(ens, E, dt);
   EHIP(hipStreamEndCapture(ens->stream, &ens->graph));
   EHIP(hipGraphInstantiate(&ens->gexec, ens->graph, nullptr, nullptr, 0));
   EHIP(hipGraphGetNodes(ens->graph, nullptr, &ens->gnodes));
@@ -261,26 +269,25 @@ int run(cv2_ens *ens, int64_t nsteps, double dt) {
   }
   const EnsDev E = ens->ens();
   const cv2_ens_tuning &t = ens->tune;
-  // step 1 opens eagerly; steps 2..n replay the captured step; the final
-  // half kick closes the last step
+  // Step 1 opens eagerly (its coordinates also into q_alt, which the first
+  // build takes over); steps 1..n-1 are the captured middle step (build,
+  // forces with the next step's integrator pass); step n builds and computes
+  // forces alone, and the final half kick closes it.
   launch_update_ens(ens->s, E, false, true, dt, ens->thresh_sq, ens->stream);
-  launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->per_atom,
-                   ens->stream);
-  launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream);
+  EHIP(hipMemcpyAsync(ens->s.q_alt, ens->s.q, sizeof(int4) * ens->s.n, hipMemcpyDeviceToDevice,
+                      ens->stream));
   if (nsteps > 1) {
     if (ens->use_graph) {
       int rc = ensure_graph(ens, dt);
       if (rc) return rc;
       for (int64_t k = 1; k < nsteps; ++k) EHIP(hipGraphLaunch(ens->gexec, ens->stream));
     } else {
-      for (int64_t k = 1; k < nsteps; ++k) {
-        launch_update_ens(ens->s, E, true, true, dt, ens->thresh_sq, ens->stream);
-        launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0,
-                         ens->per_atom, ens->stream);
-        launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream);
-      }
+      for (int64_t k = 1; k < nsteps; ++k) middle_step(ens, E, dt);
     }
   }
+  launch_build_ens(ens->s, E, ens->ncell, t.build_block, t.rebuild_all != 0, ens->per_atom,
+                   ens->stream, ens->s.q_alt);
+  launch_forces_ens(ens->s, E, ens->lj_dev, t.force_block, ens->iso, false, ens->eatom, ens->stream);
   launch_update_ens(ens->s, E, true, false, dt, ens->thresh_sq, ens->stream);
   ens->steps += nsteps;
   return check_flags(ens);
@@ -410,6 +417,7 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
   EHIP(ralloc(&st.pos_alt, n));
   EHIP(ralloc(&st.pos_ref, n));
   EHIP(ralloc(&st.q, n));
+  EHIP(ralloc(&st.q_alt, n));
   EHIP(ralloc(&st.vel, 3 * n));
   EHIP(ralloc(&st.vel_alt, 3 * n));
   EHIP(ralloc(&st.acc, 3 * n));

@@ -349,6 +349,40 @@ struct EnsDev {
   int *srebuilds;       // per system: builds done
 };
 
+// One ensemble atom's velocity-Verlet pass (k_ens_update, and the epilogue of
+// the fused ensemble force kernel): FINISH closes step s with a (v += a dt/2),
+// START opens step s+1 (v += a dt/2, x += v dt, fixed-point coordinates into
+// q_out with the atom's own system unit, displacement test raising that
+// system's flag and F_REBUILD).  Both forms run this one function, so they
+// give the same bits.
+template <bool FINISH, bool START>
+__device__ inline void ens_step_atom(int i, int n, const EnsDev &e, double4 *__restrict__ pos,
+                                     const double4 *__restrict__ pos_ref, double *__restrict__ vel,
+                                     const double a[3], int4 *__restrict__ q_out, double dt,
+                                     double thresh_sq, int *__restrict__ flags) {
+  double v[3] = {vel[i], vel[n + i], vel[2 * n + i]};
+  StepParams p{};
+  p.dt = dt;
+  double4 x = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (START) x = pos[i];
+  integrate_values<false, FINISH, START, true>(x, v, 0, a, p);
+  if (START) {
+    const int sy = e.sys[i];
+    const BoxDev &b = e.box[sy];
+    pos[i] = x;
+    q_out[i] = pack_q(x.x, x.y, x.z, (int)x.w, b.lo, b.qscale);
+    const double4 r = pos_ref[i];
+    const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
+    if (dx * dx + dy * dy + dz * dz > thresh_sq) {
+      atomicOr(&e.sflag[sy], 1);
+      atomicOr(&flags[F_REBUILD], 1);
+    }
+  }
+  vel[i] = v[0];
+  vel[n + i] = v[1];
+  vel[2 * n + i] = v[2];
+}
+
 // Host: LJ pair tables (false + message on invalid input).
 bool build_lj_table(LJDev &L, int ntypes, const double *eps, const double *sigma,
                     const double *cutoff_list, double cutoff, const double *mass, std::string &err);
@@ -391,9 +425,11 @@ void launch_set_flag(int *flags, int which, int value, hipStream_t st);
 // fixed-point virial and energy when energy is set), velocity Verlet with
 // the per-system rebuild test, per-system (ke, pe, virial).
 void launch_build_ens(DevState &s, const EnsDev &e, int ncell, int block, bool policy_all,
-                      bool per_atom, hipStream_t st);
+                      bool per_atom, hipStream_t st, const int4 *q_from = nullptr);
+// fuse_dt > 0: the integrator pass of the next step runs in the force
+// kernel's epilogue (new coordinates into s.q_alt)
 void launch_forces_ens(DevState &s, const EnsDev &e, const LJDev *lj_dev, int block, bool iso, bool energy,
-                       long long *eatom, hipStream_t st);
+                       long long *eatom, hipStream_t st, double fuse_dt = 0.0, double thresh_sq = 0.0);
 void launch_update_ens(DevState &s, const EnsDev &e, bool finish, bool start, double dt,
                        double thresh_sq, hipStream_t st);
 void launch_thermo_ens(const DevState &s, const EnsDev &e, const LJDev *lj_dev,

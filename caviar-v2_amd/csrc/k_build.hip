@@ -1149,11 +1149,16 @@ __global__ void k_ens_begin(EnsDev e, int *__restrict__ cell_count, int ncell,
 
 // Wrap and bin the atoms of flagged systems; runs of equal cells in
 // consecutive lanes take their slots with one atomic add.
+// q_from (when given): the coordinates the fused force pass of the previous
+// step wrote; they become q here, for every atom and before the early exit
+// (the chain and the next force pass read q).
 __global__ void k_ens_wrap_bin(double4 *__restrict__ pos, int n, EnsDev e, int *__restrict__ cell,
                                int *__restrict__ slot, int *__restrict__ cell_count,
-                               const int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
+                               const int *__restrict__ flags, int4 *__restrict__ q,
+                               const int4 *__restrict__ q_from) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q_from && i < n) q[i] = q_from[i];
+  if (flags[F_REBUILD] == 0) return;
   const int lane = threadIdx.x & 63;
   int ci = -1;
   const int sy = i < n ? e.sys[i] : 0;
@@ -1363,12 +1368,13 @@ void launch_fill_int(int *p, int n, int value, hipStream_t st) {
 }
 
 void launch_build_ens(DevState &s, const EnsDev &e, int ncell, int block, bool policy_all,
-                      bool per_atom, hipStream_t st) {
+                      bool per_atom, hipStream_t st, const int4 *q_from) {
   const int n = s.n;
   if (n == 0) return;
   if (policy_all) k_ens_policy<<<nblk(e.nsys, 256), 256, 0, st>>>(e, s.flags);
   k_ens_begin<<<nblk(ncell, 256), 256, 0, st>>>(e, s.cell_count, ncell, s.flags);
-  k_ens_wrap_bin<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, e, s.cell, s.slot, s.cell_count, s.flags);
+  k_ens_wrap_bin<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, e, s.cell, s.slot, s.cell_count, s.flags,
+                                                s.q, q_from);
   launch_scan(s.cell_count, s.cell_start, s.tile_sum, ncell, s.flags, st);
   k_ens_scatter<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.slot, s.cell_start, s.perm, n, e, s.flags);
   k_ens_rank<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.cell_start, s.perm, s.tag, s.order, n, e,

@@ -118,29 +118,8 @@ __global__ void __launch_bounds__(kBlock)
                  int n, EnsDev e, double dt, double thresh_sq, int *__restrict__ flags) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  double v[3] = {vel[i], vel[n + i], vel[2 * n + i]};
   const double a[3] = {acc[i], acc[n + i], acc[2 * n + i]};
-  StepParams p{};
-  p.dt = dt;
-  double4 x = make_double4(0.0, 0.0, 0.0, 0.0);
-  if (START) x = pos[i];
-  // the same arithmetic as the single-box integrator (velocity Verlet)
-  integrate_values<false, FINISH, START, true>(x, v, 0, a, p);
-  if (START) {
-    const int sy = e.sys[i];
-    const BoxDev &b = e.box[sy];
-    pos[i] = x;
-    q[i] = pack_q(x.x, x.y, x.z, (int)x.w, b.lo, b.qscale);
-    const double4 r = pos_ref[i];
-    const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
-    if (dx * dx + dy * dy + dz * dz > thresh_sq) {
-      atomicOr(&e.sflag[sy], 1);
-      atomicOr(&flags[F_REBUILD], 1);
-    }
-  }
-  vel[i] = v[0];
-  vel[n + i] = v[1];
-  vel[2 * n + i] = v[2];
+  ens_step_atom<FINISH, START>(i, n, e, pos, pos_ref, vel, a, q, dt, thresh_sq, flags);
 }
 
 // Per-system (kinetic energy, potential energy, virial): one block per

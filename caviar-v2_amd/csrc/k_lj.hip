@@ -638,12 +638,16 @@ __global__ void __launch_bounds__(1024) k_reduce_i64(const long long *__restrict
 // fixed-point unit of its own system (ISO: every system has one unit on all
 // axes); ENERGY writes per-atom fixed-point virial and energy (each pair
 // once: i < j) for the per-system sums.
-template <int BLOCK, bool ENERGY, bool ISO>
+// FUSE: the epilogue runs the integrator pass of the next step
+// (ens_step_atom) on the accelerations just summed, writing the next
+// fixed-point coordinates to q_out (other threads still read q).
+template <int BLOCK, bool ENERGY, bool ISO, bool FUSE>
 __global__ void __launch_bounds__(BLOCK)
     k_lj_ens(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
              const int *__restrict__ nnbr, int stride, int n, EnsDev e,
              const LJDev *__restrict__ lj, double *__restrict__ acc, long long *__restrict__ eatom,
-             int *__restrict__ flags) {
+             int *__restrict__ flags, double4 *__restrict__ pos, const double4 *__restrict__ pos_ref,
+             double *__restrict__ vel, int4 *__restrict__ q_out, double dt, double thresh_sq) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
   const BoxDev &b = e.box[e.sys[i]];
@@ -683,9 +687,15 @@ __global__ void __launch_bounds__(BLOCK)
     a.ovf = !((double)fcmax * sqrt(c.rc2) < 0.9 * 2251799813685248.0);
   }
   const double mi = lj->mass_inv[0] * (1.0 / kForceScale);
-  acc[i] = fx_value(a.sx, a.terms) * mi;
-  acc[n + i] = fx_value(a.sy, a.terms) * mi;
-  acc[2 * n + i] = fx_value(a.sz, a.terms) * mi;
+  const double ai[3] = {fx_value(a.sx, a.terms) * mi, fx_value(a.sy, a.terms) * mi,
+                        fx_value(a.sz, a.terms) * mi};
+  if (FUSE) {
+    ens_step_atom<true, true>(i, n, e, pos, pos_ref, vel, ai, q_out, dt, thresh_sq, flags);
+  } else {
+    acc[i] = ai[0];
+    acc[n + i] = ai[1];
+    acc[2 * n + i] = ai[2];
+  }
   if (a.ovf) atomicOr(&flags[F_FXOVF], 1);
   if (ENERGY) {
     eatom[2 * (size_t)i] = a.vir;
@@ -751,25 +761,32 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
 
 template <int B, bool ISO>
 void lj_ens_dispatch(DevState &s, const EnsDev &e, const LJDev *lj_dev, bool energy,
-                     long long *eatom, hipStream_t st) {
+                     long long *eatom, double fuse_dt, double thresh_sq, hipStream_t st) {
   const int g = (s.n + B - 1) / B;
   if (energy)
-    k_lj_ens<B, true, ISO><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev, s.acc,
-                                            eatom, s.flags);
+    k_lj_ens<B, true, ISO, false><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev,
+                                                   s.acc, eatom, s.flags, s.pos, s.pos_ref, s.vel,
+                                                   s.q_alt, 0.0, 0.0);
+  else if (fuse_dt > 0.0)
+    k_lj_ens<B, false, ISO, true><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev,
+                                                   s.acc, eatom, s.flags, s.pos, s.pos_ref, s.vel,
+                                                   s.q_alt, fuse_dt, thresh_sq);
   else
-    k_lj_ens<B, false, ISO><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev,
-                                             s.acc, eatom, s.flags);
+    k_lj_ens<B, false, ISO, false><<<g, B, 0, st>>>(s.q, s.nbr, s.nnbr, s.stride, s.n, e, lj_dev,
+                                                    s.acc, eatom, s.flags, s.pos, s.pos_ref, s.vel,
+                                                    s.q_alt, 0.0, 0.0);
 }
 
 void launch_forces_ens(DevState &s, const EnsDev &e, const LJDev *lj_dev, int block, bool iso,
-                       bool energy, long long *eatom, hipStream_t st) {
+                       bool energy, long long *eatom, hipStream_t st, double fuse_dt,
+                       double thresh_sq) {
   if (s.n == 0) return;
 #define CV2_LJE(B)                                                    \
   if (block == B) {                                                   \
     if (iso)                                                          \
-      lj_ens_dispatch<B, true>(s, e, lj_dev, energy, eatom, st);      \
+      lj_ens_dispatch<B, true>(s, e, lj_dev, energy, eatom, fuse_dt, thresh_sq, st);      \
     else                                                              \
-      lj_ens_dispatch<B, false>(s, e, lj_dev, energy, eatom, st);     \
+      lj_ens_dispatch<B, false>(s, e, lj_dev, energy, eatom, fuse_dt, thresh_sq, st);     \
     return;                                                           \
   }
   CV2_LJE(64)
