@@ -209,6 +209,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-steps", type=int, default=2000)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the batched-ensemble workload and print its JSON (for profiling)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only the force kernel and a calibration copy (for rocprofv3 --pmc)")
     a = ap.parse_args()
@@ -219,6 +221,10 @@ def main():
 
     if a.traffic_probe:
         traffic_probe(a)
+        return
+    if a.ensemble_only:
+        r = run_ensemble(a.ens_steps)
+        print(json.dumps({k: r[k] for k in ("ms_per_step", "value", "tuning")}), flush=True)
         return
     s = workload(a.ncell)
     n = len(s["pos"])
