@@ -45,6 +45,7 @@ struct cv2_ens {
   int *d_sys = nullptr, *d_cell_sys = nullptr, *d_cell_off = nullptr, *d_atom_off = nullptr;
   BoxDev *d_box = nullptr;
   int *d_sflag = nullptr, *d_srebuilds = nullptr;
+  int *d_act = nullptr;  // active-system list of a build (EnsDev::act)
   long long *eatom = nullptr;
   double *d_thermo = nullptr;
   int *h_flags = nullptr;
@@ -69,6 +70,7 @@ struct cv2_ens {
     e.box = d_box;
     e.sflag = d_sflag;
     e.srebuilds = d_srebuilds;
+    e.act = d_act;
     return e;
   }
 };
@@ -127,6 +129,7 @@ void free_all(cv2_ens *ens) {
   rfree(ens->d_box);
   rfree(ens->d_sflag);
   rfree(ens->d_srebuilds);
+  rfree(ens->d_act);
   rfree(ens->eatom);
   rfree(ens->d_thermo);
   ens->atoms = false;
@@ -444,6 +447,7 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
   EHIP(ralloc(&ens->d_box, nsys));
   EHIP(ralloc(&ens->d_sflag, nsys));
   EHIP(ralloc(&ens->d_srebuilds, nsys));
+  EHIP(ralloc(&ens->d_act, 2 * nsys + 4));
   EHIP(ralloc(&ens->eatom, 2 * n));
   EHIP(ralloc(&ens->d_thermo, 3 * nsys));
   EHIP(hipMemcpyAsync(ens->d_sys, sys.data(), sizeof(int) * n, hipMemcpyHostToDevice, ens->stream));

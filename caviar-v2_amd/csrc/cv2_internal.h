@@ -347,6 +347,10 @@ struct EnsDev {
   const BoxDev *box;    // per system: box, cell grid, fixed-point scales
   int *sflag;           // per system: rebuild requested
   int *srebuilds;       // per system: builds done
+  // the flagged systems of a build (k_ens_begin): act[0] = their number m,
+  // act[1] = their atoms, act[2 .. 2+m] = atom prefix, act[3+nsys ..] their
+  // first atoms; the per-atom build kernels walk only these atoms
+  int *act;
 };
 
 // One ensemble atom's velocity-Verlet pass (k_ens_update, and the epilogue of

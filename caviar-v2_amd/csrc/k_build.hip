@@ -1140,11 +1140,94 @@ __global__ void k_ens_policy(EnsDev e, const int *__restrict__ flags) {
     e.sflag[s] = 1;
 }
 
-__global__ void k_ens_begin(EnsDev e, int *__restrict__ cell_count, int ncell,
-                            const int *__restrict__ flags) {
+// Clear the flagged systems' cell counts; the last work-group also lists
+// the flagged systems (EnsDev::act) in system order.
+constexpr int kEnsBegin = 1024;
+__global__ void __launch_bounds__(kEnsBegin)
+    k_ens_begin(EnsDev e, int *__restrict__ cell_count, int ncell, const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < ncell && e.sflag[e.cell_sys[c]]) cell_count[c] = 0;
+  if ((int)blockIdx.x < (int)gridDim.x - 1) {
+    const int c = blockIdx.x * kEnsBegin + threadIdx.x;
+    if (c < ncell && e.sflag[e.cell_sys[c]]) cell_count[c] = 0;
+    return;
+  }
+  __shared__ int wsum[2][kEnsBegin / 64];
+  __shared__ int carry[2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x < 2) carry[threadIdx.x] = 0;
+  __syncthreads();
+  int *pre = e.act + 2, *first = e.act + 3 + e.nsys;
+  for (int s0 = 0; s0 < e.nsys; s0 += kEnsBegin) {
+    const int s = s0 + (int)threadIdx.x;
+    const bool on = s < e.nsys && e.sflag[s] != 0;
+    const int na = on ? e.atom_off[s + 1] - e.atom_off[s] : 0;
+    int x = on ? 1 : 0, y = na;  // inclusive scans: flagged systems, their atoms
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int xv = __shfl_up(x, d, 64), yv = __shfl_up(y, d, 64);
+      x += lane >= d ? xv : 0;
+      y += lane >= d ? yv : 0;
+    }
+    if (lane == 63) {
+      wsum[0][w] = x;
+      wsum[1][w] = y;
+    }
+    __syncthreads();
+    int bx = carry[0], by = carry[1];
+    for (int k = 0; k < w; ++k) {
+      bx += wsum[0][k];
+      by += wsum[1][k];
+    }
+    if (on) {  // the (bx + x - 1)-th flagged system
+      pre[bx + x - 1] = by + y - na;
+      first[bx + x - 1] = e.atom_off[s];
+    }
+    __syncthreads();
+    if (threadIdx.x == kEnsBegin - 1) {
+      carry[0] = bx + x;
+      carry[1] = by + y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    e.act[0] = carry[0];
+    e.act[1] = carry[1];
+    pre[carry[0]] = carry[1];
+  }
+}
+
+// Atom t (t < total) of the flagged systems of a build: a binary search of
+// the active list's atom prefix, staged in LDS when it fits.
+constexpr int kActLds = 1024;
+struct ActMap {
+  int m, total;
+  const int *pre, *first;
+  __device__ int atom(int t) const {
+    int lo = 0, hi = m - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pre[mid] <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    return first[lo] + (t - pre[lo]);
+  }
+};
+
+// Every thread of the block calls this (it synchronizes the block).
+__device__ inline ActMap act_map(const EnsDev &e, int *sh) {
+  ActMap a;
+  a.m = e.act[0];
+  a.total = e.act[1];
+  a.first = e.act + 3 + e.nsys;
+  a.pre = e.act + 2;
+  if (a.m + 1 <= kActLds) {
+    for (int k = threadIdx.x; k <= a.m; k += blockDim.x) sh[k] = a.pre[k];
+    __syncthreads();
+    a.pre = sh;
+  }
+  return a;
 }
 
 // Wrap and bin the atoms of flagged systems; runs of equal cells in
@@ -1156,13 +1239,20 @@ __global__ void k_ens_wrap_bin(double4 *__restrict__ pos, int n, EnsDev e, int *
                                int *__restrict__ slot, int *__restrict__ cell_count,
                                const int *__restrict__ flags, int4 *__restrict__ q,
                                const int4 *__restrict__ q_from) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q_from && i < n) q[i] = q_from[i];
+  const int gsz = gridDim.x * blockDim.x;
+  if (q_from)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gsz) q[i] = q_from[i];
   if (flags[F_REBUILD] == 0) return;
+  __shared__ int act_sh[kActLds + 1];
+  const ActMap am = act_map(e, act_sh);
+  // block-uniform loop: the lanes of a wavefront stay together for the ballots
+  for (int t0 = blockIdx.x * blockDim.x; t0 < am.total; t0 += gsz) {
+  const int t = t0 + (int)threadIdx.x;
   const int lane = threadIdx.x & 63;
   int ci = -1;
-  const int sy = i < n ? e.sys[i] : 0;
-  const bool valid = i < n && e.sflag[sy] != 0;
+  const bool valid = t < am.total;
+  const int i = valid ? am.atom(t) : 0;
+  const int sy = e.sys[i];
   if (valid) {
     const BoxDev &b = e.box[sy];
     double4 p = pos[i];
@@ -1192,29 +1282,36 @@ __global__ void k_ens_wrap_bin(double4 *__restrict__ pos, int n, EnsDev e, int *
   }
   base = __shfl(base, leader, 64);
   if (valid) slot[i] = base + lane - leader;
+  }
 }
 
 __global__ void k_ens_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
                               const int *__restrict__ start, int *__restrict__ perm, int n, EnsDev e,
                               const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !e.sflag[e.sys[i]]) return;
-  perm[start[cell[i]] + slot[i]] = i;
+  __shared__ int act_sh[kActLds + 1];
+  const ActMap am = act_map(e, act_sh);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < am.total; t += gridDim.x * blockDim.x) {
+    const int i = am.atom(t);
+    perm[start[cell[i]] + slot[i]] = i;
+  }
 }
 
 __global__ void k_ens_rank(const int *__restrict__ cell, const int *__restrict__ start,
                            const int *__restrict__ perm, const int *__restrict__ tag,
                            int *__restrict__ order, int n, EnsDev e, const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !e.sflag[e.sys[i]]) return;
-  const int c = cell[i];
-  const int a = start[c], end = start[c + 1];
-  const int t = tag[i];
-  int rank = 0;
-  for (int k = a; k < end; ++k) rank += tag[perm[k]] < t;
-  order[a + rank] = i;
+  __shared__ int act_sh[kActLds + 1];
+  const ActMap am = act_map(e, act_sh);
+  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < am.total; u += gridDim.x * blockDim.x) {
+    const int i = am.atom(u);
+    const int c = cell[i];
+    const int a = start[c], end = start[c + 1];
+    const int t = tag[i];
+    int rank = 0;
+    for (int k = a; k < end; ++k) rank += tag[perm[k]] < t;
+    order[a + rank] = i;
+  }
 }
 
 // Gather the flagged systems' atoms into their new order (k and order[k]
@@ -1226,18 +1323,21 @@ __global__ void k_ens_permute(const int *__restrict__ order, int n, EnsDev e,
                               double *__restrict__ acc_out, const int *__restrict__ tag,
                               int *__restrict__ tag_out, const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n || !e.sflag[e.sys[k]]) return;
-  const int o = order[k];
-  const double4 p = pos[o];
-  pos_out[k] = p;
-  pos_ref[k] = p;
+  __shared__ int act_sh[kActLds + 1];
+  const ActMap am = act_map(e, act_sh);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < am.total; t += gridDim.x * blockDim.x) {
+    const int k = am.atom(t);
+    const int o = order[k];
+    const double4 p = pos[o];
+    pos_out[k] = p;
+    pos_ref[k] = p;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    vel_out[d * n + k] = vel[d * n + o];
-    acc_out[d * n + k] = acc[d * n + o];
+    for (int d = 0; d < 3; ++d) {
+      vel_out[d * n + k] = vel[d * n + o];
+      acc_out[d * n + k] = acc[d * n + o];
+    }
+    tag_out[k] = tag[o];
   }
-  tag_out[k] = tag[o];
 }
 
 __global__ void k_ens_copy_back(int n, EnsDev e, const double4 *__restrict__ pos_alt,
@@ -1247,19 +1347,20 @@ __global__ void k_ens_copy_back(int n, EnsDev e, const double4 *__restrict__ pos
                                 int *__restrict__ tag, int4 *__restrict__ q,
                                 const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int sy = e.sys[k];
-  if (!e.sflag[sy]) return;
-  const double4 p = pos_alt[k];
-  pos[k] = p;
-  q[k] = qpack(p, e.box[sy]);
+  __shared__ int act_sh[kActLds + 1];
+  const ActMap am = act_map(e, act_sh);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < am.total; t += gridDim.x * blockDim.x) {
+    const int k = am.atom(t);
+    const double4 p = pos_alt[k];
+    pos[k] = p;
+    q[k] = qpack(p, e.box[e.sys[k]]);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    vel[d * n + k] = vel_alt[d * n + k];
-    if (acc) acc[d * n + k] = acc_alt[d * n + k];
+    for (int d = 0; d < 3; ++d) {
+      vel[d * n + k] = vel_alt[d * n + k];
+      if (acc) acc[d * n + k] = acc_alt[d * n + k];
+    }
+    tag[k] = tag_alt[k];
   }
-  tag[k] = tag_alt[k];
 }
 
 // Per-atom Verlet list of the flagged systems (the FP64 predicate of the
@@ -1372,17 +1473,19 @@ void launch_build_ens(DevState &s, const EnsDev &e, int ncell, int block, bool p
   const int n = s.n;
   if (n == 0) return;
   if (policy_all) k_ens_policy<<<nblk(e.nsys, 256), 256, 0, st>>>(e, s.flags);
-  k_ens_begin<<<nblk(ncell, 256), 256, 0, st>>>(e, s.cell_count, ncell, s.flags);
-  k_ens_wrap_bin<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, e, s.cell, s.slot, s.cell_count, s.flags,
+  k_ens_begin<<<nblk(ncell, kEnsBegin) + 1, kEnsBegin, 0, st>>>(e, s.cell_count, ncell, s.flags);
+  // the per-atom kernels walk the flagged systems' atoms in a grid-stride loop
+  const int ga = std::min(nblk(n, 256), 2048);
+  k_ens_wrap_bin<<<ga, 256, 0, st>>>(s.pos, n, e, s.cell, s.slot, s.cell_count, s.flags,
                                                 s.q, q_from);
   launch_scan(s.cell_count, s.cell_start, s.tile_sum, ncell, s.flags, st);
-  k_ens_scatter<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.slot, s.cell_start, s.perm, n, e, s.flags);
-  k_ens_rank<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.cell_start, s.perm, s.tag, s.order, n, e,
+  k_ens_scatter<<<ga, 256, 0, st>>>(s.cell, s.slot, s.cell_start, s.perm, n, e, s.flags);
+  k_ens_rank<<<ga, 256, 0, st>>>(s.cell, s.cell_start, s.perm, s.tag, s.order, n, e,
                                             s.flags);
-  k_ens_permute<<<nblk(n, 256), 256, 0, st>>>(s.order, n, e, s.pos, s.pos_alt, s.pos_ref, s.vel,
+  k_ens_permute<<<ga, 256, 0, st>>>(s.order, n, e, s.pos, s.pos_alt, s.pos_ref, s.vel,
                                                s.vel_alt, s.acc, s.acc_alt, s.tag, s.tag_alt,
                                                s.flags);
-  k_ens_copy_back<<<nblk(n, 256), 256, 0, st>>>(n, e, s.pos_alt, s.pos, s.vel_alt, s.vel,
+  k_ens_copy_back<<<ga, 256, 0, st>>>(n, e, s.pos_alt, s.pos, s.vel_alt, s.vel,
                                                  s.acc_alt, s.acc, s.tag_alt, s.tag, s.q, s.flags);
   // per-atom build only where some system has it (boxes the coarse filter
   // cannot serve)
