@@ -324,6 +324,13 @@ int alloc_positions(cv2_ctx *ctx, int64_t ntot) {
     HIPCHK(hipMemcpyAsync(keep_id.data(), s.id, sizeof(int) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
   }
+  // growing: the previous arrays are released (the owned positions and ids
+  // were saved above; the rest is rewritten before use)
+  for (void **p : {(void **)&s.pos, (void **)&s.pos_ref, (void **)&s.q, (void **)&s.q_alt,
+                   (void **)&s.id, (void **)&s.cell, (void **)&s.slot, (void **)&s.gcell_items}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
   HIPCHK(dalloc(&s.pos, cap));
   HIPCHK(dalloc(&s.pos_ref, cap));
   HIPCHK(dalloc(&s.q, cap));
