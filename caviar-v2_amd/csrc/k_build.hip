@@ -252,7 +252,17 @@ __global__ void k_rank_move(const int *__restrict__ cell, const int *__restrict_
   const int c = cell[i];
   const int a = start[c], e = start[c + 1];
   int rank = 0;
-  for (int k = a; k < e; ++k) rank += perm[k] < i;
+  // eight members per round: their loads are independent, so they are
+  // issued together rather than one round trip each
+  int m = a;
+  for (; m + 8 <= e; m += 8) {
+    int v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = perm[m + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) rank += v[u] < i;
+  }
+  for (; m < e; ++m) rank += perm[m] < i;
   const int k = a + rank;
   const double4 p = pos[i];
   pos_out[k] = p;
@@ -1309,7 +1319,17 @@ __global__ void k_ens_rank(const int *__restrict__ cell, const int *__restrict__
     const int a = start[c], end = start[c + 1];
     const int t = tag[i];
     int rank = 0;
-    for (int k = a; k < end; ++k) rank += tag[perm[k]] < t;
+    int k = a;
+    for (; k + 8 <= end; k += 8) {  // eight members per round (independent loads)
+      int v[8];
+#pragma unroll
+      for (int w = 0; w < 8; ++w) v[w] = perm[k + w];
+#pragma unroll
+      for (int w = 0; w < 8; ++w) v[w] = tag[v[w]];
+#pragma unroll
+      for (int w = 0; w < 8; ++w) rank += v[w] < t;
+    }
+    for (; k < end; ++k) rank += tag[perm[k]] < t;
     order[a + rank] = i;
   }
 }
