@@ -172,6 +172,24 @@ def committed_traffic():
     return None
 
 
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "k_lj_f32_task_lj2m_counters.txt")
+
+
+def committed_counters():
+    """VALU busy and L2 hit rate of the force kernel from the committed
+    rocprofv3 --pmc summary, or Nones."""
+    out = {"valu_busy": None, "l2_hit": None}
+    try:
+        with open(COUNTERS_FILE) as f:
+            for line in f:
+                k = line.split(":", 1)[0]
+                if k in out:
+                    out[k] = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return out
+
+
 def traffic_probe(a):
     """Workload for the rocprofv3 --pmc traffic runs: a few force launches on
     the melted 2M-atom configuration, then a plain copy of known size (the
@@ -283,7 +301,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(),
                      "kernel": "k_lj_f32_task (integrator fused)", "bytes_per_launch": b_force,
-                     "avg_launch_ms": t_force * 1e3},
+                     "avg_launch_ms": t_force * 1e3, **committed_counters()},
         "cpu_baseline": None,
         "ensemble": None,
     }
