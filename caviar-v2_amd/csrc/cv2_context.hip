@@ -1002,6 +1002,17 @@ int cv2_thermo(cv2_ctx *ctx, int n_r_df, double *ke, double *pe, double *virial)
   return CV2_OK;
 }
 
+int cv2_scale_velocities(cv2_ctx *ctx, double lambda) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "no atoms");
+  if (!std::isfinite(lambda)) return fail(ctx, CV2_ERR_BAD_ARG, "velocity scale is not finite");
+  (void)hipSetDevice(ctx->device);
+  launch_scale_vel(ctx->s, lambda, ctx->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return CV2_OK;
+}
+
 int cv2_download(cv2_ctx *ctx, double *pos, double *vel, double *acc) {
   if (!ctx) return CV2_ERR_BAD_ARG;
   if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "no atoms");

@@ -422,6 +422,7 @@ bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, cons
                          const UpdateArgs &u, hipStream_t st);
 void launch_check_disp(DevState &s, const BoxDev &b, hipStream_t st);
 void launch_thermo(DevState &s, const LJDev *lj_dev, hipStream_t st);
+void launch_scale_vel(DevState &s, double lambda, hipStream_t st);
 void launch_set_flag(int *flags, int which, int value, hipStream_t st);
 
 // Ensemble: list build of the flagged systems (policy_all: a request from

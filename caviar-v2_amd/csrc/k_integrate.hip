@@ -19,6 +19,8 @@
 // 100-148) -- any atom displaced by more than half the skin since the last
 // build -- is fused into the position update and raises the device flag
 // that the build kernels test.
+#include <algorithm>
+
 #include "cv2_internal.h"
 
 namespace cv2 {
@@ -218,6 +220,20 @@ void launch_thermo_ens(const DevState &s, const EnsDev &e, const LJDev *lj_dev,
                        const long long *eatom, double *out, hipStream_t st) {
   if (e.nsys == 0) return;
   k_ens_thermo<<<e.nsys, kBlock, 0, st>>>(s.vel, s.n, e, lj_dev, eatom, out);
+}
+
+// Velocity rescale of the NVE constraint (Nve::apply_thermostat,
+// nve.cpp:186-189: every owned velocity times lambda).
+__global__ void __launch_bounds__(kBlock) k_scale_vel(double *__restrict__ vel, int64_t m, double lambda) {
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < m; k += (int64_t)gridDim.x * kBlock)
+    vel[k] *= lambda;
+}
+
+void launch_scale_vel(DevState &s, double lambda, hipStream_t st) {
+  const int64_t m = 3 * (int64_t)s.n;
+  if (m == 0) return;
+  const int grid = (int)std::min<int64_t>((m + kBlock - 1) / kBlock, 4096);
+  k_scale_vel<<<grid, kBlock, 0, st>>>(s.vel, m, lambda);
 }
 
 void launch_check_disp(DevState &s, const BoxDev &b, hipStream_t st) {

@@ -78,6 +78,7 @@ def lib():
             "cv2_set_lj": ([vp, C.c_int, _d, _d, _d, C.c_double, _d], C.c_int),
             "cv2_set_neighbor": ([vp, C.c_double, C.c_double, C.c_uint], C.c_int),
             "cv2_set_jump_tol": ([vp, C.c_double], C.c_int),
+            "cv2_scale_velocities": ([vp, C.c_double], C.c_int),
             "cv2_set_precision": ([vp, C.c_int], C.c_int),
             "cv2_upload_atoms": ([vp, C.c_int64, _d, _d, _i32, _i64], C.c_int),
             "cv2_set_ghosts": ([vp, C.c_int64, _d, _i32, _i64], C.c_int),
@@ -156,6 +157,10 @@ class Context:
     def set_jump_tol(self, jump_tol):
         """jump_fix: pairs closer than sqrt(jump_tol) take the force of r^2 = jump_tol."""
         self._chk(self.L.cv2_set_jump_tol(self.h, float(jump_tol)))
+
+    def scale_velocities(self, lam):
+        """Velocity rescale of the NVE constraint: every owned velocity times lam."""
+        self._chk(self.L.cv2_scale_velocities(self.h, float(lam)))
 
     def set_neighbor(self, cutoff, extra_coef, flags=0):
         self._chk(self.L.cv2_set_neighbor(self.h, float(cutoff), float(extra_coef), int(flags)))

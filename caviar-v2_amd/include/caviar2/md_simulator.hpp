@@ -75,6 +75,7 @@ class Md_simulator {
   void advance(int64_t nsteps);
   void sync_host();
   void write_outputs(int64_t step, double t);
+  void apply_constraints(int64_t step);
   Caviar2 *caviar_ = nullptr;
   DeviceSession *session_ = nullptr;
 };

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "caviar2/caviar2.hpp"
+#include "caviar2/constraint/nve.hpp"
 #include "caviar2/tools/atom.hpp"
 #include "caviar2/writer/atom_writer.hpp"
 #include "cv2_md.h"
@@ -517,8 +518,11 @@ void Md_simulator::verify_settings() {
   if (!(use_time || use_step))
     caviar_->log.error_all("simulator md: verify_settings: please assign 'time' or 'step' variables.");
   if (dt < 0.0) caviar_->log.error_all("expected 'dt' input");
-  if (!caviar_->constraints.empty())
-    caviar_->log.error_all("constraints are not supported on the device path");
+  for (auto *c : caviar_->constraints) {
+    if (!c->device_capable())
+      caviar_->log.error_all("only velocity-rescale constraints (constraint::Nve) run on the device path");
+    c->verify_settings();
+  }
 }
 
 DeviceSession *Md_simulator::device_session() {
@@ -594,9 +598,23 @@ void Md_simulator::write_outputs(int64_t step_i, double t) {
   for (auto *w : caviar_->writers) w->write(step_i, t);
 }
 
+// The constraints acting at step_i, after its velocity update and before its
+// writers (md_simulator.cpp:683-689,725-728).
+void Md_simulator::apply_constraints(int64_t step_i) {
+  for (auto *c : caviar_->constraints) {
+    if (c->next_device_step(step_i) != step_i) continue;
+    DeviceSession *s = device_session();
+    double ke = 0.0;
+    s->check(cv2_thermo(s->ctx, atom_data->get_n_r_df(), &ke, nullptr, nullptr), "cv2_thermo");
+    const double lam = c->device_velocity_scale(step_i, ke, atom_data->degree_of_freedoms());
+    if (lam != 1.0) s->check(cv2_scale_velocities(s->ctx, lam), "cv2_scale_velocities");
+  }
+}
+
 void Md_simulator::step() {
   if (!initialized) setup();
   advance(1);
+  apply_constraints(current_step);
   write_outputs(current_step, time);
   time += dt;
 }
@@ -608,14 +626,20 @@ bool Md_simulator::run() {
   if (!initialized) setup();
   current_step = initial_step;
   while (current_step < final_step) {
-    // run up to (and including) the next step at which a writer reports
+    // run up to (and including) the next step at which a writer reports or
+    // a constraint acts
     int64_t stop = final_step - 1;
     for (auto *w : caviar_->writers) {
       const int64_t e = w->next_output_step(current_step);
       if (e >= current_step && e < stop) stop = e;
     }
+    for (auto *c : caviar_->constraints) {
+      const int64_t e = c->next_device_step(current_step);
+      if (e >= current_step && e < stop) stop = e;
+    }
     const int64_t k = stop - current_step + 1;
     advance(k);
+    apply_constraints(stop);
     time = dt * (double)stop;
     write_outputs(stop, time);
     current_step = stop + 1;
@@ -629,6 +653,64 @@ bool Md_simulator::run() {
   caviar_->log.info("md_simulation run call:Wall Time = " + std::to_string(wall1 - wall0) + " sec. , ");
   return true;
 }
+
+// ------------------------------------------------------------------ constraint::Nve
+namespace constraint {
+
+Nve::Nve(Caviar2 *caviar) : Constraint(caviar), atom_data(&caviar->atom_data) {
+  constraint_type = Constraint_t::Nve;
+}
+
+void Nve::verify_settings() {  // nve.cpp:99-130
+  const bool kbt_set = kbt > 0.0, temperature_set = temperature > 0.0, kb_set = kb > 0.0;
+  const bool kb_temperature_set = temperature_set && kb_set;
+  if (kb_temperature_set != (temperature_set || kb_set))
+    caviar_->log.error_all(FC_FILE_LINE_FUNC, "kb and temperature should be set together.");
+  const bool kbt_kb_temperature_set = kb_temperature_set || kbt_set;
+  const bool energy_set = energy_per_dof > 0.0 || energy_tot > 0.0;
+  if (!(energy_set || kbt_kb_temperature_set))
+    caviar_->log.error_all(FC_FILE_LINE_FUNC, "energy or temperature is not set.");
+  if (energy_set && kbt_kb_temperature_set)
+    caviar_->log.error_all(FC_FILE_LINE_FUNC, "Only one of 'energy' and 'temperature' can be set.");
+  if (energy_per_dof > 0.0 && energy_tot > 0.0)
+    caviar_->log.error_all(FC_FILE_LINE_FUNC, "Only one of 'energy_per_dof' and 'energy_tot' can be set.");
+  if (kbt_set && kb_temperature_set)
+    caviar_->log.error_all(FC_FILE_LINE_FUNC, "Only one of 'kbt' and 'temperatue kb' can be set.");
+  if (step <= 0) caviar_->log.error_all(FC_FILE_LINE_FUNC, "step have to non-negative.");
+}
+
+// nve.cpp:147-184.  With a zero kinetic energy the reference warns that the
+// step is ignored; lambda is then 1 (the reference divides by the zero).
+double Nve::lambda(double ke_now, int dof, int64_t timestep) const {
+  if (ke_now == 0.0) {
+    caviar_->log.warning("energy = 0. NVE thermostat step is ignored at " + std::to_string(timestep));
+    return 1.0;
+  }
+  if (energy_per_dof > 0.0) return std::sqrt(energy_per_dof * dof / ke_now);
+  if (energy_tot > 0.0) return std::sqrt(energy_tot / ke_now);
+  if (kbt > 0.0) return std::sqrt(dof * kbt * 0.5 / ke_now);
+  if (temperature > 0.0) return std::sqrt(dof * kb * temperature * 0.5 / ke_now);
+  return 1.0;
+}
+
+void Nve::apply_thermostat(int64_t timestep, bool &recalculate_temperature) {
+  if (timestep % step != 0) return;
+  recalculate_temperature = true;
+  const double lam = lambda(atom_data->kinetic_energy(), atom_data->degree_of_freedoms(), timestep);
+  for (auto &v : atom_data->atom_struct_owned.velocity) v *= lam;
+}
+
+int64_t Nve::next_device_step(int64_t from) const {
+  if (step <= 0) return -1;
+  const int64_t r = from % step;
+  return r == 0 ? from : from + (step - r);
+}
+
+double Nve::device_velocity_scale(int64_t timestep, double ke_now, int dof) {
+  return lambda(ke_now, dof, timestep);
+}
+
+}  // namespace constraint
 
 // ------------------------------------------------------------------ Atom_writer
 namespace writer {

@@ -173,6 +173,12 @@ int cv2_run_langevin(cv2_ctx *ctx, int64_t nsteps, double dt, double gamma, doub
  * (Atom_data::virialForce, lj.cpp:487-490,516).  Any pointer may be NULL. */
 int cv2_thermo(cv2_ctx *ctx, int n_r_df, double *ke, double *pe, double *virial);
 
+/* Multiply every owned velocity by lambda: the velocity rescale of the NVE
+ * constraint (constraint::Nve::apply_thermostat, nve.cpp:186-189), whose
+ * lambda (nve.cpp:147-184) the host forms from cv2_thermo's kinetic energy.
+ * Acts on the state between runs (velocities of the last completed step). */
+int cv2_scale_velocities(cv2_ctx *ctx, double lambda);
+
 /* Copy state back in upload order (AoS xyz; any pointer may be NULL). */
 int cv2_download(cv2_ctx *ctx, double *pos, double *vel, double *acc);
 

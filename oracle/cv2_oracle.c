@@ -657,6 +657,21 @@ void orc_sim_get(const orc_sim *s, double *pos, double *vel, double *acc) {
   if (acc) memcpy(acc, s->acc, b);
 }
 
+double orc_nve_lambda(double ke_now, int dof, double energy_per_dof, double energy_tot,
+                      double kbt, double kb, double temperature) {
+  if (ke_now == 0.0) return 1.0;                                           /* :147-150 */
+  if (energy_per_dof > 0.0) return sqrt(energy_per_dof * dof / ke_now);    /* :152-159 */
+  if (energy_tot > 0.0) return sqrt(energy_tot / ke_now);                  /* :160-164 */
+  if (kbt > 0.0) return sqrt(dof * kbt * 0.5 / ke_now);                    /* :165-172 */
+  if (temperature > 0.0) return sqrt(dof * kb * temperature * 0.5 / ke_now); /* :173-179 */
+  return 1.0;
+}
+
+void orc_sim_scale_velocities(orc_sim *s, double lambda) {
+  for (int64_t i = 0; i < 3 * s->n; ++i) s->vel[i] *= lambda;
+  sim_ke(s);
+}
+
 int64_t orc_sim_nghost(const orc_sim *s) { return s->ng; }
 int64_t orc_sim_npairs(const orc_sim *s) { return s->npairs; }
 

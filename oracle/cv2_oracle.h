@@ -122,6 +122,15 @@ int64_t orc_sim_nghost(const orc_sim *s);
 int64_t orc_sim_npairs(const orc_sim *s);
 void orc_sim_thermo(const orc_sim *s, double *ke, double *temperature, double *virial,
                     double *pe);
+/* Nve::apply_thermostat, nve.cpp:147-184: the factor that takes the kinetic
+ * energy ke_now of dof degrees of freedom to the target set by exactly one of
+ * energy_per_dof, energy_tot, kbt or kb*temperature (the others <= 0).
+ * With ke_now = 0 the reference warns that the step is ignored (:147-150)
+ * and then divides by the zero; 1 here, the step ignored as the warning says. */
+double orc_nve_lambda(double ke_now, int dof, double energy_per_dof, double energy_tot,
+                      double kbt, double kb, double temperature);
+/* nve.cpp:186-189: every velocity times lambda (kinetic energy refreshed). */
+void orc_sim_scale_velocities(orc_sim *s, double lambda);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,9 @@ def oracle_lib():
         L.orc_sim_npairs.argtypes = [_vp]
         L.orc_sim_npairs.restype = C.c_int64
         L.orc_sim_thermo.argtypes = [_vp, _d, _d, _d, _d]
+        L.orc_nve_lambda.argtypes = [C.c_double, C.c_int] + [C.c_double] * 5
+        L.orc_nve_lambda.restype = C.c_double
+        L.orc_sim_scale_velocities.argtypes = [_vp, C.c_double]
         _orc = L
     return _orc
 
@@ -198,6 +201,12 @@ def pair_energy(r2, eps, sigma, rc2):
     return oracle_lib().orc_lj_pair_energy(r2, eps, sigma, rc2)
 
 
+def nve_lambda(ke_now, dof, energy_per_dof=0.0, energy_tot=0.0, kbt=0.0, kb=0.0, temperature=0.0):
+    """Velocity factor of constraint::Nve::apply_thermostat (nve.cpp:147-184)."""
+    return oracle_lib().orc_nve_lambda(float(ke_now), int(dof), float(energy_per_dof), float(energy_tot),
+                                       float(kbt), float(kb), float(temperature))
+
+
 def kinetic_energy(vel, type_, mass, n_r_df):
     v = _f64(vel); t = np.ascontiguousarray(type_, np.int32); m = _f64(np.atleast_1d(mass))
     return oracle_lib().orc_kinetic_energy(len(v), _p(v, _d), _p(t, _i32), _p(m, _d), n_r_df)
@@ -262,6 +271,9 @@ class OracleSim(_SimBase):
 
     def nghost(self):
         return self.L.orc_sim_nghost(self.h)
+
+    def scale_velocities(self, lam):
+        self.L.orc_sim_scale_velocities(self.h, float(lam))
 
     def close(self):
         if self.h:

@@ -84,3 +84,35 @@ def test_lattice_example_runs(tmp_path):
     t = np.loadtxt(tmp_path / "lattice_temperature.txt")
     assert t.shape[1] == 3 and np.all(np.isfinite(t[:, 2]))
     assert [int(x) for x in t[:, 0]] == [0] + list(range(0, 3000, 500))
+
+
+def test_nve_constraint_host(tmp_path):
+    """constraint::Nve on the host state: at a multiple of `step` the kinetic
+    energy becomes dof * kbt / 2 (nve.cpp:165-172,186-189), elsewhere it is
+    untouched; conflicting settings end the program (nve.cpp:119-120)."""
+    exe = _compile(os.path.join(ROOT, "tests", "cpp", "nve_check.cpp"), str(tmp_path / "nve"))
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    ke0, ke7, ke1 = (float.fromhex(x) for x in out[:3])
+    dof, recalc, nxt = int(out[3]), int(out[4]), int(out[5])
+    vel = np.array([[1.0, -0.5, 0.25], [-0.75, 0.5, 1.5], [0.0, 2.0, -1.0], [0.5, 0.5, 0.5]])
+    ref = O.kinetic_energy(vel, np.zeros(4, np.int32), np.array([2.0]), 3)
+    assert dof == 3 * 4 - 3 and recalc == 1 and nxt == 30
+    assert ke0 == pytest.approx(ref, rel=1e-15) and ke7 == ke0
+    assert ke1 == pytest.approx(0.5 * dof * 1.3, rel=1e-14)
+    r = subprocess.run([exe, "bad"], capture_output=True, text=True)
+    assert r.returncode != 0 and "Only one of 'energy' and 'temperature' can be set." in r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_nve_example_rescales_on_device(tmp_path):
+    """The velocity-rescale constraint in a device run (Md_simulator chunks at
+    its steps): the temperature the writer reports at each multiple of 50 --
+    after the rescale, as md_simulator.cpp:683-689,725-728 order them -- is
+    the constraint's kbt / k_b."""
+    r = subprocess.run([os.path.join(EX, "lj_nve"), "500"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    t = np.loadtxt(tmp_path / "nve_temperature.txt")
+    assert [int(x) for x in t[:, 0]] == [0] + list(range(0, 500, 50))
+    assert abs(t[0, 2] - 0.9) > 0.1  # the random start is far from the target
+    assert np.allclose(t[1:, 2], 0.9, rtol=1e-12, atol=0)

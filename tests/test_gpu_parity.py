@@ -308,3 +308,40 @@ def test_jump_fix_forces(prec, mix):
     else:
         assert emax < FP32_TOL_MAX and erms < FP32_TOL_RMS
         assert abs(th["virial"] - vir) < 1e-5 * abs(vir)
+
+
+def test_nve_rescale_trajectory_matches_oracle():
+    """Velocity Verlet with the velocity-rescale constraint (constraint::Nve,
+    kbt = 1.1 every 5 steps) through the C ABI: the device's kinetic energy
+    gives lambda (nve.cpp:165-172) and cv2_scale_velocities applies it, as
+    Md_simulator does at the constraint's steps; the oracle runs the same
+    steps with its restatement.  Every lambda and the final state agree, and
+    the kinetic energy after each rescale is dof * kbt / 2."""
+    s = perturbed(lj_fluid(10, temperature=1.44, seed=13), 0.05, 13)
+    s["use_cell_list"] = 1
+    n = len(s["pos"])
+    dof = 3 * n - 3
+    with O.OracleSim(s) as o, D.Context.from_sysdef(s) as c:
+        o.setup()
+        step = 0
+        while step < 20:
+            # steps up to the next multiple of 5, then the rescale
+            k = 1 if step == 0 else 5 - (step - 1) % 5
+            for _ in range(k):
+                o.step()
+            c.run_vv(k, s["dt"])
+            step += k
+            lo = O.nve_lambda(o.thermo()["ke"], dof, kbt=1.1)
+            lg = O.nve_lambda(c.thermo()["ke"], dof, kbt=1.1)
+            print(f"[nve] after step {step - 1}: lambda {lg:.15f} (oracle {lo:.15f})", flush=True)
+            assert abs(lg - lo) < 1e-12
+            o.scale_velocities(lo)
+            c.scale_velocities(lg)
+            assert c.thermo()["ke"] == pytest.approx(0.5 * dof * 1.1, rel=1e-13)
+        po, vo, _ = o.state()
+        pg, vg, _ = c.download()
+    L = s["hi"] - s["lo"]
+    d = pg - po
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() < 1e-10
+    assert np.abs(vg - vo).max() < 1e-8

@@ -258,3 +258,32 @@ def test_jump_fix_dimer_closed_form():
                           + 12 * (sig ** 2 / c2) ** 6 / c2 - 6 * (sig ** 2 / c2) ** 3 / c2)
         assert np.isclose(acc[0, 0], coef * r, rtol=1e-13) and np.isclose(acc[1, 0], -coef * r, rtol=1e-13)
         assert np.isclose(vir, -coef * r2, rtol=1e-13)
+
+
+@pytest.mark.parametrize("mode", ["energy_per_dof", "energy_tot", "kbt", "kb_temperature"])
+@pytest.mark.parametrize("n_r_df", [0, 3])
+def test_nve_rescale_reaches_target(mode, n_r_df):
+    """constraint::Nve (nve.cpp:147-189): after the rescale the kinetic energy
+    is the target.  KE(lambda v) = lambda^2 KE(v) with or without the
+    centre-of-mass term, so each mode's closed form pins lambda; a target
+    formula with a dropped factor (dof, 1/2, kb) fails one of them."""
+    s = lj_fluid(3, seed=3)
+    rng = np.random.default_rng(2)
+    s = dict(s, vel=rng.normal(0.3, 1.0, s["pos"].shape), n_r_df=n_r_df)
+    n = len(s["pos"])
+    dof = 3 * n - n_r_df
+    arg = dict(energy_per_dof=dict(energy_per_dof=0.7), energy_tot=dict(energy_tot=450.0),
+               kbt=dict(kbt=0.9), kb_temperature=dict(kb=2.0, temperature=0.45))[mode]
+    target = dict(energy_per_dof=0.7 * dof, energy_tot=450.0, kbt=0.5 * dof * 0.9,
+                  kb_temperature=0.5 * dof * 2.0 * 0.45)[mode]
+    with O.OracleSim(s) as o:
+        o.setup()
+        ke0 = o.thermo()["ke"]
+        assert ke0 > 0
+        lam = O.nve_lambda(ke0, dof, **arg)
+        o.scale_velocities(lam)
+        ke1 = o.thermo()["ke"]
+        v = o.state()[1]
+    assert ke1 == pytest.approx(target, rel=1e-13)
+    assert np.allclose(v, lam * s["vel"], rtol=0, atol=1e-15 * np.abs(s["vel"]).max())
+    assert O.nve_lambda(0.0, dof, kbt=0.9) == 1.0  # the ignored step
