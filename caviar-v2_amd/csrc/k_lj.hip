@@ -168,6 +168,7 @@ __global__ void __launch_bounds__(kBlock)
 struct PairF32 {
   double rc2, sig2, r2ovf;  // in units of g^2
   double jt;                // jump_fix clamp of r^2 (0: off), units of g^2
+  float rc2if;              // 1 / rc2 rounded to FP32 (cutoff test of the branchless entry)
   float e6, e12, fsh;
   float eps4;
   double ecut, ecut2;       // energy: ecut in units of 1/g^2
@@ -181,6 +182,7 @@ __device__ inline PairF32 pair_f32_table(const LJDev *__restrict__ lj, int tp) {
   c.sig2 = lj->sig2[tp];
   c.r2ovf = lj->r2ovf[tp];
   c.jt = lj->jump_tol;
+  c.rc2if = (float)(1.0 / c.rc2);
   c.e6 = lj->e6f[tp];
   c.e12 = lj->e12f[tp];
   c.fsh = lj->fshf[tp];
@@ -198,6 +200,7 @@ __device__ inline PairF32 pair_f32(const LJDev *__restrict__ lj, int tp, double 
   c.sig2 = lj->sig2[tp] * g2i;
   c.r2ovf = lj->r2ovf[tp] * g2i;
   c.jt = lj->jump_tol * g2i;
+  c.rc2if = (float)(1.0 / c.rc2);
   const double e4s = lj->eps4[tp] * kForceScale / g;
   c.e6 = (float)(6.0 * e4s);
   c.e12 = (float)(12.0 * e4s);
@@ -371,8 +374,12 @@ __device__ inline void pair_f32_flat(const int4 qi, const int4 qj, double u0, do
   const float s6 = (float)(s2 * s2 * s2);
   const float r2if = (float)r2i;
   const float fc = fmaf(s6 * fmaf(-c.e12, s6, c.e6), r2if, c.fsh);
-  // SELF: the list pads with the atom itself (r^2 = 0), which adds nothing
-  float fsel = (r2 <= c.rc2 && (!SELF || r2 > 0.0)) ? fc : 0.0f;
+  // The cutoff test compares r^-2 in FP32 (one FP32 compare instead of an
+  // FP64 one): it can differ from r2 <= rc2 only within an FP32 rounding of
+  // the cutoff, where the shifted force is zero to first order
+  // (forceCoef(rc) = 0, lj.cpp:475-484).  SELF: the list pads with the atom
+  // itself (r^2 = 0), which adds nothing.
+  float fsel = (SELF ? (r2 <= c.rc2 && r2 > 0.0) : (r2if >= c.rc2if)) ? fc : 0.0f;
   __asm__ volatile("" : "+v"(fsel));  // select in FP32, before the conversion
   fcmax = fmaxf(fcmax, fabsf(fsel));
   const double fcs = (double)fsel;
