@@ -687,7 +687,7 @@ __device__ inline void row_range(const int *__restrict__ cstart, const CellWaveB
   }
 }
 
-constexpr int kSlots = 10;                  // candidates per pass: kSlots * 64
+constexpr int kSlots = 9;                  // candidates per pass: kSlots * 64
 constexpr int kPassCand = kSlots * 64;
 constexpr int kSegs = 2 * kWaveRows;         // row ranges: 9 direct-x + 9 wrapped-x
 
