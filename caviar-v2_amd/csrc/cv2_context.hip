@@ -53,6 +53,8 @@ struct cv2_ctx {
   long long *h_fx = nullptr;
   int partial_blocks = 0;
   bool fuse = true;  // integrator fused into the FP32 task kernel where it runs
+  int min_mpr = 0;   // task slots per cell row asked for after a row ran out of them
+  int task_grid_cap = 0;  // CV2_TASK_GRID: cap of the force launch's grid (tests: tail launch)
   // HIP-event timing of kernel groups (cv2_set_timing): every group, or
   // (period > 1) the force launch of every period-th force call only
   bool timing = false;
@@ -140,7 +142,8 @@ void free_state(DevState &s) {
   void *ptrs[] = {s.pos, s.pos_alt, s.pos_ref, s.q, s.vel, s.vel_alt, s.acc, s.acc_alt, s.tag,
                   s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.order, s.box_dev, s.wstat, s.cell_count, s.cell_start, s.cell_local,
                   s.gcell_count, s.gcell_start, s.gcell_items, s.tile_sum, s.nbr, s.nnbr, s.flags,
-                  s.partials, s.fx_acc, s.reduce_out, s.tpiece, s.tstage, s.q_alt};
+                  s.partials, s.fx_acc, s.reduce_out, s.tpiece, s.tinfo, s.ctask, s.rowtasks,
+                  s.ctinfo, s.ctpiece, s.q_alt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   s = DevState{};
@@ -233,10 +236,23 @@ int setup_box(cv2_ctx *ctx) {
   return CV2_OK;
 }
 
+bool ref_rebuilds(const cv2_ctx *ctx) { return (ctx->nl_flags & CV2_NL_REFERENCE_REBUILDS) != 0; }
+
 int ensure_ready(cv2_ctx *ctx) {
   if (!ctx->box_set || !ctx->lj_set || !ctx->nl_set || !ctx->atoms)
     return fail(ctx, CV2_ERR_STATE, "box, LJ parameters, neighbour settings and atoms must be set first");
   return CV2_OK;
+}
+
+// Launch grid of the force task kernel for ntask tasks: a multiple of 8
+// (one share per XCD) with 6% + 16 spare work-groups, so that the task
+// count of later builds rarely exceeds it (the tail launch then takes the
+// rest); at most cap (> 0) when set.
+int task_grid_for(int64_t ntask, int64_t nslot, int cap) {
+  int64_t g = ntask + ntask / 16 + 16;
+  g = std::min(g, std::max<int64_t>(nslot, 8));
+  if (cap > 0) g = std::min<int64_t>(g, cap);
+  return (int)((g + 7) / 8 * 8);
 }
 
 int check_flags(cv2_ctx *ctx) {
@@ -244,6 +260,17 @@ int check_flags(cv2_ctx *ctx) {
                         ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   const int *f = ctx->h_flags;
+  DevState &s = ctx->s;
+  if (s.slots && f[F_NTASK] > 0) s.task_grid = task_grid_for(f[F_NTASK], s.tg.nslot, ctx->task_grid_cap);
+  if (f[F_TASKOVF]) {
+    // some cell row wanted more task slots: its last task took the rest of
+    // the row (correct, slower); widen the rows at the next build
+    ctx->min_mpr = std::max(ctx->min_mpr, s.tg.mpr + 2);
+    ctx->need_build = true;
+    int zero = 0;
+    HIPCHK(hipMemcpyAsync(s.flags + F_TASKOVF, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
   if (f[F_OVERFLOW])
     return fail(ctx, CV2_ERR_CAPACITY,
                 "neighbour list capacity " + std::to_string(ctx->s.cap) + " exceeded (longest list " +
@@ -266,13 +293,23 @@ int prepare_list_form(cv2_ctx *ctx) {
   s.slots = ctx->precision != CV2_PREC_FP64 && ctx->b.coarse && s.n > 0 &&
             rc2max < 1.9 * ctx->b.ce_sq;
   if (!s.slots) return CV2_OK;
-  s.tg = task_geom(s.n, ctx->b);
-  if (s.tg.ntask > s.task_cap) {
-    HIPCHK(dalloc(&s.tpiece, (size_t)s.tg.ntask * kTaskPieces));
-    HIPCHK(dalloc(&s.tstage, (size_t)s.tg.ntask));
-    s.task_cap = s.tg.ntask;
+  const TaskGeom old = s.tg;
+  s.tg = task_geom(s.n, ctx->b, ctx->min_mpr);
+  if (s.tg.nslot > s.task_cap) {
+    HIPCHK(dalloc(&s.tpiece, (size_t)s.tg.nslot * kTaskPieces));
+    HIPCHK(dalloc(&s.tinfo, (size_t)s.tg.nslot));
+    HIPCHK(dalloc(&s.ctinfo, (size_t)s.tg.nslot));
+    HIPCHK(dalloc(&s.ctpiece, (size_t)s.tg.nslot * kTaskPieces));
+    s.task_cap = s.tg.nslot;
   }
-  const int nb = s.tg.ntask + 8;  // per-task energy partials
+  const int rows = s.tg.ny * s.tg.nz;
+  if (rows != old.ny * old.nz || !s.rowtasks) HIPCHK(dalloc(&s.rowtasks, rows));
+  HIPCHK(dalloc(&s.ctask, s.ncell));
+  // until a build has counted the tasks: about one per 200 atoms, plus one
+  // per row for the row ends
+  if (s.task_grid <= 0 || s.tg.mpr != old.mpr)
+    s.task_grid = task_grid_for((int64_t)s.n / 200 + rows, s.tg.nslot, ctx->task_grid_cap);
+  const int nb = s.tg.nslot + 80;  // per-block energy partials (main grid <= nslot + 7, tail 64)
   if (nb > ctx->partial_blocks) {
     HIPCHK(dalloc(&s.partials, (size_t)nb * 8));
     ctx->partial_blocks = nb;
@@ -517,6 +554,9 @@ int cv2_create(cv2_ctx **out, int device) {
     ctx->s.stage_limit = std::max(0, std::min(kStage, std::atoi(lim)));
   // CV2_FUSE=0 keeps the integrator in its own kernel (tests compare both)
   if (const char *fz = std::getenv("CV2_FUSE")) ctx->fuse = std::atoi(fz) != 0;
+  // CV2_TASK_GRID caps the force launch's grid, so that the tail launch
+  // takes the remaining tasks (tests compare both)
+  if (const char *tg = std::getenv("CV2_TASK_GRID")) ctx->task_grid_cap = std::max(0, std::atoi(tg));
   if (e != hipSuccess) {
     fprintf(stderr, "cv2_create: %s\n", hipGetErrorString(e));
     cv2_destroy(ctx);
@@ -727,7 +767,7 @@ int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *t
   s.ng = (int)ng;
   if (count_changed) ctx->need_build = true;  // neighborlist.cpp:118-121
   if (s.write_q) launch_pack_q(s, ctx->b, ctx->stream);
-  launch_check_disp(s, ctx->b, ctx->stream);  // ghosts compared by index, :139-144
+  launch_check_disp(s, ctx->b, ref_rebuilds(ctx), ctx->stream);  // ghosts compared by index, :139-144
   HIPCHK(hipStreamSynchronize(ctx->stream));  // p4/idv are stack buffers
   return CV2_OK;
 }
@@ -841,7 +881,7 @@ int cv2_halo_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const
                      ctx->halo_base + 128, (int)ng, ctx->stream);
   if (count_changed) ctx->need_build = true;  // neighborlist.cpp:118-121
   if (s.write_q) launch_pack_q(s, ctx->b, ctx->stream);
-  launch_check_disp(s, ctx->b, ctx->stream);  // ghosts compared by index, :139-144
+  launch_check_disp(s, ctx->b, ref_rebuilds(ctx), ctx->stream);  // ghosts compared by index, :139-144
   HIPCHK(hipStreamSynchronize(ctx->stream));  // tab is a stack buffer
   return CV2_OK;
 }
@@ -855,7 +895,7 @@ int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt) {
   if (rebuilt) {
     HIPCHK(hipMemcpy(&before, ctx->s.flags + F_REBUILDS, sizeof(int), hipMemcpyDeviceToHost));
   }
-  if (!force) launch_check_disp(ctx->s, ctx->b, ctx->stream);
+  if (!force) launch_check_disp(ctx->s, ctx->b, ref_rebuilds(ctx), ctx->stream);
   rc = queue_build(ctx, force != 0, true);
   if (rc) return rc;
   ctx->forces_valid = false;
@@ -885,6 +925,25 @@ int cv2_compute_forces(cv2_ctx *ctx) {
   return CV2_OK;
 }
 
+int cv2_compute_accelerations(cv2_ctx *ctx) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  int rc = ensure_ready(ctx);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->need_build) {
+    rc = queue_build(ctx, true, false);
+    if (rc) return rc;
+  }
+  if (ctx->precision != CV2_PREC_FP64) launch_pack_q(ctx->s, ctx->b, ctx->stream);
+  const int t = span_begin(ctx, 0);
+  launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, false, ctx->stream);
+  span_end(ctx, t, 0);
+  rc = check_flags(ctx);
+  if (rc) return rc;
+  ctx->forces_valid = true;  // the energy and virial of cv2_thermo stay those of the last energy pass
+  return CV2_OK;
+}
+
 // Shared step loop.  On entry the forces of the current positions are
 // known; each iteration closes the previous step and opens the next one in
 // one integrator pass, runs the (device-conditional) list build, and
@@ -894,6 +953,7 @@ int cv2_compute_forces(cv2_ctx *ctx) {
 // build.
 int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
   u.writeq = ctx->precision != CV2_PREC_FP64;
+  u.ref_rebuild = ref_rebuilds(ctx);
   const uint64_t s0 = (uint64_t)ctx->steps;
   bool opened = false;  // this iteration's integrator pass already ran (fused)
   for (int64_t k = 0; k < nsteps; ++k) {
@@ -1027,12 +1087,23 @@ int cv2_download(cv2_ctx *ctx, double *pos, double *vel, double *acc) {
   if (vel) HIPCHK(hipMemcpyAsync(v.data(), s.vel, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
   if (acc) HIPCHK(hipMemcpyAsync(a.data(), s.acc, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  // The device wraps positions into the box at list builds; between builds
+  // an atom may lie up to half the skin beyond a periodic face.  The copy
+  // gets the reference's single wrap (Domain::fix_position,
+  // domain.cpp:443-493), which a build would apply to the same value.
+  const BoxDev &b = ctx->b;
+  auto wrap = [&](double c, int d) {
+    if (b.bc[d] != 1) return c;
+    if (c < b.lo[d]) return c + b.L[d];
+    if (c > b.hi[d]) return c - b.L[d];
+    return c;
+  };
   for (int64_t k = 0; k < n; ++k) {
     const int64_t o = tg[k];
     if (pos) {
-      pos[3 * o] = p4[k].x;
-      pos[3 * o + 1] = p4[k].y;
-      pos[3 * o + 2] = p4[k].z;
+      pos[3 * o] = wrap(p4[k].x, 0);
+      pos[3 * o + 1] = wrap(p4[k].y, 1);
+      pos[3 * o + 2] = wrap(p4[k].z, 2);
     }
     for (int d = 0; d < 3; ++d) {
       if (vel) vel[3 * o + d] = v[d * n + k];
@@ -1057,12 +1128,14 @@ int cv2_get_pairs(cv2_ctx *ctx, int64_t cap, int64_t *npairs, int64_t *pairs) {
   // task-slot form: rebuild the 32-bit entries (atom | image) from the task
   // piece tables; the image of j follows from its cell relative to i's cell
   // (periodic grid, at least 5 cells per axis)
-  std::vector<int> cst, tst;
+  std::vector<int> cst, ctk;
   std::vector<int4> tpc;
   if (s.slots) {
     cst.resize(s.ncell + 1);
-    tpc.resize((size_t)s.tg.ntask * kTaskPieces);
+    ctk.resize(s.ncell);
+    tpc.resize((size_t)s.tg.nslot * kTaskPieces);
     HIPCHK(hipMemcpyAsync(cst.data(), s.cell_start, sizeof(int) * cst.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctk.data(), s.ctask, sizeof(int) * ctk.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(tpc.data(), s.tpiece, sizeof(int4) * tpc.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
   }
@@ -1076,7 +1149,7 @@ int cv2_get_pairs(cv2_ctx *ctx, int64_t cap, int64_t *npairs, int64_t *pairs) {
     const int sl = nb16[((size_t)(m >> 3) * s.stride + i) * 8 + (m & 7)];
     const int ci = cell_of(i);
     const int cx = ci % g.nx, cy = (ci / g.nx) % g.ny, cz = ci / (g.nx * g.ny);
-    const int4 *tp = &tpc[(size_t)((cy + g.ny * cz) * g.tpr + task_of_x(g, cx)) * kTaskPieces];
+    const int4 *tp = &tpc[(size_t)ctk[ci] * kTaskPieces];
     int p = 0;
     for (int k2 = 1; k2 < kTaskPieces; ++k2) p = tp[k2].y <= sl ? k2 : p;
     const int j = tp[p].x + (sl - tp[p].y);
@@ -1128,6 +1201,7 @@ int cv2_get_stats(cv2_ctx *ctx, cv2_stats *out) {
   for (int d = 0; d < 3; ++d) out->ncell[d] = ctx->b.nc[d];
   out->precision = ctx->precision;
   out->build_kind = ctx->b.coarse ? CV2_BUILD_CELL_WAVE : CV2_BUILD_PER_ATOM;
+  out->force_tasks = ctx->s.slots ? ctx->h_flags[F_NTASK] : 0;
   return CV2_OK;
 }
 

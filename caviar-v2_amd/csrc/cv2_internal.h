@@ -40,7 +40,9 @@ enum Flag : int {
   F_STAGE = 7,     // a force task's neighbourhood exceeds 16-bit slots
   F_STATBLK = 8,   // blocks of k_build_stats done (reset by the last one)
   F_BINBLK = 9,    // blocks of k_wrap_bin done (reset by the last one)
-  F_COUNT = 10
+  F_NTASK = 10,    // force tasks of the current build (compact task map length)
+  F_TASKOVF = 11,  // some cell row needed more task slots than it has
+  F_COUNT = 12
 };
 
 struct BoxDev {
@@ -174,7 +176,30 @@ struct StepParams {
   uint64_t step;      // step index s of the finishing half; s+1 starts
   double thresh_sq;
   double lo[3], qscale[3];
+  // CV2_NL_REFERENCE_REBUILDS: also rebuild when an atom leaves the box
+  // (outside [wlo, whi] on a periodic axis) or crosses the ghost band of a
+  // face (below blo / above bhi, i.e. within the ghost cutoff of it) since
+  // the last build; open axes carry infinite bounds
+  int ref_rebuild;
+  double wlo[3], whi[3], blo[3], bhi[3];
 };
+
+// The reference's other rebuild triggers for one atom at x, built at r:
+// Atom_data::exchange_owned_single_md_domain reports a wrap
+// (exchange_owned_single_md_domain.cpp:48-56, domain.cpp:443-493) and
+// Neighborlist::update_is_needed sees a changed ghost set
+// (neighborlist.cpp:117-120,139-144) whenever an atom enters or leaves the
+// ghost band of a face.
+__device__ inline bool ref_trigger(const double4 &x, const double4 &r, const StepParams &p) {
+  const double c[3] = {x.x, x.y, x.z}, o[3] = {r.x, r.y, r.z};
+  bool t = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    t = t || c[d] < p.wlo[d] || c[d] > p.whi[d];
+    t = t || (c[d] < p.blo[d]) != (o[d] < p.blo[d]) || (c[d] > p.bhi[d]) != (o[d] > p.bhi[d]);
+  }
+  return t;
+}
 
 
 // The integrator for one atom (k_update, and the epilogue of the fused FP32
@@ -230,7 +255,8 @@ __device__ inline void integrate_store(int i, int n, const double4 &x, const dou
     pos[i] = x;
     if (WRITEQ) q[i] = pack_q(x.x, x.y, x.z, (int)x.w, p.lo, p.qscale);
     const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
-    if (dx * dx + dy * dy + dz * dz > p.thresh_sq) atomicOr(&flags[F_REBUILD], 1);
+    if (dx * dx + dy * dy + dz * dz > p.thresh_sq || (p.ref_rebuild && ref_trigger(x, r, p)))
+      atomicOr(&flags[F_REBUILD], 1);
   }
   vel[i] = v[0];
   vel[n + i] = v[1];
@@ -254,23 +280,22 @@ bool coarse_filter_setup(BoxDev &b, double ce, double wmax);
 // contiguous atom ranges ("pieces") per row, staged back to back in LDS by
 // the force kernel; the list of the FP32 path stores, for each neighbour, its
 // 16-bit position ("slot") in that staging area.
+//
+// The rows are cut into tasks at every build from the actual cell counts
+// (k_scatter): greedily, each task takes cells while it holds at most
+// kTaskBlock home atoms and its neighbourhood fits the LDS stage, so a
+// lattice whose cells hold uneven counts gets tasks of even size.  Row r
+// owns task slots [r * mpr, (r + 1) * mpr); a row that would need more takes
+// the rest of the row into its last slot (and raises F_TASKOVF, so the host
+// widens mpr at the next forced build).  The used slots of all rows, in row
+// order, are copied to compact tables (ctinfo, ctpiece) that the force
+// kernel walks.
 constexpr int kTaskPieces = 18;  // 9 rows x 2 pieces; piece 2r + p of row r
 struct TaskGeom {
   int nx, ny, nz;  // cell grid
-  int tpr;         // tasks per x-row
-  int ntask;       // ny * nz * tpr
+  int mpr;         // task slots per x-row
+  int nslot;       // ny * nz * mpr
 };
-
-__host__ __device__ inline int task_x0(const TaskGeom &g, int t) {
-  return t * g.nx / g.tpr;  // nx, tpr <= 1024
-}
-
-// Task (within its row) of x-cell cx.
-__host__ __device__ inline int task_of_x(const TaskGeom &g, int cx) {
-  int t = cx * g.tpr / g.nx;
-  while (t + 1 < g.tpr && task_x0(g, t + 1) <= cx) ++t;
-  return t;
-}
 
 // x-cells [a0, a1) of piece p of a task over [x0, x1): the whole row once
 // when the neighbourhood covers it, else the direct range and, at a row
@@ -323,9 +348,13 @@ struct DevState {
   int *nnbr;
   bool slots;                   // FP32 path on a wave-build box: task-slot list
   TaskGeom tg;
-  int4 *tpiece;                 // per task: kTaskPieces x (first atom, slot base, x0, x1)
-  int *tstage;                  // per task: staged atoms
-  int64_t task_cap;
+  int4 *tpiece;                 // per task slot: kTaskPieces x (first atom, slot base, a0, a1)
+  int4 *tinfo;                  // per task slot: first home atom, end, staged atoms S, x0 | x1 << 16
+  int *ctask;                   // per cell: its task slot
+  int *rowtasks;                // per x-row: tasks used
+  int4 *ctinfo, *ctpiece;       // the used slots' tinfo / tpiece in row order (count: flags[F_NTASK])
+  int64_t task_cap;             // task slots allocated
+  int task_grid;                // work-groups of a force launch (tasks beyond loop)
   int stage_limit;              // staged atoms per block held in LDS
   int *flags;
   double *partials;            // per-block reduction partials
@@ -398,15 +427,18 @@ void launch_pack_q(const DevState &s, const BoxDev &b, hipStream_t st);
 void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, bool keep_acc);
 void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                    int precision, bool energy, hipStream_t st);
-// Task geometry for n atoms on b's cell grid (same result for the build and
-// the force kernel); staged-atom capacity of the force kernel's LDS.
-TaskGeom task_geom(int n, const BoxDev &b);
+// Task slots for n atoms on b's cell grid: mpr slots per x-row, at least
+// min_mpr; staged-atom capacity of the force kernel's LDS and home atoms per
+// task (one thread each).
+TaskGeom task_geom(int n, const BoxDev &b, int min_mpr);
 constexpr int kStage = 2560;
+constexpr int kTaskBlock = 256;
 // One pass of the integrator: finish = close step `step` (velocity update
 // after its forces), start = open step `step + 1` (velocity and position
 // update, fixed-point coordinates when writeq, rebuild check).
 struct UpdateArgs {
   bool langevin, finish, start, writeq;
+  bool ref_rebuild;  // CV2_NL_REFERENCE_REBUILDS
   double dt, la, lb, lc;
   uint64_t seed, step;
 };
@@ -420,7 +452,9 @@ StepParams step_params(const BoxDev &b, const UpdateArgs &u);
 // where the fused form does not apply.
 bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                          const UpdateArgs &u, hipStream_t st);
-void launch_check_disp(DevState &s, const BoxDev &b, hipStream_t st);
+// Displacement check (and, ref_rebuild, the wrap / ghost-band triggers)
+// of the current positions against the build positions.
+void launch_check_disp(DevState &s, const BoxDev &b, bool ref_rebuild, hipStream_t st);
 void launch_thermo(DevState &s, const LJDev *lj_dev, hipStream_t st);
 void launch_scale_vel(DevState &s, double lambda, hipStream_t st);
 void launch_set_flag(int *flags, int which, int value, hipStream_t st);

@@ -192,22 +192,33 @@ struct CellStarts {
   }
 };
 
-__device__ void task_table_entry(int task, const CellStarts &cstart, const TaskGeom &g,
-                                 int4 *__restrict__ tpiece, int *__restrict__ tstage,
-                                 int *__restrict__ flags);
+constexpr int kRowWaves = 4;  // x-rows partitioned per work-group (one wavefront each)
+constexpr int kMaxRowCells = 1024;
+
+__device__ void task_row_partition(int row, int lane, int *P, int *H,
+                                   const int *__restrict__ count, const CellStarts &cs,
+                                   const TaskGeom &g, int4 *__restrict__ tpiece,
+                                   int4 *__restrict__ tinfo, int *__restrict__ ctask,
+                                   int *__restrict__ rowtasks, int *__restrict__ flags);
 
 // Scatter atom indices to their cell slots (work-groups [0, nbs)); the
-// next nfix work-groups store the cell starts, and the rest fill the
-// force-task piece table.
+// next nfix work-groups store the cell starts, and the rest (ntb) cut the
+// x-rows of the cell grid into force tasks, one wavefront per row.
 __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
                           CellStarts cs, int *start, int *__restrict__ perm, int n,
                           int *__restrict__ flags, int nbs, int nfix, TaskGeom g,
-                          int4 *__restrict__ tpiece, int *__restrict__ tstage) {
+                          const int *__restrict__ count, int4 *__restrict__ tpiece,
+                          int4 *__restrict__ tinfo, int *__restrict__ ctask,
+                          int *__restrict__ rowtasks) {
   if (flags[F_REBUILD] == 0) return;
   const int b = blockIdx.x;
   if (b >= nbs + nfix) {
-    const int task = (b - nbs - nfix) * blockDim.x + threadIdx.x;
-    if (task < g.ntask) task_table_entry(task, cs, g, tpiece, tstage, flags);
+    // exclusive prefix sums over x of one row's 9-row column counts (P) and
+    // of its own counts (H), per wavefront
+    __shared__ int P[kRowWaves][kMaxRowCells + 1], H[kRowWaves][kMaxRowCells + 1];
+    const int wv = (int)(threadIdx.x >> 6);
+    task_row_partition((b - nbs - nfix) * kRowWaves + wv, (int)(threadIdx.x & 63), P[wv], H[wv],
+                       count, cs, g, tpiece, tinfo, ctask, rowtasks, flags);
     return;
   }
   if (b >= nbs) {
@@ -227,6 +238,52 @@ __device__ inline int4 qpack(const double4 &p, const BoxDev &b) {
   return pack_q(p.x, p.y, p.z, (int)p.w, b.lo, b.qscale);
 }
 
+// The force tasks of a build in compact order: the used task slots of all
+// x-rows, in row order, copied to ctinfo / ctpiece (what the force kernel
+// walks: one level of indirection less than a map of slot numbers); their
+// number goes to flags[F_NTASK].  kCompactParts extra work-groups of
+// k_rank_move (256 threads each): every one scans the per-row task counts
+// up to the end of its share of the rows and copies that share.
+constexpr int kCompactParts = 32;
+struct TaskMapArgs {
+  const int *rowtasks;
+  const int4 *tinfo, *tpiece;  // by slot
+  int4 *ctinfo, *ctpiece;      // compact
+  int rows, mpr;               // rows = 0: no force tasks
+};
+
+__device__ void compact_tasks(const TaskMapArgs &tm, int part, int *__restrict__ flags) {
+  __shared__ int sh[256];
+  const int t = threadIdx.x;
+  const int r_lo = (int)((long long)tm.rows * part / kCompactParts);
+  const int r_hi = (int)((long long)tm.rows * (part + 1) / kCompactParts);
+  int carry = 0;
+  for (int r0 = 0; r0 < r_hi; r0 += 256) {
+    const int r = r0 + t;
+    const int v = r < tm.rows ? tm.rowtasks[r] : 0;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int a = t >= off ? sh[t - off] : 0;
+      __syncthreads();
+      sh[t] += a;
+      __syncthreads();
+    }
+    const int at = carry + sh[t] - v;
+    if (r >= r_lo && r < r_hi) {
+      for (int k = 0; k < v; ++k) {
+        const size_t sl = (size_t)r * tm.mpr + k, c = (size_t)at + k;
+        tm.ctinfo[c] = tm.tinfo[sl];
+#pragma unroll 6
+        for (int p = 0; p < kTaskPieces; ++p) tm.ctpiece[c * kTaskPieces + p] = tm.tpiece[sl * kTaskPieces + p];
+      }
+    }
+    carry += sh[255];
+    __syncthreads();  // sh is rewritten by the next chunk
+  }
+  if (part == kCompactParts - 1 && t == 0) flags[F_NTASK] = carry;  // r_hi = rows: the total
+}
+
 // Order atoms inside each cell by their position in the previous order so
 // that the sorted order, and with it every list and sum, is independent of
 // atomic arrival order (atoms that stay in a cell keep their relative
@@ -244,10 +301,14 @@ __global__ void k_rank_move(const int *__restrict__ cell, const int *__restrict_
                             const int *__restrict__ tag, int *__restrict__ tag_out,
                             const int *__restrict__ id, int *__restrict__ id_out,
                             int *__restrict__ cell_count, int ncell, int4 *__restrict__ q,
-                            BoxDev b, const int *__restrict__ flags) {
+                            BoxDev b, int *__restrict__ flags, TaskMapArgs tm) {
   if (flags[F_REBUILD] == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   for (int c = i; c < ncell; c += gridDim.x * blockDim.x) cell_count[c] = 0;
+  if (tm.rows > 0 && (int)blockIdx.x >= (int)gridDim.x - kCompactParts) {
+    compact_tasks(tm, (int)blockIdx.x - ((int)gridDim.x - kCompactParts), flags);
+    return;
+  }
   if (i >= n) return;
   const int c = cell[i];
   const int a = start[c], e = start[c + 1];
@@ -890,17 +951,17 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
                           const int *__restrict__ cstart, uint32_t *__restrict__ nbr,
                           int *__restrict__ nnbr, int stride, int cap, int c, int wv, int lane,
                           uint32_t (*lds)[kPassCand + 4 + 64], uint16_t *cslot,
-                          int (*cnt_sh)[kMaxHomeCell], const TaskGeom &tg,
-                          const int4 *__restrict__ tpiece, const int *__restrict__ tstage) {
+                          int (*cnt_sh)[kMaxHomeCell], const int *__restrict__ ctask,
+                          const int4 *__restrict__ tpiece, const int4 *__restrict__ tinfo) {
   const int nx = b.nc[0], ny = b.nc[1];
   const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
   const int h0 = cstart[c], h1 = cstart[c + 1];
   if (h0 == h1) return make_int2(0, 0);
   uint32_t *cand = lds[wv];
-  // SLOTS: this cell's task and its piece table
-  const int4 *tp = SLOTS ? tpiece + (size_t)((cy + ny * cz) * tg.tpr + task_of_x(tg, cx)) * kTaskPieces
-                         : nullptr;
-  const int pad_slot = SLOTS ? tstage[(cy + ny * cz) * tg.tpr + task_of_x(tg, cx)] : 0;
+  // SLOTS: this cell's task, its piece table and sentinel slot
+  const int tsl = SLOTS ? ctask[c] : 0;
+  const int4 *tp = SLOTS ? tpiece + (size_t)tsl * kTaskPieces : nullptr;
+  const int pad_slot = SLOTS ? tinfo[tsl].z : 0;
   // segment table, one row range per lane (lanes < kSegs): first entry,
   // length, image bits, (SLOTS) slot offset and flattened start.  The
   // loads of all ranges are issued together as vector loads; the passes
@@ -1006,8 +1067,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
                   const BoxDev *__restrict__ bdev, const int *__restrict__ cstart,
                   uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
                   const int *__restrict__ flags, int nblk_real, int2 *__restrict__ bstat,
-                  TaskGeom tg, const int4 *__restrict__ tpiece, const int *__restrict__ tstage,
-                  int ncopy, CopyBack cpb) {
+                  const int *__restrict__ ctask, const int4 *__restrict__ tpiece,
+                  const int4 *__restrict__ tinfo, int ncopy, CopyBack cpb) {
   if (flags[F_REBUILD] == 0) return;
   const int lgrid = (int)gridDim.x - ncopy;  // list work-groups, a multiple of 8
   if ((int)blockIdx.x >= lgrid) {
@@ -1037,7 +1098,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
   int2 st = make_int2(0, 0);
   if (c < b.nc[0] * b.nc[1] * b.nc[2])
     st = cell_wave<GHOST_OK, SLOTS>(pos, q, b, bdev, cstart, nbr, nnbr, stride, cap, c, wv, lane,
-                                    lds, cslot_sh[wv], cnt_sh, tg, tpiece, tstage);
+                                    lds, cslot_sh[wv], cnt_sh, ctask, tpiece, tinfo);
   if (lane == 0) st_sh[wv] = st;
   __syncthreads();
   if (threadIdx.x == 0) {  // per-block statistics, reduced by k_build_stats
@@ -1086,32 +1147,111 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Piece table of every force task of the FP32 path (after the cell starts
-// of a build are known): first atom, slot base and x-cells of each piece,
-// and the staged total S; slot S is the task's sentinel, so S must fit 16
-// bits.
-__device__ void task_table_entry(int task, const CellStarts &cstart, const TaskGeom &g,
-                                 int4 *__restrict__ tpiece, int *__restrict__ tstage,
-                                 int *__restrict__ flags) {
-  const int t = task % g.tpr, row = task / g.tpr;
-  const int cy = row % g.ny, cz = row / g.ny;
-  const int x0 = task_x0(g, t), x1 = task_x0(g, t + 1);
-  int base = 0;
+// Force tasks of x-row `row` (one wavefront; k_scatter, after the cell
+// starts of a build are known).  P and H (LDS, nx + 1 entries each) receive
+// the exclusive prefix sums over x of the row's 9-row column counts and of
+// its own counts; from them the home atoms and the staged neighbourhood of
+// any run of cells [x0, x1) are O(1).  The row is cut greedily: each task
+// takes the longest run from x0 that keeps at most kTaskBlock home atoms and
+// a neighbourhood (plus the sentinel) within kStage, at least one cell; the
+// last slot of the row takes whatever remains.  Each task slot gets its
+// piece table (first atom, slot base, x-cells of every piece) and tinfo
+// (home atoms [h0, h1), staged count S, x0 | x1 << 16), each cell its slot.
+__device__ void task_row_partition(int row, int lane, int *P, int *H,
+                                   const int *__restrict__ count, const CellStarts &cs,
+                                   const TaskGeom &g, int4 *__restrict__ tpiece,
+                                   int4 *__restrict__ tinfo, int *__restrict__ ctask,
+                                   int *__restrict__ rowtasks, int *__restrict__ flags) {
+  if (row >= g.ny * g.nz) return;  // whole wavefronts
+  const int nx = g.nx, cy = row % g.ny, cz = row / g.ny;
+  const int home = task_row_base(g, cy, cz, 4);  // dy = dz = 0
+  int carry_p = 0, carry_h = 0;
+  for (int c0 = 0; c0 < nx; c0 += 64) {
+    const int x = c0 + lane;
+    int col = 0, hm = 0;
+    if (x < nx) {
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const int rb = task_row_base(g, cy, cz, r);
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      int a0, a1;
-      task_piece_cells(g.nx, x0, x1, p, a0, a1);
-      const int j0 = cstart(rb + a0);
-      const int len = a1 > a0 ? cstart(rb + a1) - j0 : 0;
-      tpiece[(size_t)task * kTaskPieces + 2 * r + p] = make_int4(j0, base, a0, a1);
-      base += len;
+      for (int r = 0; r < 9; ++r) col += count[task_row_base(g, cy, cz, r) + x];
+      hm = count[home + x];
     }
+    int ic = col, ih = hm;  // inclusive scans over the wavefront
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int vc = __shfl_up(ic, d, 64), vh = __shfl_up(ih, d, 64);
+      if (lane >= d) {
+        ic += vc;
+        ih += vh;
+      }
+    }
+    if (x < nx) {
+      P[x + 1] = carry_p + ic;
+      H[x + 1] = carry_h + ih;
+    }
+    carry_p += __shfl(ic, 63, 64);
+    carry_h += __shfl(ih, 63, 64);
   }
-  tstage[task] = base;
-  if (base > 65535) atomicOr(&flags[F_STAGE], 1);
+  if (lane == 0) {
+    P[0] = 0;
+    H[0] = 0;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // staged atoms of [x0, x1): x-cells x0-1 .. x1 (periodic) of the 9 rows,
+  // the whole row once when that covers it (task_piece_cells)
+  auto staged = [&](int x0, int x1) {
+    if (x1 - x0 + 2 >= nx) return P[nx];
+    if (x0 == 0) return P[x1 + 1] + (P[nx] - P[nx - 1]);
+    if (x1 == nx) return P[nx] - P[x0 - 1] + P[1];
+    return P[x1 + 1] - P[x0 - 1];
+  };
+  int x0 = 0, t = 0;
+  bool ovf = false;
+  while (x0 < nx) {  // uniform over the wavefront
+    int x1 = x0 + 1;
+    if (t + 1 < g.mpr) {
+      // both limits grow with x1, so the admissible ends are a prefix
+      for (int e0 = x0 + 1; e0 <= nx; e0 += 64) {
+        const int e = e0 + lane;
+        const bool ok = e <= nx && H[e] - H[x0] <= kTaskBlock && staged(x0, e) + 1 <= kStage;
+        const unsigned long long m = __ballot(ok);
+        if (m) x1 = e0 + 63 - __clzll(m);
+        if (m != ~0ull) break;
+      }
+    } else {
+      x1 = nx;
+      ovf = ovf || H[nx] - H[x0] > kTaskBlock || staged(x0, nx) + 1 > kStage;
+    }
+    const int sl = row * g.mpr + t;
+    // pieces: lane 2r + p holds piece p of stencil row r
+    int j0 = 0, len = 0, a0 = 0, a1 = 0;
+    if (lane < kTaskPieces) {
+      const int rb = task_row_base(g, cy, cz, lane >> 1);
+      task_piece_cells(nx, x0, x1, lane & 1, a0, a1);
+      j0 = cs(rb + a0);
+      len = a1 > a0 ? cs(rb + a1) - j0 : 0;
+    }
+    int base = len;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+      const int v = __shfl_up(base, d, 64);
+      if (lane >= d) base += v;
+    }
+    base -= len;  // exclusive
+    if (lane < kTaskPieces) tpiece[(size_t)sl * kTaskPieces + lane] = make_int4(j0, base, a0, a1);
+    if (lane == kTaskPieces - 1) {
+      const int S = base + len;
+      tinfo[sl] = make_int4(cs(home + x0), cs(home + x1), S, x0 | (x1 << 16));
+      if (S > 65535) atomicOr(&flags[F_STAGE], 1);
+    }
+    for (int x = x0 + lane; x < x1; x += 64) ctask[home + x] = sl;
+    x0 = x1;
+    ++t;
+  }
+  if (lane == 0) {
+    rowtasks[row] = t;
+    if (ovf) atomicOr(&flags[F_TASKOVF], 1);
+  }
 }
 
 __global__ void k_build_begin(int *__restrict__ flags, int *__restrict__ cell_count, int ncell,
@@ -1435,7 +1575,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
       cb.cthr_out = bd->cthr_out;
       const int off = e.cell_off[sy];
       st = cell_wave<true, false>(pos, q, cb, bd, cstart + off, nbr, nnbr, stride, cap, c - off, wv,
-                                  lane, lds, nullptr, cnt_sh, TaskGeom{}, nullptr, nullptr);
+                                  lane, lds, nullptr, cnt_sh, nullptr, nullptr, nullptr);
     }
   }
   if (lane == 0) st_sh[wv] = st;
@@ -1552,16 +1692,17 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
     const CellStarts cs{s.cell_local, s.tile_sum, s.cell_start, nc};
     const int nbs = nblk(n, 256), nfix = nblk(nc, 256);
     const bool tasks = coarse && s.slots;
-    TaskGeom tg = s.tg;
-    if (!tasks) tg.ntask = 0;
-    k_scatter<<<nbs + nfix + nblk(tg.ntask, 256), 256, 0, st>>>(
-        s.cell, s.slot, cs, s.cell_start, s.perm, n, s.flags, nbs, nfix, tg, s.tpiece, s.tstage);
-    // accelerations are carried along only when no force pass follows
-    k_rank_move<<<nblk(n, 256), 256, 0, st>>>(s.cell, s.cell_start, s.perm, n, s.pos, s.pos_alt,
-                                              s.pos_ref, s.vel, s.vel_alt,
-                                              keep_acc ? s.acc : nullptr, s.acc_alt, s.tag,
-                                              s.tag_alt, s.id, s.id_alt, s.cell_count, nc,
-                                              (s.write_q || b.coarse) ? s.q : nullptr, b, s.flags);
+    const int rows = tasks ? s.tg.ny * s.tg.nz : 0;
+    k_scatter<<<nbs + nfix + nblk(rows, kRowWaves), 256, 0, st>>>(
+        s.cell, s.slot, cs, s.cell_start, s.perm, n, s.flags, nbs, nfix, s.tg, s.cell_count,
+        s.tpiece, s.tinfo, s.ctask, s.rowtasks);
+    // accelerations are carried along only when no force pass follows; the
+    // extra last work-groups (tasks) copy the task tables to compact order
+    const TaskMapArgs tm{s.rowtasks, s.tinfo, s.tpiece, s.ctinfo, s.ctpiece, rows, s.tg.mpr};
+    k_rank_move<<<nblk(n, 256) + (tasks ? kCompactParts : 0), 256, 0, st>>>(
+        s.cell, s.cell_start, s.perm, n, s.pos, s.pos_alt, s.pos_ref, s.vel, s.vel_alt,
+        keep_acc ? s.acc : nullptr, s.acc_alt, s.tag, s.tag_alt, s.id, s.id_alt, s.cell_count, nc,
+        (s.write_q || b.coarse) ? s.q : nullptr, b, s.flags, tm);
     // The host cannot know whether the device ran the build, so the new
     // order is copied back instead of swapping buffer pointers.
     if (!copy_in_list) k_copy_back<<<nblk(n, 256), 256, 0, st>>>(cpb, s.flags);
@@ -1591,7 +1732,7 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
   if (b.ghost_ok == G && s.slots == S)                                                           \
     k_nlist_cells<G, S><<<ncopy + grid, 256, 0, st>>>(pos_new, s.q, cb, s.box_dev, s.cell_start, \
                                                       s.nbr, s.nnbr, s.stride, s.cap, s.flags,   \
-                                                      real, s.wstat, s.tg, s.tpiece, s.tstage,   \
+                                                      real, s.wstat, s.ctask, s.tpiece, s.tinfo, \
                                                       ncopy, cpb);
     CV2_NLC(true, true)
     CV2_NLC(true, false)
@@ -1609,19 +1750,18 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
                                                                            s.cap);
 }
 
-TaskGeom task_geom(int n, const BoxDev &b) {
+TaskGeom task_geom(int n, const BoxDev &b, int min_mpr) {
   TaskGeom g{};
   g.nx = b.nc[0];
   g.ny = b.nc[1];
   g.nz = b.nc[2];
-  // about 224 home atoms per task (256 threads), with the staged
-  // neighbourhood, 9 (T + 2) cells of the mean density, within 95% of the
-  // LDS stage (denser tasks read their neighbours from q)
-  const double rho = (double)n / ((double)g.nx * g.ny * g.nz);
-  double T = rho > 0.0 ? 224.0 / rho : (double)g.nx;
-  while (T > 1.0 && 9.0 * (T + 2.0) * rho > 0.95 * kStage) T -= 0.25;
-  g.tpr = std::max(1, std::min(g.nx, (int)std::lround(g.nx / std::max(T, 1.0))));
-  g.ntask = g.ny * g.nz * g.tpr;
+  // a row of the mean density needs about row / 230 tasks (at most
+  // kTaskBlock home atoms, neighbourhood within the LDS stage); rows of a
+  // lattice or a dense region need more, so the slots allow 1.5x that + 2
+  const double row = (double)n / ((double)g.ny * g.nz);
+  const int want = (int)std::ceil(1.5 * row / 230.0) + 2;
+  g.mpr = std::max(1, std::min(g.nx, std::max(min_mpr, want)));
+  g.nslot = g.ny * g.nz * g.mpr;
   return g;
 }
 

@@ -54,13 +54,15 @@ __global__ void __launch_bounds__(kBlock)
 // Displacement check alone (Neighborlist::update_is_needed), owned atoms
 // and host-halo ghosts.
 __global__ void __launch_bounds__(kBlock)
-    k_check_disp(const double4 *__restrict__ pos, const double4 *__restrict__ pos_ref, int ntot,
-                 double thresh_sq, int *__restrict__ flags) {
+    k_check_disp(const double4 *__restrict__ pos, const double4 *__restrict__ pos_ref, int n,
+                 int ntot, StepParams p, int *__restrict__ flags) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= ntot) return;
   const double4 x = pos[i], r = pos_ref[i];
   const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
-  if (dx * dx + dy * dy + dz * dz > thresh_sq) atomicOr(&flags[F_REBUILD], 1);
+  // the wrap / ghost-band triggers concern owned atoms
+  if (dx * dx + dy * dy + dz * dz > p.thresh_sq || (p.ref_rebuild && i < n && ref_trigger(x, r, p)))
+    atomicOr(&flags[F_REBUILD], 1);
 }
 
 // Per-block sums of (m v.v, m vx, m vy, m vz, m).
@@ -183,7 +185,14 @@ StepParams step_params(const BoxDev &b, const UpdateArgs &u) {
   for (int d = 0; d < 3; ++d) {
     p.lo[d] = b.lo[d];
     p.qscale[d] = b.qscale[d];
+    const bool per = b.bc[d] == 1;
+    const double inf = __builtin_inf();
+    p.wlo[d] = per ? b.lo[d] : -inf;
+    p.whi[d] = per ? b.hi[d] : inf;
+    p.blo[d] = per ? b.lo[d] + b.ghost_cut : -inf;
+    p.bhi[d] = per ? b.hi[d] - b.ghost_cut : inf;
   }
+  p.ref_rebuild = u.ref_rebuild ? 1 : 0;
   return p;
 }
 
@@ -236,10 +245,13 @@ void launch_scale_vel(DevState &s, double lambda, hipStream_t st) {
   k_scale_vel<<<grid, kBlock, 0, st>>>(s.vel, m, lambda);
 }
 
-void launch_check_disp(DevState &s, const BoxDev &b, hipStream_t st) {
+void launch_check_disp(DevState &s, const BoxDev &b, bool ref_rebuild, hipStream_t st) {
   const int ntot = s.n + s.ng;
   if (ntot == 0) return;
-  k_check_disp<<<nblk(ntot, kBlock), kBlock, 0, st>>>(s.pos, s.pos_ref, ntot, b.thresh_sq, s.flags);
+  UpdateArgs u{};
+  u.ref_rebuild = ref_rebuild;
+  k_check_disp<<<nblk(ntot, kBlock), kBlock, 0, st>>>(s.pos, s.pos_ref, s.n, ntot, step_params(b, u),
+                                                      s.flags);
 }
 
 // Writes (sum m v^2, sum m v (3), sum m) to reduce_out[2..6].

@@ -336,8 +336,8 @@ __global__ void __launch_bounds__(kBlock)
 // The lists hold no self entries, so the pair arithmetic needs no r^2 > 0
 // test and runs without branches: a pair beyond the cutoff adds a zero
 // force, every entry (padding included) adds one magic bias.
-constexpr int kTaskBlock = 256;
 constexpr int kSortBins = 64;  // list lengths in groups of 8 (longer lists share the last bin)
+constexpr int kTailGrid = 64;  // work-groups of the tail launch (tasks beyond the main grid)
 
 __device__ inline u32x4 load_group8(const uint32_t *__restrict__ nbr, size_t stride, int i, int k) {
   return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(nbr) + (size_t)(k >> 3) * stride + i);
@@ -452,16 +452,25 @@ struct FuseArgs {
   int langevin;
 };
 
+// One work-group per force task, in the compact order of the build
+// (ctinfo / ctpiece, flags[F_NTASK] of them).  Work-groups are dealt
+// round-robin to the 8 XCDs: XCD x takes the x-th eighth of the tasks
+// (spatially contiguous rows, one L2 working set).  The main launch
+// (TAIL = false, main_per work-groups per XCD) runs one task per work-group;
+// a build with more tasks than that leaves the rest of each eighth to the
+// tail launch (TAIL), whose few work-groups loop over them (and otherwise
+// return at once).  A loop in the main launch would keep the pair constants
+// live across the integrator epilogue: it measured 8% slower.
 // FUSE: five wavefronts per SIMD (96 registers); the LDS stage allows five
 // blocks per CU
-template <bool MULTI, bool ENERGY, bool ISO, bool FUSE>
+template <bool MULTI, bool ENERGY, bool ISO, bool FUSE, bool TAIL>
 __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu(FUSE ? 5 : 1)))
     k_lj_f32_task(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
-                  const int *__restrict__ nnbr, int stride, int n, const int *__restrict__ cstart,
-                  TaskGeom tg, const int4 *__restrict__ tpiece, const int *__restrict__ tstage,
+                  const int *__restrict__ nnbr, int stride, int n, const int4 *__restrict__ tinfo,
+                  const int4 *__restrict__ tpiece,
                   int stage_limit, const LJDev *__restrict__ lj, BoxDev b,
                   double *__restrict__ acc, long long *__restrict__ partials,
-                  int *__restrict__ flags, int nblk_real, FuseArgs fz) {
+                  int *__restrict__ flags, FuseArgs fz, int main_per) {
   static_assert(!(MULTI && ISO), "species tables are in length units");
   static_assert(!(FUSE && ENERGY), "the energy pass is not fused");
   __shared__ __attribute__((aligned(16))) int sx[kStage];
@@ -472,82 +481,15 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
   __shared__ long long sh[kTaskBlock / 64];
   __shared__ int lhist[kSortBins + 1];
   __shared__ unsigned char lorder[kTaskBlock];
-  const int task = xcd_block(nblk_real);
-  if (task < 0) return;
-  if (threadIdx.x < kTaskPieces) pc[threadIdx.x] = tpiece[(size_t)task * kTaskPieces + threadIdx.x];
-  const int S = tstage[task];
-  const bool staged = S + 1 <= stage_limit;  // uniform over the block (sentinel included)
-  const int t = task % tg.tpr, row = task / tg.tpr;
-  const int h0 = cstart[row * tg.nx + task_x0(tg, t)];
-  const int h1 = cstart[row * tg.nx + task_x0(tg, t + 1)];
-  // the sentinel, in LDS (read by the staging and the unstaged path only)
-  __shared__ int4 sent;
-  if (threadIdx.x == 0) {
-    int4 v = make_int4(0, 0, 0, 0);
-    if (h0 < h1) {
-      v = q[h0];
-      v.y = (int)((unsigned)v.y + 0x80000000u);
-      v.z = (int)((unsigned)v.z + 0x80000000u);
-    }
-    sent = v;
-  }
-  __syncthreads();
-  if (staged) {
-    // slots threadIdx.x + k * kTaskBlock; their pieces are non-decreasing in k
-    constexpr int K = kStage / kTaskBlock;
-    int jj[K];
-    int p = 0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int sl = k * kTaskBlock + (int)threadIdx.x;
-      while (p + 1 < kTaskPieces && pc[p + 1].y <= sl) ++p;
-      jj[k] = pc[p].x + (sl - pc[p].y);
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int sl = k * kTaskBlock + (int)threadIdx.x;
-      if (sl <= S) {
-        const int4 v = sl < S ? q[jj[k]] : sent;
-        sx[sl] = v.x;
-        sy[sl] = v.y;
-        sz[sl] = v.z;
-        if (MULTI) stype[MULTI ? sl : 0] = (unsigned char)(v.w & 7);
-        if (ENERGY) sj[ENERGY ? sl : 0] = sl < S ? jj[k] : 0x7fffffff;
-      }
-    }
-  }
-  // A wavefront runs until its longest list is done: give each wavefront
-  // home atoms of similar list length (counting sort on the group count,
-  // longest first).  Which thread computes an atom does not change its
-  // result.
-  const int nh = h1 - h0;
-  const bool sorted = nh <= kTaskBlock;
-  if (sorted) {
-    for (int k = threadIdx.x; k <= kSortBins; k += kTaskBlock) lhist[k] = 0;
-    __syncthreads();
-    int bin = 0, rank = 0;
-    if ((int)threadIdx.x < nh) {
-      bin = kSortBins - 1 - min((nnbr[h0 + threadIdx.x] + 7) >> 3, kSortBins - 1);
-      rank = atomicAdd(&lhist[bin + 1], 1);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-      for (int k = 1; k <= kSortBins; ++k) lhist[k] += lhist[k - 1];
-    __syncthreads();
-    if ((int)threadIdx.x < nh) lorder[lhist[bin] + rank] = (unsigned char)threadIdx.x;
-  }
-  __syncthreads();
-  const double u0 = b.qinv[0], u1 = b.qinv[1], u2 = b.qinv[2];
-  const PairF32 c0 = pair_f32(lj, 0, ISO ? u0 : 1.0);
-  const double fc_rc = sqrt(c0.rc2);  // cutoff in the units of the separations
-  const int nt = MULTI ? lj->nt : 1;
+  __shared__ int4 sent;  // the sentinel (read by the staging and the unstaged path only)
+  const int ntask = flags[F_NTASK];
+  const int xcd = blockIdx.x & 7;  // grids: multiples of 8
+  const int k0 = (int)((long long)ntask * xcd / 8), k1 = (int)((long long)ntask * (xcd + 1) / 8);
   long long vir = 0, pe = 0;
   // FUSE: accelerations of the (at most kTaskBlock) home atoms, in the
   // staging area once every neighbour has been read
   double *at_sh = reinterpret_cast<double *>(sx);
   static_assert(3 * kTaskBlock * sizeof(double) <= sizeof(int) * kStage, "staging area too small");
-  double my_a[3] = {0.0, 0.0, 0.0};
-  int my_t = -1;
   auto fuse_step = [&](int ia, double4 x, double v[3], const double4 &r, int idv, const double *aa) {
     if (fz.langevin)
       integrate_values<true, true, true, true>(x, v, idv, aa, fz.p);
@@ -556,69 +498,150 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
     integrate_store<true, true>(ia, n, x, v, r, fz.pos, fz.vel, fz.q_out, fz.p, flags);
   };
 
-  for (int t = (int)threadIdx.x; t < nh; t += kTaskBlock) {
-    const int i = h0 + (sorted ? (int)lorder[t] : t);
-    const int4 qi = q[i];
-    const int ti = MULTI ? (qi.w & 7) : 0;
-    const int cp = (nnbr[i] + 7) & ~7;
-    AccF32 a;
-    float fcmax = 0.0f;
-    if (staged)
-      task_atom<MULTI, ENERGY, ISO, true>(q, nbr, stride, i, cp, qi, ti, nt, lj, c0, u0, u1, u2, sx,
-                                          sy, sz, stype, sj, pc, S, sent, a, fcmax);
-    else
-      task_atom<MULTI, ENERGY, ISO, false>(q, nbr, stride, i, cp, qi, ti, nt, lj, c0, u0, u1, u2,
-                                           sx, sy, sz, stype, sj, pc, S, sent, a, fcmax);
-    // the flat form biases every entry, the branched form counts its terms
-    const int terms = (ENERGY || MULTI) ? a.terms : cp;
-    const bool ovf = (ENERGY || MULTI) ? a.ovf : !((double)fcmax * fc_rc < 0.9 * 2251799813685248.0);
-    const double mi = lj->mass_inv[ti] * (1.0 / kForceScale);
-    const double ai[3] = {fx_value(a.sx, terms) * mi, fx_value(a.sy, terms) * mi,
-                          fx_value(a.sz, terms) * mi};
-    if (FUSE && sorted) {
-      // one atom per thread: handed back in atom order through LDS (below)
-      my_a[0] = ai[0];
-      my_a[1] = ai[1];
-      my_a[2] = ai[2];
-      my_t = i - h0;
-    } else if (FUSE) {
-      double v[3] = {fz.vel[i], fz.vel[n + i], fz.vel[2 * n + i]};
-      fuse_step(i, fz.pos[i], v, fz.pos_ref[i], fz.langevin ? fz.id[i] : 0, ai);
-    } else {
-      acc[i] = ai[0];
-      acc[n + i] = ai[1];
-      acc[2 * n + i] = ai[2];
-    }
-    if (ovf) atomicOr(&flags[F_FXOVF], 1);
-    vir += a.vir;
-    pe += a.pe;
-  }
-  if (FUSE && sorted) {
-    // the integrator in atom order: coalesced position / velocity traffic
-    __syncthreads();  // every neighbour read of the staging area is done
-    if (my_t >= 0) {
-      at_sh[3 * my_t] = my_a[0];
-      at_sh[3 * my_t + 1] = my_a[1];
-      at_sh[3 * my_t + 2] = my_a[2];
+  auto run_task = [&](int k, double u0, double u1, double u2, const PairF32 &c0, double fc_rc, int nt) {
+    if (threadIdx.x < kTaskPieces) pc[threadIdx.x] = tpiece[(size_t)k * kTaskPieces + threadIdx.x];
+    const int4 ti4 = tinfo[k];
+    const int h0 = ti4.x, h1 = ti4.y, S = ti4.z;
+    const bool staged = S + 1 <= stage_limit;  // uniform over the block (sentinel included)
+    if (threadIdx.x == 0) {
+      int4 v = make_int4(0, 0, 0, 0);
+      if (h0 < h1) {
+        v = q[h0];
+        v.y = (int)((unsigned)v.y + 0x80000000u);
+        v.z = (int)((unsigned)v.z + 0x80000000u);
+      }
+      sent = v;
     }
     __syncthreads();
-    // (the integrator's loads are issued here rather than before the force
-    // loop: held through it they cost a wavefront per SIMD of occupancy)
-    const int t = (int)threadIdx.x;
-    if (t < nh) {
-      const int ia = h0 + t;
-      double v[3] = {fz.vel[ia], fz.vel[n + ia], fz.vel[2 * n + ia]};
-      const double ai[3] = {at_sh[3 * t], at_sh[3 * t + 1], at_sh[3 * t + 2]};
-      fuse_step(ia, fz.pos[ia], v, fz.pos_ref[ia], fz.langevin ? fz.id[ia] : 0, ai);
+    if (staged) {
+      // slots threadIdx.x + k * kTaskBlock; their pieces are non-decreasing in k
+      constexpr int K = kStage / kTaskBlock;
+      int jj[K];
+      int p = 0;
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const int sl = kk * kTaskBlock + (int)threadIdx.x;
+        while (p + 1 < kTaskPieces && pc[p + 1].y <= sl) ++p;
+        jj[kk] = pc[p].x + (sl - pc[p].y);
+      }
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const int sl = kk * kTaskBlock + (int)threadIdx.x;
+        if (sl <= S) {
+          const int4 v = sl < S ? q[jj[kk]] : sent;
+          sx[sl] = v.x;
+          sy[sl] = v.y;
+          sz[sl] = v.z;
+          if (MULTI) stype[MULTI ? sl : 0] = (unsigned char)(v.w & 7);
+          if (ENERGY) sj[ENERGY ? sl : 0] = sl < S ? jj[kk] : 0x7fffffff;
+        }
+      }
+    }
+    // A wavefront runs until its longest list is done: give each wavefront
+    // home atoms of similar list length (counting sort on the group count,
+    // longest first).  Which thread computes an atom does not change its
+    // result.
+    const int nh = h1 - h0;
+    const bool sorted = nh <= kTaskBlock;
+    if (sorted) {
+      for (int kk = threadIdx.x; kk <= kSortBins; kk += kTaskBlock) lhist[kk] = 0;
+      __syncthreads();
+      int bin = 0, rank = 0;
+      if ((int)threadIdx.x < nh) {
+        bin = kSortBins - 1 - min((nnbr[h0 + threadIdx.x] + 7) >> 3, kSortBins - 1);
+        rank = atomicAdd(&lhist[bin + 1], 1);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0)
+        for (int kk = 1; kk <= kSortBins; ++kk) lhist[kk] += lhist[kk - 1];
+      __syncthreads();
+      if ((int)threadIdx.x < nh) lorder[lhist[bin] + rank] = (unsigned char)threadIdx.x;
+    }
+    __syncthreads();
+    double my_a[3] = {0.0, 0.0, 0.0};
+    int my_t = -1;
+    for (int t = (int)threadIdx.x; t < nh; t += kTaskBlock) {
+      const int i = h0 + (sorted ? (int)lorder[t] : t);
+      const int4 qi = q[i];
+      const int ti = MULTI ? (qi.w & 7) : 0;
+      const int cp = (nnbr[i] + 7) & ~7;
+      AccF32 a;
+      float fcmax = 0.0f;
+      if (staged)
+        task_atom<MULTI, ENERGY, ISO, true>(q, nbr, stride, i, cp, qi, ti, nt, lj, c0, u0, u1, u2, sx,
+                                            sy, sz, stype, sj, pc, S, sent, a, fcmax);
+      else
+        task_atom<MULTI, ENERGY, ISO, false>(q, nbr, stride, i, cp, qi, ti, nt, lj, c0, u0, u1, u2,
+                                             sx, sy, sz, stype, sj, pc, S, sent, a, fcmax);
+      // the flat form biases every entry, the branched form counts its terms
+      const int terms = (ENERGY || MULTI) ? a.terms : cp;
+      const bool ovf = (ENERGY || MULTI) ? a.ovf : !((double)fcmax * fc_rc < 0.9 * 2251799813685248.0);
+      const double mi = lj->mass_inv[ti] * (1.0 / kForceScale);
+      const double ai[3] = {fx_value(a.sx, terms) * mi, fx_value(a.sy, terms) * mi,
+                            fx_value(a.sz, terms) * mi};
+      if (FUSE && sorted) {
+        // one atom per thread: handed back in atom order through LDS (below)
+        my_a[0] = ai[0];
+        my_a[1] = ai[1];
+        my_a[2] = ai[2];
+        my_t = i - h0;
+      } else if (FUSE) {
+        double v[3] = {fz.vel[i], fz.vel[n + i], fz.vel[2 * n + i]};
+        fuse_step(i, fz.pos[i], v, fz.pos_ref[i], fz.langevin ? fz.id[i] : 0, ai);
+      } else {
+        acc[i] = ai[0];
+        acc[n + i] = ai[1];
+        acc[2 * n + i] = ai[2];
+      }
+      if (ovf) atomicOr(&flags[F_FXOVF], 1);
+      vir += a.vir;
+      pe += a.pe;
+    }
+    if (FUSE && sorted) {
+      // the integrator in atom order: coalesced position / velocity traffic
+      __syncthreads();  // every neighbour read of the staging area is done
+      if (my_t >= 0) {
+        at_sh[3 * my_t] = my_a[0];
+        at_sh[3 * my_t + 1] = my_a[1];
+        at_sh[3 * my_t + 2] = my_a[2];
+      }
+      __syncthreads();
+      // (the integrator's loads are issued here rather than before the force
+      // loop: held through it they cost a wavefront per SIMD of occupancy)
+      const int t = (int)threadIdx.x;
+      if (t < nh) {
+        const int ia = h0 + t;
+        double v[3] = {fz.vel[ia], fz.vel[n + ia], fz.vel[2 * n + ia]};
+        const double ai[3] = {at_sh[3 * t], at_sh[3 * t + 1], at_sh[3 * t + 2]};
+        fuse_step(ia, fz.pos[ia], v, fz.pos_ref[ia], fz.langevin ? fz.id[ia] : 0, ai);
+      }
+    }
+  };
+
+  if (!TAIL) {
+    const double u0 = b.qinv[0], u1 = b.qinv[1], u2 = b.qinv[2];
+    const PairF32 c0 = pair_f32(lj, 0, ISO ? u0 : 1.0);
+    const double fc_rc = sqrt(c0.rc2);  // cutoff in the units of the separations
+    const int k = k0 + (int)(blockIdx.x >> 3);
+    if (k < min(k1, k0 + main_per)) run_task(k, u0, u1, u2, c0, fc_rc, MULTI ? lj->nt : 1);
+  } else {
+    for (int k = k0 + main_per + (int)(blockIdx.x >> 3); k < k1; k += (int)(gridDim.x >> 3)) {
+      // the constants are formed per task (not hoisted out of the loop)
+      __asm__ volatile("" ::: "memory");
+      const double u0 = b.qinv[0], u1 = b.qinv[1], u2 = b.qinv[2];
+      const PairF32 c0 = pair_f32(lj, 0, ISO ? u0 : 1.0);
+      run_task(k, u0, u1, u2, c0, sqrt(c0.rc2), MULTI ? lj->nt : 1);
+      __syncthreads();  // the LDS of this task is reused by the next
     }
   }
   if (ENERGY) {
     const long long v = block_sum_i64(vir, sh);
     __syncthreads();
     const long long u = block_sum_i64(pe, sh);
-    if (threadIdx.x == 0) {  // per-block partials, summed by k_reduce_i64
-      partials[2 * task] = v;
-      partials[2 * task + 1] = u;
+    if (threadIdx.x == 0) {  // per-block partials (tail: after the main launch's), summed by k_reduce_i64
+      const int blk = TAIL ? 8 * main_per + (int)blockIdx.x : (int)blockIdx.x;
+      partials[2 * blk] = v;
+      partials[2 * blk + 1] = u;
     }
   }
 }
@@ -741,17 +764,22 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
   } else {
     long long *part = reinterpret_cast<long long *>(s.partials);
     if (s.slots) {
-      const int treal = s.tg.ntask, tgrid = ((treal + 7) / 8) * 8;
-#define CV2_LJT(I)                                                                             \
-  k_lj_f32_task<MULTI, ENERGY, I, false><<<tgrid, kTaskBlock, 0, st>>>(                        \
-      s.q, s.nbr, s.nnbr, s.stride, s.n, s.cell_start, s.tg, s.tpiece, s.tstage, s.stage_limit, \
-      lj_dev, b, s.acc, part, s.flags, treal, FuseArgs{})
+      const int tgrid = s.task_grid;
+#define CV2_LJT(I)                                                                              \
+  do {                                                                                           \
+  k_lj_f32_task<MULTI, ENERGY, I, false, false><<<tgrid, kTaskBlock, 0, st>>>(                   \
+      s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,    \
+      part, s.flags, FuseArgs{}, tgrid / 8);                                                     \
+  k_lj_f32_task<MULTI, ENERGY, I, false, true><<<kTailGrid, kTaskBlock, 0, st>>>(                \
+      s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,    \
+      part, s.flags, FuseArgs{}, tgrid / 8);                                                     \
+  } while (0)
       if (!MULTI && b.qiso)
         CV2_LJT(!MULTI);
       else
         CV2_LJT(false);
 #undef CV2_LJT
-      if (ENERGY) k_reduce_i64<<<1, 1024, 0, st>>>(part, treal, s.fx_acc);
+      if (ENERGY) k_reduce_i64<<<1, 1024, 0, st>>>(part, tgrid + kTailGrid, s.fx_acc);
       return;
     }
     if (!MULTI && b.qiso)
@@ -807,11 +835,16 @@ bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, cons
   if (!s.slots || s.n == 0 || s.ng > 0 || !s.q_alt || !u.finish || !u.start) return false;
   FuseArgs fz{s.pos, s.pos_ref, s.vel, s.id, s.q_alt, step_params(b, u), u.langevin ? 1 : 0};
   long long *part = reinterpret_cast<long long *>(s.partials);
-  const int treal = s.tg.ntask, tgrid = ((treal + 7) / 8) * 8;
-#define CV2_LJF(M, I)                                                                          \
-  k_lj_f32_task<M, false, I, true><<<tgrid, kTaskBlock, 0, st>>>(                              \
-      s.q, s.nbr, s.nnbr, s.stride, s.n, s.cell_start, s.tg, s.tpiece, s.tstage, s.stage_limit, \
-      lj_dev, b, s.acc, part, s.flags, treal, fz)
+  const int tgrid = s.task_grid;
+#define CV2_LJF(M, I)                                                                           \
+  do {                                                                                           \
+  k_lj_f32_task<M, false, I, true, false><<<tgrid, kTaskBlock, 0, st>>>(                          \
+      s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,    \
+      part, s.flags, fz, tgrid / 8);                                                             \
+  k_lj_f32_task<M, false, I, true, true><<<kTailGrid, kTaskBlock, 0, st>>>(                       \
+      s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,    \
+      part, s.flags, fz, tgrid / 8);                                                             \
+  } while (0)
   if (lj.nt > 1 || lj.jump_tol > 0.0)
     CV2_LJF(true, false);
   else if (b.qiso)

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libcv2md.so")
 CV2_PREC_FP64 = 0
 CV2_PREC_FP32_FIXED = 1
 CV2_NL_REBUILD_EVERY_STEP = 1
+CV2_NL_REFERENCE_REBUILDS = 2
 
 ERRORS = {1: "HIP", 2: "BAD_ARG", 3: "NAN", 4: "FIXED_OVERFLOW", 5: "CAPACITY", 6: "STATE"}
 
@@ -28,7 +29,7 @@ class Stats(C.Structure):
     _fields_ = [("natoms", C.c_int64), ("nghost", C.c_int64), ("steps", C.c_int64),
                 ("rebuilds", C.c_int64), ("list_entries", C.c_int64), ("max_neighbors", C.c_int32),
                 ("capacity", C.c_int32), ("ncell", C.c_int32 * 3), ("precision", C.c_int32),
-                ("build_kind", C.c_int32)]
+                ("build_kind", C.c_int32), ("force_tasks", C.c_int32)]
 
 
 class Timing(C.Structure):
@@ -86,6 +87,7 @@ def lib():
             "cv2_halo_unpack": ([vp, C.c_int32, _i64, _d, C.c_int32], C.c_int),
             "cv2_build_neighbors": ([vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
             "cv2_compute_forces": ([vp], C.c_int),
+            "cv2_compute_accelerations": ([vp], C.c_int),
             "cv2_run_vv": ([vp, C.c_int64, C.c_double], C.c_int),
             "cv2_run_langevin": ([vp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint64], C.c_int),
             "cv2_thermo": ([vp, C.c_int, _d, _d, _d], C.c_int),
@@ -182,11 +184,14 @@ class Context:
         self._chk(self.L.cv2_set_ghosts(self.h, len(p), _p(p, _d), _p(t, _i32), _p(i, _i64)))
 
     @classmethod
-    def from_sysdef(cls, s: dict, precision=CV2_PREC_FP64, device=0, rebuild_every_step=False):
+    def from_sysdef(cls, s: dict, precision=CV2_PREC_FP64, device=0, rebuild_every_step=False,
+                    reference_rebuilds=False):
         c = cls(device)
         c.set_box(s["lo"], s["hi"], s.get("bc", (1, 1, 1)), s["ghost_cutoff"])
         c.set_lj(s["eps"], s["sigma"], s["lj_cutoff"], s["mass"], s.get("cutoff_list"))
-        c.set_neighbor(s["nl_cutoff"], s["nl_coef"], CV2_NL_REBUILD_EVERY_STEP if rebuild_every_step else 0)
+        flags = (CV2_NL_REBUILD_EVERY_STEP if rebuild_every_step else 0) | \
+            (CV2_NL_REFERENCE_REBUILDS if reference_rebuilds else 0)
+        c.set_neighbor(s["nl_cutoff"], s["nl_coef"], flags)
         c.set_precision(precision)
         c.upload(s["pos"], s.get("vel"), s.get("type"))
         return c
@@ -226,6 +231,10 @@ class Context:
     def compute_forces(self):
         self._chk(self.L.cv2_compute_forces(self.h))
 
+    def compute_accelerations(self):
+        """Accelerations through the kernel the step loop runs (no energy pass)."""
+        self._chk(self.L.cv2_compute_accelerations(self.h))
+
     def run_vv(self, nsteps, dt):
         self._chk(self.L.cv2_run_vv(self.h, int(nsteps), float(dt)))
 
@@ -260,7 +269,8 @@ class Context:
         return dict(natoms=s.natoms, nghost=s.nghost, steps=s.steps, rebuilds=s.rebuilds,
                     list_entries=s.list_entries, max_neighbors=s.max_neighbors, capacity=s.capacity,
                     ncell=tuple(s.ncell), precision=s.precision,
-                    build_kind=('cell_wave' if s.build_kind == 1 else 'per_atom'))
+                    build_kind=('cell_wave' if s.build_kind == 1 else 'per_atom'),
+                    force_tasks=s.force_tasks)
 
     def set_timing(self, enable=True, period=1):
         """HIP-event timing: every kernel group (period 1), or only the force

@@ -35,6 +35,15 @@ extern "C" {
 
 /* neighbour-list flags (cv2_set_neighbor) */
 #define CV2_NL_REBUILD_EVERY_STEP 1u /* Neighborlist::set_rebuild_test (neighborlist.cpp:107-110) */
+/* Rebuild also where the reference's single-domain step does besides the
+ * half-skin test: when an owned atom leaves the box through a periodic face
+ * (exchange_owned_single_md_domain.cpp:48-56 -> md_simulator.cpp:425-431)
+ * and when the set of single-domain ghosts changes because an atom entered
+ * or left the ghost band of a face (count or order change,
+ * neighborlist.cpp:117-120,139-144).  Off by default: on large boxes some
+ * atom crosses a face on almost every step, so the reference rebuilds on
+ * almost every step; the pair set within the cutoff does not depend on it. */
+#define CV2_NL_REFERENCE_REBUILDS 2u
 
 typedef struct cv2_ctx cv2_ctx;
 
@@ -48,6 +57,7 @@ typedef struct {
   int32_t ncell[3];         /* cell grid */
   int32_t precision;
   int32_t build_kind;       /* CV2_BUILD_CELL_WAVE or CV2_BUILD_PER_ATOM */
+  int32_t force_tasks;      /* force tasks of the current build (FP32 form, wave-build boxes), else 0 */
 } cv2_stats;
 
 /* list build in use (cv2_stats.build_kind): one wavefront per cell with a
@@ -156,6 +166,14 @@ int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt);
  * (atom_data.cpp:424-432, lj.cpp:393-517). */
 int cv2_compute_forces(cv2_ctx *ctx);
 
+/* Accelerations only, through the same force kernel the step loop runs
+ * (for CV2_PREC_FP32_FIXED on wave-build boxes: the branchless pair
+ * arithmetic with the cutoff tested on r^-2 in FP32); the potential energy
+ * and virial that cv2_thermo reports stay those of the last
+ * cv2_compute_forces.  Same reference counterpart as cv2_compute_forces
+ * (lj.cpp:393-517 without the virial sum of :487-490,516). */
+int cv2_compute_accelerations(cv2_ctx *ctx);
+
 /* nsteps of velocity Verlet.  Md_simulator::integrate_velocity_verlet
  * (md_simulator.cpp:617-729) including re_calculate_acc (:422-437).  The
  * state must be consistent (forces computed) on entry; it is on exit. */
@@ -179,7 +197,9 @@ int cv2_thermo(cv2_ctx *ctx, int n_r_df, double *ke, double *pe, double *virial)
  * Acts on the state between runs (velocities of the last completed step). */
 int cv2_scale_velocities(cv2_ctx *ctx, double lambda);
 
-/* Copy state back in upload order (AoS xyz; any pointer may be NULL). */
+/* Copy state back in upload order (AoS xyz; any pointer may be NULL).
+ * Positions are wrapped into [lo, hi] on periodic axes with the single
+ * image shift of Domain::fix_position (domain.cpp:443-493). */
 int cv2_download(cv2_ctx *ctx, double *pos, double *vel, double *acc);
 
 /* The current Verlet list as pairs in the reference's half-list convention

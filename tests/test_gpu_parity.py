@@ -176,6 +176,34 @@ def test_fp32_unstaged_tasks_identical(name, mid, monkeypatch):
         assert out[lim][1:] == out["3072"][1:]
 
 
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_task_tail_launch_identical(fuse, monkeypatch):
+    """Force tasks beyond the main launch's grid go to the tail launch
+    (CV2_TASK_GRID caps the grid at 8 work-groups, one per XCD: all but 8 of
+    the 4000-atom liquid's tasks run there); trajectories, energy and virial carry the
+    same bits as with the full grid, across rebuilds."""
+    s = perturbed(lj_fluid(10, temperature=1.44, seed=14), 0.05, 14)
+    monkeypatch.setenv("CV2_FUSE", fuse)
+    out = {}
+    for cap in ("0", "8"):
+        monkeypatch.setenv("CV2_TASK_GRID", cap)
+        print(f"[tail] fuse={fuse} cap={cap}", flush=True)
+        with D.Context.from_sysdef(s, precision=D.CV2_PREC_FP32_FIXED) as c:
+            c.compute_forces()
+            c.run_langevin(60, 0.005, 1.0, 1.44, 5)
+            c.compute_forces()
+            th = c.thermo()
+            st = c.stats()
+            out[cap] = (c.download(), th, st["rebuilds"], st["force_tasks"])
+    monkeypatch.delenv("CV2_TASK_GRID")
+    monkeypatch.delenv("CV2_FUSE")
+    (p1, v1, a1), th1, r1, nt1 = out["0"]
+    (p0, v0, a0), th0, r0, nt0 = out["8"]
+    assert nt1 == nt0 and nt1 > 8 and r1 == r0 and r1 > 1
+    assert np.array_equal(p1, p0) and np.array_equal(v1, v0) and np.array_equal(a1, a0)
+    assert th1 == th0
+
+
 @pytest.mark.parametrize("langevin", [False, True])
 def test_fused_integrator_identical(langevin, monkeypatch):
     """The FP32 task kernel with the integrator in its epilogue gives the
