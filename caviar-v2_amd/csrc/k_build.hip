@@ -241,47 +241,65 @@ __device__ inline int4 qpack(const double4 &p, const BoxDev &b) {
 // The force tasks of a build in compact order: the used task slots of all
 // x-rows, in row order, copied to ctinfo / ctpiece (what the force kernel
 // walks: one level of indirection less than a map of slot numbers); their
-// number goes to flags[F_NTASK].  kCompactParts extra work-groups of
-// k_rank_move (256 threads each): every one scans the per-row task counts
-// up to the end of its share of the rows and copies that share.
-constexpr int kCompactParts = 32;
+// number goes to flags[F_NTASK].  tm.parts work-groups of k_rank_move
+// (256 threads, the first ones of its grid so that they overlap the atom
+// moves), each owning a contiguous share of the rows: the tasks of the rows
+// before its share give its base (a strided sum), a scan over its share
+// gives each row's first compact index, and all threads then copy the
+// share's task records.
 struct TaskMapArgs {
   const int *rowtasks;
   const int4 *tinfo, *tpiece;  // by slot
   int4 *ctinfo, *ctpiece;      // compact
   int rows, mpr;               // rows = 0: no force tasks
+  int parts;                   // work-groups; shares of at most 256 rows
 };
+
+// Work-groups of the compaction for rows x-rows: at least 64, and shares
+// of at most 256 rows.
+inline int compact_parts(int rows) { return rows > 0 ? std::max(64, (rows + 255) / 256) : 0; }
 
 __device__ void compact_tasks(const TaskMapArgs &tm, int part, int *__restrict__ flags) {
   __shared__ int sh[256];
+  __shared__ int red[4];
   const int t = threadIdx.x;
-  const int r_lo = (int)((long long)tm.rows * part / kCompactParts);
-  const int r_hi = (int)((long long)tm.rows * (part + 1) / kCompactParts);
-  int carry = 0;
-  for (int r0 = 0; r0 < r_hi; r0 += 256) {
-    const int r = r0 + t;
-    const int v = r < tm.rows ? tm.rowtasks[r] : 0;
-    sh[t] = v;
+  const int r_lo = (int)((long long)tm.rows * part / tm.parts);
+  const int r_hi = (int)((long long)tm.rows * (part + 1) / tm.parts);
+  // base: tasks of the rows before the share
+  int acc = 0;
+  for (int r = t; r < r_lo; r += 256) acc += tm.rowtasks[r];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((t & 63) == 0) red[t >> 6] = acc;
+  // inclusive scan of the share's row counts (at most 256 rows)
+  const int r = r_lo + t;
+  const int v = r < r_hi ? tm.rowtasks[r] : 0;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int a = t >= off ? sh[t - off] : 0;
     __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-      const int a = t >= off ? sh[t - off] : 0;
-      __syncthreads();
-      sh[t] += a;
-      __syncthreads();
-    }
-    const int at = carry + sh[t] - v;
-    if (r >= r_lo && r < r_hi) {
-      for (int k = 0; k < v; ++k) {
-        const size_t sl = (size_t)r * tm.mpr + k, c = (size_t)at + k;
-        tm.ctinfo[c] = tm.tinfo[sl];
-#pragma unroll 6
-        for (int p = 0; p < kTaskPieces; ++p) tm.ctpiece[c * kTaskPieces + p] = tm.tpiece[sl * kTaskPieces + p];
-      }
-    }
-    carry += sh[255];
-    __syncthreads();  // sh is rewritten by the next chunk
+    sh[t] += a;
+    __syncthreads();
   }
-  if (part == kCompactParts - 1 && t == 0) flags[F_NTASK] = carry;  // r_hi = rows: the total
+  const int base = red[0] + red[1] + red[2] + red[3];
+  const int nrow = r_hi - r_lo, ntask_share = nrow > 0 ? sh[nrow - 1] : 0;
+  // records of the share: task m (compact index base + m) of row r_lo + w,
+  // w the row whose inclusive count first exceeds m (binary search in sh)
+  for (int it = t; it < ntask_share * (kTaskPieces + 1); it += 256) {
+    const int m = it / (kTaskPieces + 1), p = it % (kTaskPieces + 1);
+    int lo = 0, hi = nrow - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (sh[mid] > m) hi = mid; else lo = mid + 1;
+    }
+    const int k = m - (lo > 0 ? sh[lo - 1] : 0);  // task within its row
+    const size_t sl = (size_t)(r_lo + lo) * tm.mpr + k, c = (size_t)base + m;
+    if (p == kTaskPieces)
+      tm.ctinfo[c] = tm.tinfo[sl];
+    else
+      tm.ctpiece[c * kTaskPieces + p] = tm.tpiece[sl * kTaskPieces + p];
+  }
+  if (part == tm.parts - 1 && t == 0) flags[F_NTASK] = base + ntask_share;
 }
 
 // Order atoms inside each cell by their position in the previous order so
@@ -303,12 +321,14 @@ __global__ void k_rank_move(const int *__restrict__ cell, const int *__restrict_
                             int *__restrict__ cell_count, int ncell, int4 *__restrict__ q,
                             BoxDev b, int *__restrict__ flags, TaskMapArgs tm) {
   if (flags[F_REBUILD] == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int c = i; c < ncell; c += gridDim.x * blockDim.x) cell_count[c] = 0;
-  if (tm.rows > 0 && (int)blockIdx.x >= (int)gridDim.x - kCompactParts) {
-    compact_tasks(tm, (int)blockIdx.x - ((int)gridDim.x - kCompactParts), flags);
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
+    cell_count[c] = 0;
+  const int cp = tm.parts;  // the first cp work-groups compact the tasks
+  if ((int)blockIdx.x < cp) {
+    compact_tasks(tm, (int)blockIdx.x, flags);
     return;
   }
+  const int i = ((int)blockIdx.x - cp) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int c = cell[i];
   const int a = start[c], e = start[c + 1];
@@ -1698,8 +1718,9 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
         s.tpiece, s.tinfo, s.ctask, s.rowtasks);
     // accelerations are carried along only when no force pass follows; the
     // extra last work-groups (tasks) copy the task tables to compact order
-    const TaskMapArgs tm{s.rowtasks, s.tinfo, s.tpiece, s.ctinfo, s.ctpiece, rows, s.tg.mpr};
-    k_rank_move<<<nblk(n, 256) + (tasks ? kCompactParts : 0), 256, 0, st>>>(
+    const TaskMapArgs tm{s.rowtasks, s.tinfo, s.tpiece, s.ctinfo, s.ctpiece, rows, s.tg.mpr,
+                         compact_parts(rows)};
+    k_rank_move<<<nblk(n, 256) + tm.parts, 256, 0, st>>>(
         s.cell, s.cell_start, s.perm, n, s.pos, s.pos_alt, s.pos_ref, s.vel, s.vel_alt,
         keep_acc ? s.acc : nullptr, s.acc_alt, s.tag, s.tag_alt, s.id, s.id_alt, s.cell_count, nc,
         (s.write_q || b.coarse) ? s.q : nullptr, b, s.flags, tm);
