@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,53 @@
 #include "cv2_md.h"
 
 using namespace cv2;
+
+// ---- device scratch of cv2_halo_pack_noalloc ------------------------------
+// One static array per GPU (each device holds its own copy until the process
+// ends): per-atom direction codes and sources, per-tile counts and offsets,
+// per-direction totals and bases, the 28 results copied back, and the packet
+// staging area.  Host code never uses the array's name as an address: the
+// device address comes from hipGetSymbolAddress, taken once per device at
+// cv2_create with that device current.
+namespace {
+constexpr size_t kNaMaxAtoms = CV2_NOALLOC_MAX_ATOMS;
+constexpr size_t kNaMaxTiles = kNaMaxAtoms / 1024;  // halo tiles of 1024 upload indices
+constexpr size_t kNaCode = 0;
+constexpr size_t kNaSrc = kNaCode + 4 * kNaMaxAtoms;
+constexpr size_t kNaTcnt = kNaSrc + 4 * kNaMaxAtoms;
+constexpr size_t kNaToff = kNaTcnt + 4 * 27 * kNaMaxTiles;
+constexpr size_t kNaDirN = kNaToff + 4 * 27 * kNaMaxTiles;
+constexpr size_t kNaBase = kNaDirN + 256;
+constexpr size_t kNaRes = kNaBase + 256;
+constexpr size_t kNaBuf = kNaRes + 256;
+constexpr size_t kNaBytes = kNaBuf + 8 * (size_t)CV2_NOALLOC_MAX_DOUBLES;
+static_assert(kNaBuf % 256 == 0, "scratch sections are 256-byte aligned");
+
+__device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mutex;
+unsigned char *g_na_ptr[kNaMaxDevices] = {};
+hipError_t g_na_err[kNaMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current), once per device.
+hipError_t noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kNaMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_na_mutex);
+  if (g_na_ptr[device]) return hipSuccess;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_na_err[device] = e;
+  if (e == hipSuccess) g_na_ptr[device] = static_cast<unsigned char *>(ptr);
+  return e;
+}
+
+unsigned char *noalloc_scratch(int device) {
+  if (device < 0 || device >= kNaMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_na_mutex);
+  return g_na_ptr[device];
+}
+}  // namespace
 
 struct cv2_ctx {
   int device = 0;
@@ -51,6 +99,7 @@ struct cv2_ctx {
   int *h_flags = nullptr;
   double *h_red = nullptr;
   long long *h_fx = nullptr;
+  long long *h_halo = nullptr;  // cv2_halo_pack_noalloc: the 28 results of the device scan
   int partial_blocks = 0;
   bool fuse = true;  // integrator fused into the FP32 task kernel where it runs
   int min_mpr = 0;   // task slots per cell row asked for after a row ran out of them
@@ -547,6 +596,10 @@ int cv2_create(cv2_ctx **out, int device) {
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_flags, sizeof(int) * F_COUNT, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_red, sizeof(double) * 16, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_fx, sizeof(long long) * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_halo, sizeof(long long) * 32, hipHostMallocDefault);
+  // the no-allocation halo pack's scratch on this device (its error, if
+  // any, is reported by that entry point)
+  if (e == hipSuccess) (void)noalloc_scratch_init(device);
   // LDS staging capacity of the FP32 task kernel; CV2_STAGE_LIMIT lowers it
   // (tests use it to exercise the unstaged path)
   ctx->s.stage_limit = kStage;
@@ -580,6 +633,7 @@ int cv2_destroy(cv2_ctx *ctx) {
   if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);
   if (ctx->h_fx) (void)hipHostFree(ctx->h_fx);
+  if (ctx->h_halo) (void)hipHostFree(ctx->h_halo);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CV2_OK;
@@ -822,6 +876,54 @@ int cv2_halo_pack(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[27], int6
   launch_halo_write(ctx->s, h, ctx->halo_code, ctx->halo_src, ctx->halo_toff, ctx->halo_base,
                     ctx->halo_dn, ctx->halo_buf, ctx->stream);
   HIPCHK(hipMemcpyAsync(out, ctx->halo_buf, sizeof(double) * at, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return CV2_OK;
+}
+
+int cv2_halo_pack_noalloc(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[27], int64_t cap,
+                          double *out, int64_t *need) {
+  if (!ctx || !g || !counts || !need || cap < 0 || (cap > 0 && !out))
+    return fail(ctx, CV2_ERR_BAD_ARG, "bad halo pack arguments");
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload atoms before packing a halo");
+  for (int a = 0; a < 3; ++a)
+    if (g->nprocs[a] < 1 || g->grid_index[a] < 0 || g->grid_index[a] >= g->nprocs[a] ||
+        !(g->hi_local[a] > g->lo_local[a]))
+      return fail(ctx, CV2_ERR_BAD_ARG, "bad process grid or local box");
+  if (ctx->n > (int64_t)kNaMaxAtoms)
+    return fail(ctx, CV2_ERR_BAD_ARG,
+                "cv2_halo_pack_noalloc: " + std::to_string(ctx->n) +
+                    " owned atoms exceed its scratch limit of " + std::to_string(kNaMaxAtoms));
+  (void)hipSetDevice(ctx->device);
+  unsigned char *sc = noalloc_scratch(ctx->device);
+  if (!sc)
+    return fail(ctx, CV2_ERR_HIP,
+                std::string("cv2_halo_pack_noalloc: no scratch address on this device: ") +
+                    hipGetErrorString(ctx->device >= 0 && ctx->device < kNaMaxDevices
+                                          ? g_na_err[ctx->device] : hipErrorInvalidDevice));
+  int *code = reinterpret_cast<int *>(sc + kNaCode), *src = reinterpret_cast<int *>(sc + kNaSrc);
+  int *tcnt = reinterpret_cast<int *>(sc + kNaTcnt), *toff = reinterpret_cast<int *>(sc + kNaToff);
+  int *dir_n = reinterpret_cast<int *>(sc + kNaDirN);
+  long long *dir_base = reinterpret_cast<long long *>(sc + kNaBase);
+  long long *res = reinterpret_cast<long long *>(sc + kNaRes);
+  double *buf = reinterpret_cast<double *>(sc + kNaBuf);
+  const HaloGeomDev h = halo_geom(g);
+  const int n = (int)ctx->n;
+  const int per_atom = g->with_velocity ? 8 : 5;
+  // every scratch word read below is written first by this call's kernels
+  launch_halo_classify(ctx->s, h, code, src, tcnt, ctx->stream);
+  launch_halo_scan(halo_tiles(n), per_atom, tcnt, toff, dir_n, dir_base, res, ctx->stream);
+  HIPCHK(hipMemcpyAsync(ctx->h_halo, res, sizeof(long long) * 28, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int d = 0; d < 27; ++d) counts[d] = n > 0 ? ctx->h_halo[d] : 0;
+  const int64_t at = n > 0 ? ctx->h_halo[27] : 0;
+  *need = at;
+  if (at == 0 || at > cap) return CV2_OK;
+  if (at > (int64_t)CV2_NOALLOC_MAX_DOUBLES)
+    return fail(ctx, CV2_ERR_BAD_ARG,
+                "cv2_halo_pack_noalloc: " + std::to_string(at) + " packet doubles exceed its scratch limit of " +
+                    std::to_string((int64_t)CV2_NOALLOC_MAX_DOUBLES));
+  launch_halo_write(ctx->s, h, code, src, toff, dir_base, dir_n, buf, ctx->stream);
+  HIPCHK(hipMemcpyAsync(out, buf, sizeof(double) * at, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return CV2_OK;
 }

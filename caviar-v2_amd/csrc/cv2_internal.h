@@ -485,6 +485,11 @@ struct HaloGeomDev {
 int halo_tiles(int n);
 void launch_halo_classify(const DevState &s, const HaloGeomDev &g, int *code_by_tag,
                           int *src_by_tag, int *tile_counts, hipStream_t st);
+// Device-side scan of the tile counts: tile offsets, per-direction totals
+// (dir_n, res[0..26]), packet bases (dir_base, doubles) and their total
+// (res[27]); per_atom = doubles per atom (5, or 8 with velocities).
+void launch_halo_scan(int ntiles, int per_atom, const int *tile_counts, int *tile_off, int *dir_n,
+                      long long *dir_base, long long *res, hipStream_t st);
 void launch_halo_write(const DevState &s, const HaloGeomDev &g, const int *code_by_tag,
                        const int *src_by_tag, const int *tile_off, const long long *dir_base,
                        const int *dir_n, double *out, hipStream_t st);

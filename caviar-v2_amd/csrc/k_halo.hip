@@ -134,6 +134,49 @@ __global__ void __launch_bounds__(kHaloTile)
   }
 }
 
+// Tile offsets, per-direction totals and packet bases on the device (the
+// host does this scan in cv2_halo_pack): one work-group; direction d's
+// exclusive scan over the tiles' counts gives tile_off, its total N_d goes
+// to dir_n and res[d], and the packets follow each other in direction order
+// (dir_base, in doubles, res[27] = their total).
+__global__ void __launch_bounds__(1024)
+    k_halo_scan(const int *__restrict__ tile_counts, int ntiles, int per_atom,
+                int *__restrict__ tile_off, int *__restrict__ dir_n, long long *__restrict__ dir_base,
+                long long *__restrict__ res) {
+  __shared__ int sh[1024];
+  __shared__ long long tot[27];
+  const int t = threadIdx.x;
+  for (int d = 0; d < 27; ++d) {
+    int carry = 0;
+    for (int t0 = 0; t0 < ntiles; t0 += 1024) {
+      const int v = t0 + t < ntiles ? tile_counts[(t0 + t) * 27 + d] : 0;
+      sh[t] = v;
+      __syncthreads();
+      for (int off = 1; off < 1024; off <<= 1) {
+        const int a = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+      }
+      if (t0 + t < ntiles) tile_off[(t0 + t) * 27 + d] = carry + sh[t] - v;
+      carry += sh[1023];
+      __syncthreads();
+    }
+    if (t == 0) tot[d] = carry;
+  }
+  __syncthreads();
+  if (t == 0) {
+    long long at = 0;
+    for (int d = 0; d < 27; ++d) {
+      dir_n[d] = (int)tot[d];
+      dir_base[d] = at;
+      res[d] = tot[d];
+      at += (long long)per_atom * tot[d];
+    }
+    res[27] = at;
+  }
+}
+
 // Ghost g of the concatenated received packets (pk_first[p] = first ghost of
 // packet p, pk_base[p] = its first double, pk_n[p] = its atoms).
 __global__ void k_halo_unpack(const double *__restrict__ in, int npk,
@@ -161,6 +204,11 @@ void launch_halo_classify(const DevState &s, const HaloGeomDev &g, int *code_by_
   if (s.n == 0) return;
   k_halo_dirs<<<(s.n + 255) / 256, 256, 0, st>>>(s.pos, s.tag, s.n, g, code_by_tag, src_by_tag);
   k_halo_count<<<halo_tiles(s.n), kHaloTile, 0, st>>>(code_by_tag, s.n, tile_counts);
+}
+
+void launch_halo_scan(int ntiles, int per_atom, const int *tile_counts, int *tile_off, int *dir_n,
+                      long long *dir_base, long long *res, hipStream_t st) {
+  k_halo_scan<<<1, 1024, 0, st>>>(tile_counts, ntiles, per_atom, tile_off, dir_n, dir_base, res);
 }
 
 void launch_halo_write(const DevState &s, const HaloGeomDev &g, const int *code_by_tag,

@@ -85,6 +85,7 @@ def lib():
             "cv2_set_ghosts": ([vp, C.c_int64, _d, _i32, _i64], C.c_int),
             "cv2_halo_pack": ([vp, C.POINTER(HaloGeom), _i64, C.c_int64, _d, _i64], C.c_int),
             "cv2_halo_unpack": ([vp, C.c_int32, _i64, _d, C.c_int32], C.c_int),
+            "cv2_halo_pack_noalloc": ([vp, C.POINTER(HaloGeom), _i64, C.c_int64, _d, _i64], C.c_int),
             "cv2_build_neighbors": ([vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
             "cv2_compute_forces": ([vp], C.c_int),
             "cv2_compute_accelerations": ([vp], C.c_int),
@@ -198,9 +199,11 @@ class Context:
 
     # ---- work
     def halo_pack(self, grid_index, nprocs, lo_local, hi_local, size_global, bc, ghost_cutoff,
-                  with_velocity=False):
+                  with_velocity=False, noalloc=False):
         """MPI halo packets of the owned atoms in the reference's layout:
-        (counts[27], packets concatenated in direction order d = i*9+j*3+k)."""
+        (counts[27], packets concatenated in direction order d = i*9+j*3+k).
+        noalloc: through cv2_halo_pack_noalloc (the library's static scratch)."""
+        fn = self.L.cv2_halo_pack_noalloc if noalloc else self.L.cv2_halo_pack
         g = HaloGeom()
         for a in range(3):
             g.grid_index[a] = int(grid_index[a]); g.nprocs[a] = int(nprocs[a]); g.bc[a] = int(bc[a])
@@ -210,10 +213,9 @@ class Context:
         g.with_velocity = int(bool(with_velocity))
         counts = np.zeros(27, np.int64)
         need = C.c_int64()
-        self._chk(self.L.cv2_halo_pack(self.h, C.byref(g), _p(counts, _i64), 0, None, C.byref(need)))
+        self._chk(fn(self.h, C.byref(g), _p(counts, _i64), 0, None, C.byref(need)))
         out = np.empty(max(need.value, 1))
-        self._chk(self.L.cv2_halo_pack(self.h, C.byref(g), _p(counts, _i64), need.value, _p(out, _d),
-                                       C.byref(need)))
+        self._chk(fn(self.h, C.byref(g), _p(counts, _i64), need.value, _p(out, _d), C.byref(need)))
         return counts, out[:need.value]
 
     def halo_unpack(self, counts, packets, with_velocity=False):

@@ -146,6 +146,25 @@ typedef struct {
 int cv2_halo_pack(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[27], int64_t cap,
                   double *out, int64_t *need);
 
+/* cv2_halo_pack without device memory allocation: the same counts and
+ * packets (bit for bit), from the same kernels, with every device buffer the
+ * packing needs -- per-atom direction codes, per-tile counts and offsets,
+ * per-direction totals and bases, the packet staging area -- taken from one
+ * static per-device scratch array of the library (about 25 MiB, held by each
+ * GPU the library runs on until the process ends) instead of context
+ * buffers allocated on first use; the tile offsets are scanned on the device.
+ * No counterpart in the reference: this is for callers that cannot call
+ * hipMalloc, such as a per-step exchange loop that must not allocate or code
+ * around HIP graph capture.  Limits: at most CV2_NOALLOC_MAX_ATOMS owned
+ * atoms and CV2_NOALLOC_MAX_DOUBLES packet doubles (CV2_ERR_BAD_ARG beyond).
+ * Every call on one GPU uses that GPU's one array: calls that overlap on one
+ * GPU (two contexts on different host threads) race; no state carries over
+ * from one call to the next. */
+#define CV2_NOALLOC_MAX_ATOMS 1048576
+#define CV2_NOALLOC_MAX_DOUBLES 2097152
+int cv2_halo_pack_noalloc(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[27], int64_t cap,
+                          double *out, int64_t *need);
+
 /* The unpacking half (:405-470): the ghosts become the atoms of npackets
  * received packets, appended in the order given (the reference appends the
  * received packets in direction order, then the domain's own periodic
