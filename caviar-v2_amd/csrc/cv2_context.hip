@@ -320,14 +320,47 @@ int check_flags(cv2_ctx *ctx) {
     HIPCHK(hipMemcpyAsync(s.flags + F_TASKOVF, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
   }
-  if (f[F_OVERFLOW])
+  if (f[F_OVERFLOW])  // the callers grow the lists and go on (grow_lists)
     return fail(ctx, CV2_ERR_CAPACITY,
                 "neighbour list capacity " + std::to_string(ctx->s.cap) + " exceeded (longest list " +
                     std::to_string(f[F_MAXNBR]) + ")");
-  if (f[F_NAN]) return fail(ctx, CV2_ERR_NAN, "NaN acceleration");
-  if (f[F_FXOVF]) return fail(ctx, CV2_ERR_FIXED_OVERFLOW, "fixed-point force range exceeded (r < sigma/2)");
+  if (f[F_NAN] || f[F_FXOVF]) {
+    // fatal at a step: the device halted there.  The flags are cleared so
+    // that the context can take a new state (cv2_upload_atoms) and go on.
+    const std::string where = " (step " + std::to_string(f[F_STEP]) + " of the run)";
+    const int code = f[F_NAN] ? CV2_ERR_NAN : CV2_ERR_FIXED_OVERFLOW;
+    const std::string msg = f[F_NAN] ? "NaN acceleration, position or velocity" + where
+                                     : "fixed-point force range exceeded (r < sigma/2)" + where;
+    const int zero[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(s.flags + F_NAN, zero, sizeof(int) * 2, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(s.flags + F_HALT, zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return fail(ctx, code, msg);
+  }
   if (f[F_STAGE])
     return fail(ctx, CV2_ERR_CAPACITY, "a force task's neighbourhood exceeds 65536 atoms (16-bit slots)");
+  return CV2_OK;
+}
+
+// After a build halted on the list capacity (F_OVERFLOW): lists of
+// 1.5 times the longest one (at least 1.5 times the old capacity), and the
+// halt cleared; the build request stays set, so the next build runs again.
+// Returns CV2_OK when the lists have grown.
+int grow_lists(cv2_ctx *ctx) {
+  DevState &s = ctx->s;
+  const int64_t longest = ctx->h_flags[F_MAXNBR];
+  int64_t cap = std::max<int64_t>(longest + longest / 2, s.cap + s.cap / 2) + 16;
+  cap = (cap + 15) / 16 * 16;
+  if (cap > (1 << 16))
+    return fail(ctx, CV2_ERR_CAPACITY, "a neighbour list of " + std::to_string(longest) +
+                                           " entries (more than 65536 slots would be needed)");
+  HIPCHK(dalloc(&s.nbr, (size_t)cap * s.stride));
+  s.cap = (int)cap;
+  const int zero = 0;
+  HIPCHK(hipMemcpyAsync(s.flags + F_OVERFLOW, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(s.flags + F_HALT, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->err.clear();
   return CV2_OK;
 }
 
@@ -383,7 +416,7 @@ int fetch_energy(cv2_ctx *ctx) {
   return CV2_OK;
 }
 
-int queue_build(cv2_ctx *ctx, bool force, bool keep_acc) {
+int queue_build(cv2_ctx *ctx, bool force, bool keep_acc, int step = 0) {
   force = force || ctx->need_build;
   ctx->need_build = false;
   if (force) {
@@ -392,7 +425,7 @@ int queue_build(cv2_ctx *ctx, bool force, bool keep_acc) {
   }
   if (force || (ctx->nl_flags & CV2_NL_REBUILD_EVERY_STEP))
     launch_set_flag(ctx->s.flags, F_REBUILD, 1, ctx->stream);
-  launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, keep_acc);
+  launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, keep_acc, step);
   return CV2_OK;
 }
 
@@ -768,6 +801,8 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
   const double ce = ctx->nl_cutoff * ctx->nl_coef;
   const double expect = 4.18879020478639 * ce * ce * ce * (double)n / vol;
   int cap = (int)std::ceil(1.6 * expect + 64.0);
+  // CV2_LIST_CAP lowers it (tests: lists that outgrow it during a run)
+  if (const char *lc = std::getenv("CV2_LIST_CAP")) cap = std::max(16, std::min(cap, std::atoi(lc)));
   cap = (cap + 15) / 16 * 16;
   s.cap = cap;
   s.stride = (int)((n + 63) / 64 * 64);  // atoms per 4-entry group row
@@ -1002,9 +1037,35 @@ int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt) {
   if (rc) return rc;
   ctx->forces_valid = false;
   rc = check_flags(ctx);
+  // a list cut at its capacity: grow the lists and build again
+  for (int tries = 0; rc == CV2_ERR_CAPACITY && ctx->h_flags[F_OVERFLOW] && tries < 8; ++tries) {
+    rc = grow_lists(ctx);
+    if (rc) return rc;
+    launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, true);
+    rc = check_flags(ctx);
+  }
   if (rebuilt) *rebuilt = ctx->h_flags[F_REBUILDS] != before;
   return rc;
 }
+
+namespace {
+// One force pass at the current positions (after the build queued before
+// it); a build that halted on the list capacity is run again with grown
+// lists first.
+int forces_pass(cv2_ctx *ctx, bool energy) {
+  for (int tries = 0;; ++tries) {
+    if (ctx->precision != CV2_PREC_FP64) launch_pack_q(ctx->s, ctx->b, ctx->stream);
+    const int t = span_begin(ctx, 0);
+    launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, energy, ctx->stream);
+    span_end(ctx, t, 0);
+    int rc = check_flags(ctx);
+    if (rc != CV2_ERR_CAPACITY || !ctx->h_flags[F_OVERFLOW] || tries == 8) return rc;
+    rc = grow_lists(ctx);
+    if (rc) return rc;
+    launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, false);
+  }
+}
+}  // namespace
 
 int cv2_compute_forces(cv2_ctx *ctx) {
   if (!ctx) return CV2_ERR_BAD_ARG;
@@ -1015,11 +1076,7 @@ int cv2_compute_forces(cv2_ctx *ctx) {
     rc = queue_build(ctx, true, false);
     if (rc) return rc;
   }
-  if (ctx->precision != CV2_PREC_FP64) launch_pack_q(ctx->s, ctx->b, ctx->stream);
-  const int t = span_begin(ctx, 0);
-  launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, true, ctx->stream);
-  span_end(ctx, t, 0);
-  rc = check_flags(ctx);
+  rc = forces_pass(ctx, true);
   if (rc) return rc;
   rc = fetch_energy(ctx);
   if (rc) return rc;
@@ -1036,11 +1093,7 @@ int cv2_compute_accelerations(cv2_ctx *ctx) {
     rc = queue_build(ctx, true, false);
     if (rc) return rc;
   }
-  if (ctx->precision != CV2_PREC_FP64) launch_pack_q(ctx->s, ctx->b, ctx->stream);
-  const int t = span_begin(ctx, 0);
-  launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, false, ctx->stream);
-  span_end(ctx, t, 0);
-  rc = check_flags(ctx);
+  rc = forces_pass(ctx, false);
   if (rc) return rc;
   ctx->forces_valid = true;  // the energy and virial of cv2_thermo stay those of the last energy pass
   return CV2_OK;
@@ -1057,53 +1110,69 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
   u.writeq = ctx->precision != CV2_PREC_FP64;
   u.ref_rebuild = ref_rebuilds(ctx);
   const uint64_t s0 = (uint64_t)ctx->steps;
+  int64_t k0 = 0;
   bool opened = false;  // this iteration's integrator pass already ran (fused)
-  for (int64_t k = 0; k < nsteps; ++k) {
-    if (!opened) {
-      u.finish = k > 0;
-      u.start = true;
-      u.step = s0 + (uint64_t)k - 1;
+  launch_set_flag(ctx->s.flags, F_STEP, 0, ctx->stream);  // steps are counted from this run's first
+  for (int attempt = 0;; ++attempt) {
+    for (int64_t k = k0; k < nsteps; ++k) {
+      if (!opened) {
+        u.finish = k > 0;
+        u.start = true;
+        u.step = s0 + (uint64_t)k - 1;
+        const int t = span_begin(ctx, 2);
+        launch_update(ctx->s, ctx->b, u, ctx->stream);
+        span_end(ctx, t, 2);
+      }
+      int t = span_begin(ctx, 1);
+      {
+        const int rc = queue_build(ctx, false, false, (int)k);
+        if (rc) return rc;
+      }
+      span_end(ctx, t, 1);
+      t = span_begin(ctx, 0);
+      opened = false;
+      if (ctx->fuse && k + 1 < nsteps && ctx->precision != CV2_PREC_FP64) {
+        UpdateArgs f = u;
+        f.finish = true;
+        f.start = true;
+        f.step = s0 + (uint64_t)k;  // what iteration k + 1 would close and open
+        if (launch_forces_fused(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, f, ctx->stream)) {
+          std::swap(ctx->s.q, ctx->s.q_alt);
+          opened = true;
+        }
+      }
+      if (!opened)
+        launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
+      span_end(ctx, t, 0);
+      if (ctx->spans.size() > 4000) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        spans_collect(ctx);
+      }
+    }
+    if (nsteps > 0) {
+      u.finish = true;
+      u.start = false;
+      u.step = s0 + (uint64_t)nsteps - 1;
       const int t = span_begin(ctx, 2);
       launch_update(ctx->s, ctx->b, u, ctx->stream);
       span_end(ctx, t, 2);
     }
-    int t = span_begin(ctx, 1);
-    {
-      const int rc = queue_build(ctx, false, false);
+    int rc = check_flags(ctx);
+    if (rc == CV2_ERR_CAPACITY && ctx->h_flags[F_OVERFLOW] && attempt < 8) {
+      // The build of step F_STEP cut a list at its capacity and halted the
+      // run there: that step's integrator pass had run, its forces had not.
+      // Grow the lists and go on from that step (its build runs again; it
+      // writes the fixed-point coordinates the force kernel reads).
+      k0 = ctx->h_flags[F_STEP];
+      opened = true;
+      rc = grow_lists(ctx);
       if (rc) return rc;
+      continue;
     }
-    span_end(ctx, t, 1);
-    t = span_begin(ctx, 0);
-    opened = false;
-    if (ctx->fuse && k + 1 < nsteps && ctx->precision != CV2_PREC_FP64) {
-      UpdateArgs f = u;
-      f.finish = true;
-      f.start = true;
-      f.step = s0 + (uint64_t)k;  // what iteration k + 1 would close and open
-      if (launch_forces_fused(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, f, ctx->stream)) {
-        std::swap(ctx->s.q, ctx->s.q_alt);
-        opened = true;
-      }
-    }
-    if (!opened)
-      launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
-    span_end(ctx, t, 0);
-    if (ctx->spans.size() > 4000) {
-      HIPCHK(hipStreamSynchronize(ctx->stream));
-      spans_collect(ctx);
-    }
-  }
-  if (nsteps > 0) {
-    u.finish = true;
-    u.start = false;
-    u.step = s0 + (uint64_t)nsteps - 1;
-    const int t = span_begin(ctx, 2);
-    launch_update(ctx->s, ctx->b, u, ctx->stream);
-    span_end(ctx, t, 2);
+    if (rc) return rc;
+    break;
   }
   ctx->steps += nsteps;
-  int rc = check_flags(ctx);
-  if (rc) return rc;
   if (nsteps > 0) return fetch_energy(ctx);
   return CV2_OK;
 }

@@ -33,7 +33,7 @@ enum Flag : int {
   F_REBUILD = 0,   // a rebuild is pending (set by the displacement check)
   F_OVERFLOW = 1,  // some list exceeded its capacity
   F_MAXNBR = 2,    // longest list of the last build
-  F_NAN = 3,       // NaN acceleration seen
+  F_NAN = 3,       // NaN acceleration, position or velocity seen
   F_FXOVF = 4,     // fixed-point overflow seen
   F_REBUILDS = 5,  // number of builds done on the device
   F_ENTRIES = 6,   // list entries of the last build (low 32 bits)
@@ -42,8 +42,26 @@ enum Flag : int {
   F_BINBLK = 9,    // blocks of k_wrap_bin done (reset by the last one)
   F_NTASK = 10,    // force tasks of the current build (compact task map length)
   F_TASKOVF = 11,  // some cell row needed more task slots than it has
-  F_COUNT = 12
+  F_HALT = 12,     // a fatal condition (NaN, fixed-point overflow, list capacity) was
+                   // raised: every later kernel of the run returns at once, so the
+                   // state stays at the step that raised it
+  F_STEP = 13,     // index (within the run) of the step whose kernels run now;
+                   // frozen by F_HALT
+  F_COUNT = 14
 };
+
+// A list build runs when one is requested and the run has not halted.
+__device__ inline bool build_pending(const int *flags) {
+  return flags[F_REBUILD] != 0 && flags[F_HALT] == 0;
+}
+
+// Raise a fatal flag and halt the run (Md_simulator::integrate_* ends the
+// program at the step of a NaN acceleration, md_simulator.cpp:646-652; here
+// the device stops advancing at that step and the host reports it).
+__device__ inline void raise_fatal(int *flags, int which) {
+  atomicOr(&flags[which], 1);
+  atomicOr(&flags[F_HALT], 1);
+}
 
 struct BoxDev {
   double lo[3], hi[3], L[3], halfL[3];
@@ -251,6 +269,8 @@ __device__ inline void integrate_store(int i, int n, const double4 &x, const dou
                                        const double4 &r, double4 *__restrict__ pos,
                                        double *__restrict__ vel, int4 *__restrict__ q,
                                        const StepParams &p, int *__restrict__ flags) {
+  if (isnan(x.x) | isnan(x.y) | isnan(x.z) | isnan(v[0]) | isnan(v[1]) | isnan(v[2]))
+    raise_fatal(flags, F_NAN);
   if (START) {
     pos[i] = x;
     if (WRITEQ) q[i] = pack_q(x.x, x.y, x.z, (int)x.w, p.lo, p.qscale);
@@ -424,7 +444,10 @@ bool build_lj_table(LJDev &L, int ntypes, const double *eps, const double *sigma
 void launch_pack_q(const DevState &s, const BoxDev &b, hipStream_t st);
 // keep_acc: carry the accelerations into the new order (false when a force
 // pass follows the build before anything reads them)
-void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, bool keep_acc);
+// step: index of the step within the run (kept in flags[F_STEP] while the
+// run has not halted)
+void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, bool keep_acc,
+                  int step = 0);
 void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                    int precision, bool energy, hipStream_t st);
 // Task slots for n atoms on b's cell grid: mpr slots per x-row, at least

@@ -51,7 +51,7 @@ __device__ inline int cell_of(const double4 &p, const BoxDev &b) {
 __global__ void k_wrap_bin(double4 *__restrict__ pos, int n, BoxDev b, int *__restrict__ cell,
                            int *__restrict__ slot, int *__restrict__ cell_count,
                            int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {  // statistics of this build (k_build_stats)
     flags[F_MAXNBR] = 0;
@@ -112,7 +112,7 @@ __device__ inline int block_exclusive_scan(int v, int *sh) {
 __global__ void __launch_bounds__(kScanTile)
     k_scan_tiles(const int *__restrict__ count, int *__restrict__ start,
                  int *__restrict__ tile_sum, int ncell, const int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   __shared__ int sh[kScanTile];
   const int c = blockIdx.x * kScanTile + threadIdx.x;
   const int v = c < ncell ? count[c] : 0;
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(kScanTile)
 __global__ void __launch_bounds__(kScanTile)
     k_scan_add(int *__restrict__ start, const int *__restrict__ tile_sum, int ncell, int ntiles,
                const int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   __shared__ int sh[kScanTile];
   // offset of this tile: sum of the totals of the tiles before it
   int s = 0;
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(kScanTile)
 __global__ void __launch_bounds__(kScanTile)
     k_scan_local(const int *__restrict__ count, int *__restrict__ local, int *tile_sum,
                  int *__restrict__ start, int ncell, int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   __shared__ int sh[kScanTile];
   __shared__ int last_sh;
   const int c = blockIdx.x * kScanTile + threadIdx.x;
@@ -210,7 +210,7 @@ __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ 
                           const int *__restrict__ count, int4 *__restrict__ tpiece,
                           int4 *__restrict__ tinfo, int *__restrict__ ctask,
                           int *__restrict__ rowtasks) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   const int b = blockIdx.x;
   if (b >= nbs + nfix) {
     // exclusive prefix sums over x of one row's 9-row column counts (P) and
@@ -320,7 +320,7 @@ __global__ void k_rank_move(const int *__restrict__ cell, const int *__restrict_
                             const int *__restrict__ id, int *__restrict__ id_out,
                             int *__restrict__ cell_count, int ncell, int4 *__restrict__ q,
                             BoxDev b, int *__restrict__ flags, TaskMapArgs tm) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
     cell_count[c] = 0;
   const int cp = tm.parts;  // the first cp work-groups compact the tasks
@@ -349,6 +349,7 @@ __global__ void k_rank_move(const int *__restrict__ cell, const int *__restrict_
   pos_out[k] = p;
   pos_ref[k] = p;
   if (q) q[k] = qpack(p, b);
+  if (isnan(p.x) | isnan(p.y) | isnan(p.z)) raise_fatal(flags, F_NAN);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     vel_out[d * n + k] = vel[d * n + i];
@@ -386,7 +387,7 @@ __device__ inline void copy_back_atom(int k, const CopyBack &c) {
 }
 
 __global__ void k_copy_back(CopyBack c, const int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < c.n) copy_back_atom(k, c);
 }
@@ -397,7 +398,7 @@ __global__ void k_ghost_bin(const double4 *__restrict__ pos, double4 *__restrict
                             int n, int ng, BoxDev b, int *__restrict__ cell,
                             int *__restrict__ slot, int *__restrict__ cell_count,
                             const int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ng) return;
   const double4 p = pos[n + g];
@@ -410,7 +411,7 @@ __global__ void k_ghost_bin(const double4 *__restrict__ pos, double4 *__restrict
 __global__ void k_ghost_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
                                 const int *__restrict__ start, int *__restrict__ items, int ng,
                                 const int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ng) return;
   items[start[cell[g]] + slot[g]] = g;
@@ -418,7 +419,7 @@ __global__ void k_ghost_scatter(const int *__restrict__ cell, const int *__restr
 
 __global__ void k_ghost_sort_cells(int *__restrict__ items, const int *__restrict__ start,
                                    int ncell, const int *__restrict__ flags) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const int a = start[c], e = start[c + 1];
@@ -578,7 +579,7 @@ __global__ void __launch_bounds__(256)
             const int *__restrict__ gstart, const int *__restrict__ gitems,
             uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
             const int *__restrict__ flags, int host_halo, int2 *__restrict__ wstat) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int cnt = 0, kept = 0;
   if (i < n) nlist_atom(pos, n, b, cstart, gstart, gitems, nbr, nnbr, stride, cap, host_halo, i, cnt, kept);
@@ -1089,7 +1090,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
                   const int *__restrict__ flags, int nblk_real, int2 *__restrict__ bstat,
                   const int *__restrict__ ctask, const int4 *__restrict__ tpiece,
                   const int4 *__restrict__ tinfo, int ncopy, CopyBack cpb) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   const int lgrid = (int)gridDim.x - ncopy;  // list work-groups, a multiple of 8
   if ((int)blockIdx.x >= lgrid) {
     // the last ncopy work-groups copy the new order back to the primary
@@ -1133,8 +1134,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 // k_wrap_bin cleared); the last block to finish checks the capacity and
 // clears the request.
 __global__ void __launch_bounds__(256)
-    k_build_stats(int *__restrict__ flags, const int2 *__restrict__ wstat, int nw, int cap) {
-  if (flags[F_REBUILD] == 0) return;
+    k_build_stats(int *__restrict__ flags, const int2 *__restrict__ wstat, int nw, int cap,
+                  int step) {
+  // the step index of the run, kept while the run goes on (the last kernel
+  // of the build chain runs on every step)
+  if (blockIdx.x == 0 && threadIdx.x == 0 && flags[F_HALT] == 0) flags[F_STEP] = step;
+  if (!build_pending(flags)) return;
   __shared__ int smax[4], ssum[4];
   int m = 0, t = 0;
   for (int k = blockIdx.x * 256 + threadIdx.x; k < nw; k += gridDim.x * 256) {
@@ -1159,10 +1164,17 @@ __global__ void __launch_bounds__(256)
     __threadfence();
     if (atomicAdd(&flags[F_STATBLK], 1) == (int)gridDim.x - 1) {
       // every block's atomics are done
-      if (atomicAdd(&flags[F_MAXNBR], 0) > cap) flags[F_OVERFLOW] = 1;
       flags[F_STATBLK] = 0;
-      flags[F_REBUILD] = 0;
-      flags[F_REBUILDS] += 1;
+      if (atomicAdd(&flags[F_MAXNBR], 0) > cap) {
+        // some list was cut at its capacity: halt before any force uses it,
+        // the request stays set so that the host can grow the lists and
+        // run the build again
+        flags[F_OVERFLOW] = 1;
+        flags[F_HALT] = 1;
+      } else {
+        flags[F_REBUILD] = 0;
+        flags[F_REBUILDS] += 1;
+      }
     }
   }
 }
@@ -1276,7 +1288,7 @@ __device__ void task_row_partition(int row, int lane, int *P, int *H,
 
 __global__ void k_build_begin(int *__restrict__ flags, int *__restrict__ cell_count, int ncell,
                               int *__restrict__ gcount) {
-  if (flags[F_REBUILD] == 0) return;
+  if (!build_pending(flags)) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < ncell) {
     cell_count[c] = 0;
@@ -1284,10 +1296,13 @@ __global__ void k_build_begin(int *__restrict__ flags, int *__restrict__ cell_co
   }
 }
 
-__global__ void k_pack_q(const double4 *__restrict__ pos, int4 *__restrict__ q, int ntot, BoxDev b) {
+__global__ void k_pack_q(const double4 *__restrict__ pos, int4 *__restrict__ q, int ntot, BoxDev b,
+                         int *__restrict__ flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ntot) return;
-  q[i] = qpack(pos[i], b);
+  const double4 p = pos[i];
+  if (isnan(p.x) | isnan(p.y) | isnan(p.z)) raise_fatal(flags, F_NAN);
+  q[i] = qpack(p, b);
 }
 
 __global__ void k_set_flag(int *flags, int which, int value) { flags[which] = value; }
@@ -1686,10 +1701,11 @@ void launch_set_flag(int *flags, int which, int value, hipStream_t st) {
 void launch_pack_q(const DevState &s, const BoxDev &b, hipStream_t st) {
   const int ntot = s.n + s.ng;
   if (ntot == 0) return;
-  k_pack_q<<<nblk(ntot, 256), 256, 0, st>>>(s.pos, s.q, ntot, b);
+  k_pack_q<<<nblk(ntot, 256), 256, 0, st>>>(s.pos, s.q, ntot, b, s.flags);
 }
 
-void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, bool keep_acc) {
+void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, bool keep_acc,
+                  int step) {
   const int n = s.n;
   const int nc = s.ncell;
   const bool ghosts = host_halo && s.ng > 0;
@@ -1768,7 +1784,7 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
     nw = nblk(n, 256);
   }
   k_build_stats<<<std::max(1, std::min(nblk(nw, 256), 64)), 256, 0, st>>>(s.flags, s.wstat, nw,
-                                                                           s.cap);
+                                                                           s.cap, step);
 }
 
 TaskGeom task_geom(int n, const BoxDev &b, int min_mpr) {

@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(kBlock)
              double *__restrict__ vel, const double *__restrict__ acc, const int *__restrict__ id,
              int4 *__restrict__ q, int n, StepParams p, int *__restrict__ flags) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || flags[F_HALT]) return;
   const double a[3] = {acc[i], acc[n + i], acc[2 * n + i]};
   double v[3] = {vel[i], vel[n + i], vel[2 * n + i]};
   double4 x = make_double4(0.0, 0.0, 0.0, 0.0), r = x;

@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(kBlock)
              double *__restrict__ partials, int *__restrict__ flags, int nblk_real) {
   __shared__ double sh[kBlock / 64];
   const int blk = xcd_block(nblk_real);
-  if (blk < 0) return;
+  if (blk < 0 || flags[F_HALT]) return;
   const int i = blk * kBlock + threadIdx.x;
   double vir = 0.0, pe = 0.0;
   if (i < n) {
@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(kBlock)
     acc[i] = ax;
     acc[n + i] = ay;
     acc[2 * n + i] = az;
-    if (isnan(ax) || isnan(ay) || isnan(az)) atomicOr(&flags[F_NAN], 1);
+    if (isnan(ax) || isnan(ay) || isnan(az)) raise_fatal(flags, F_NAN);
   }
   if (ENERGY) {
     const double v = block_sum(vir, sh);
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(kBlock)
   static_assert(!(MULTI && ISO), "species tables are in length units");
   __shared__ long long sh[kBlock / 64];
   const int blk = xcd_block(nblk_real);
-  if (blk < 0) return;
+  if (blk < 0 || flags[F_HALT]) return;
   const int i = blk * kBlock + threadIdx.x;
   const double u0 = b.qinv[0], u1 = b.qinv[1], u2 = b.qinv[2];
   AccF32 a;
@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(kBlock)
     acc[i] = fx_value(a.sx, a.terms) * mi;
     acc[n + i] = fx_value(a.sy, a.terms) * mi;
     acc[2 * n + i] = fx_value(a.sz, a.terms) * mi;
-    if (a.ovf) atomicOr(&flags[F_FXOVF], 1);
+    if (a.ovf) raise_fatal(flags, F_FXOVF);
   }
   if (ENERGY) {
     const long long v = block_sum_i64(a.vir, sh);
@@ -482,6 +482,7 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
   __shared__ int lhist[kSortBins + 1];
   __shared__ unsigned char lorder[kTaskBlock];
   __shared__ int4 sent;  // the sentinel (read by the staging and the unstaged path only)
+  if (flags[F_HALT]) return;  // uniform over the launch
   const int ntask = flags[F_NTASK];
   const int xcd = blockIdx.x & 7;  // grids: multiples of 8
   const int k0 = (int)((long long)ntask * xcd / 8), k1 = (int)((long long)ntask * (xcd + 1) / 8);
@@ -593,7 +594,7 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
         acc[n + i] = ai[1];
         acc[2 * n + i] = ai[2];
       }
-      if (ovf) atomicOr(&flags[F_FXOVF], 1);
+      if (ovf) raise_fatal(flags, F_FXOVF);
       vir += a.vir;
       pe += a.pe;
     }

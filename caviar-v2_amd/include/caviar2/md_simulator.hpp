@@ -16,6 +16,12 @@ struct DeviceSession;
 
 enum class Integrator_t { Verlet, Velocity_verlet, Leap_frog, Velocity_verlet_langevin, Unknown };
 
+// Step index of a time, as Md_simulator::setup converts the initial and
+// final time (md_simulator.cpp:569-572: (int)time / dt, the time truncated
+// to an integer before the division, the quotient truncated on assignment;
+// 2.5 with dt = 0.005 gives 400 steps).
+inline int64_t steps_of_time(double t, double dt) { return (int64_t)((double)(int)t / dt); }
+
 // Pair-force precision on the device.
 enum class Precision { fp64, fp32_fixed };
 

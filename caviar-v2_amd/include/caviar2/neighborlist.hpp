@@ -4,7 +4,9 @@
 // The list itself lives on the device (full list, cell-sorted order).
 // neighlist, the reference's half list of owned j > i plus ghost images
 // (ghost g stored as N + g, ghosts in atom_struct_ghost order), is filled on
-// the host only by update_host_list().
+// the host only by update_host_list(); with set_all_atom_test(true) it is
+// the reference's all-pairs debug list (neighborlist.cpp:170-200), while the
+// device keeps its distance list, which gives the same forces.
 #pragma once
 #include <vector>
 

@@ -364,6 +364,18 @@ void Neighborlist::update_host_list() {
     }
   }
   neighlist.assign(n, {});
+  if (all_atom_test) {
+    // Neighborlist::all_atom_test_function (neighborlist.cpp:170-200): every
+    // owned j > i and every ghost, whatever the distance.  The device keeps
+    // its distance list: pairs beyond the cutoff add nothing to the forces
+    // (lj.cpp:456), so both give the same accelerations.
+    const unsigned ng = (unsigned)g.position.size();
+    for (size_t i = 0; i < n; ++i) {
+      for (size_t j = i + 1; j < n; ++j) neighlist[i].push_back((unsigned)j);
+      for (unsigned q = 0; q < ng; ++q) neighlist[i].push_back((unsigned)(n + q));
+    }
+    return;
+  }
   for (int64_t k = 0; k < np; ++k) {
     const int64_t i = pairs[3 * k], j = pairs[3 * k + 1];
     const int code = (int)pairs[3 * k + 2];
@@ -521,6 +533,12 @@ void Md_simulator::verify_settings() {
   for (auto *c : caviar_->constraints) {
     if (!c->device_capable())
       caviar_->log.error_all("only velocity-rescale constraints (constraint::Nve) run on the device path");
+    // The reference's Langevin integrator applies the thermostat between the
+    // first half-kick and the drift (md_simulator.cpp:766-775); the device
+    // step has no such stop, so the pairing is refused rather than run with
+    // the rescale at another point of the step.
+    if (integrator_type == Integrator_t::Velocity_verlet_langevin)
+      caviar_->log.error_all("constraint::Nve with Velocity_verlet_langevin is not supported on the device path");
     c->verify_settings();
   }
 }
@@ -541,8 +559,8 @@ void Md_simulator::sync_host_mirror() {
 
 void Md_simulator::setup() {
   if (use_time) {
-    initial_step = (int64_t)(initial_time / dt);
-    final_step = (int64_t)(final_time / dt);
+    initial_step = steps_of_time(initial_time, dt);
+    final_step = steps_of_time(final_time, dt);
   }
   if (caviar_->forces.empty()) caviar_->log.warning("Md_simulator::setup: force_field.size() = 0");
   if (integrator_type == Integrator_t::Velocity_verlet_langevin) {
@@ -729,9 +747,9 @@ Atom_writer::~Atom_writer() = default;
 
 int64_t Atom_writer::next_output_step(int64_t from) const {
   int64_t best = -1;
-  const std::pair<bool, int> outs[4] = {{output_xyz, xyz_step}, {output_energy, energy_step},
+  const std::pair<bool, int> outs[5] = {{output_xyz, xyz_step}, {output_energy, energy_step},
                                         {output_temperature, temperature_step},
-                                        {output_pressure, pressure_step}};
+                                        {output_pressure, pressure_step}, {output_pe, pe_step}};
   for (const auto &o : outs) {
     if (!o.first) continue;
     const int64_t e = next_multiple(from, o.second);
@@ -752,6 +770,10 @@ void Atom_writer::write(int64_t i, double t) {
   if (output_pressure && i % pressure_step == 0) {
     if (!ofs_pressure.is_open()) ofs_pressure.open(file_name_pressure + ".txt");
     ofs_pressure << i << " " << t << " " << atom_data->pressure() << std::endl;
+  }
+  if (output_pe && i % pe_step == 0) {
+    if (!ofs_pe.is_open()) ofs_pe.open(file_name_pe + ".txt");
+    ofs_pe << i << " " << t << " " << atom_data->potential_energy << std::endl;
   }
   if (output_xyz && i % xyz_step == 0) {
     if (!ofs_xyz.is_open()) ofs_xyz.open(file_name_xyz + ".xyz");

@@ -1,0 +1,113 @@
+// Drop-in checks: a user program of the caviar2 API (as the reference's own
+// tests/t00x programs are), driven by tests/test_dropin.py.
+//
+//   dropin_checks steps <time> <dt>          print steps_of_time(time, dt) (no device)
+//   dropin_checks run <input> <vv|lf> <steps> <cadence>
+//       read the system, run Velocity_verlet or Leap_frog with every writer
+//       (energy, temperature, pressure, potential energy, xyz with id,
+//       velocity and acceleration) at the cadence, print the final state
+//   dropin_checks neighlist <input> <all>
+//       build the list (Neighborlist::build(true)), materialise neighlist
+//       (all = 1: the all_atom_test form), print it and the ghosts
+//
+// Input: n L rc coef ghost_cutoff, then n lines "x y z vx vy vz" (box [0, L)^3,
+// periodic, one species eps = sigma = mass = 1).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+
+#include "caviar2/caviar2.hpp"
+#include "caviar2/tools/all.hpp"
+#include "caviar2/writer/atom_writer.hpp"
+
+using namespace caviar2;
+
+static void load(Caviar2 &sim, const char *path, double &rc, double &coef) {
+  std::ifstream in(path);
+  long n = 0;
+  double L = 0.0, gc = 0.0;
+  in >> n >> L >> rc >> coef >> gc;
+  sim.domain.set_lower_global(Vector3d<double>{0.0, 0.0, 0.0});
+  sim.domain.set_upper_global(Vector3d<double>{L, L, L});
+  sim.domain.set_boundary_condition(Vector3d<int>{1, 1, 1});
+  sim.domain.generate();
+  sim.atom_data.set_ghost_cutoff(gc);
+  for (long i = 0; i < n; ++i) {
+    double x, y, z, vx, vy, vz;
+    in >> x >> y >> z >> vx >> vy >> vz;
+    sim.atom_data.add_atom((size_t)i, 0, Vector3d<double>{x, y, z}, Vector3d<double>{vx, vy, vz});
+  }
+  sim.atom_data.add_type_mass(0, 1.0);
+  sim.atom_data.set_k_b(1.0);
+  sim.atom_data.set_n_r_df(3);
+  sim.atom_data.set_temperature_process(true);
+  sim.atom_data.set_pressure_process(true);
+  sim.neighborlist.set_cutoff(rc);
+  sim.neighborlist.set_cutoff_extra_coef(coef);
+}
+
+int main(int argc, char **argv) {
+  if (argc < 2) return 2;
+  const std::string mode = argv[1];
+  if (mode == "steps") {
+    std::printf("%lld\n", (long long)steps_of_time(std::atof(argv[2]), std::atof(argv[3])));
+    return 0;
+  }
+  Caviar2 sim;
+  double rc = 0.0, coef = 1.0;
+  load(sim, argv[2], rc, coef);
+  force_field::Lj lj(&sim);
+  lj.set_cutoff(rc);
+  lj.set_sigma(0, 0, 1.0);
+  lj.set_epsilon(0, 0, 1.0);
+  sim.add_force_field(&lj);
+  if (mode == "run") {
+    const bool lf = std::strcmp(argv[3], "lf") == 0;
+    const long steps = std::atol(argv[4]);
+    const int cad = std::atoi(argv[5]);
+    writer::Atom_writer out(&sim);
+    out.set_energy_step(cad);
+    out.set_temperature_step(cad);
+    out.set_pressure_step(cad);
+    out.set_potential_energy_step(cad);
+    out.set_xyz_step(cad);
+    out.set_xyz_output_id(1);
+    out.set_xyz_output_velocity(1);
+    out.set_xyz_output_acceleration(1);
+    out.set_file_name_energy("w_energy");
+    out.set_file_name_temperature("w_temperature");
+    out.set_file_name_pressure("w_pressure");
+    out.set_file_name_potential_energy("w_pe");
+    out.set_file_name_xyz("w_xyz");
+    sim.add_writer(&out);
+    sim.md_simulator.set_integrator_type(lf ? Integrator_t::Leap_frog : Integrator_t::Velocity_verlet);
+    sim.md_simulator.set_initial_step(0);
+    sim.md_simulator.set_final_step((size_t)steps);
+    sim.md_simulator.set_dt(0.005);
+    sim.md_simulator.run();
+    const auto &o = sim.atom_data.atom_struct_owned;
+    for (size_t i = 0; i < o.position.size(); ++i)
+      std::printf("%a %a %a %a %a %a\n", o.position[i].x, o.position[i].y, o.position[i].z,
+                  o.velocity[i].x, o.velocity[i].y, o.velocity[i].z);
+    return 0;
+  }
+  if (mode == "neighlist") {
+    sim.neighborlist.set_all_atom_test(std::atoi(argv[3]) != 0);
+    sim.neighborlist.build(true);
+    sim.neighborlist.update_host_list();
+    const auto &nl = sim.neighborlist.neighlist;
+    const auto &g = sim.atom_data.atom_struct_ghost;
+    std::printf("%zu %zu\n", nl.size(), g.position.size());
+    for (size_t q = 0; q < g.position.size(); ++q)
+      std::printf("%zu %a %a %a\n", g.id[q], g.position[q].x, g.position[q].y, g.position[q].z);
+    for (const auto &l : nl) {
+      std::printf("%zu", l.size());
+      for (unsigned j : l) std::printf(" %u", j);
+      std::printf("\n");
+    }
+    return 0;
+  }
+  return 2;
+}

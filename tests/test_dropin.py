@@ -116,3 +116,143 @@ def test_nve_example_rescales_on_device(tmp_path):
     assert [int(x) for x in t[:, 0]] == [0] + list(range(0, 500, 50))
     assert abs(t[0, 2] - 0.9) > 0.1  # the random start is far from the target
     assert np.allclose(t[1:, 2], 0.9, rtol=1e-12, atol=0)
+
+
+# ---- writers, Leap_frog, host neighlist, all_atom_test, time -> steps ------
+
+def test_time_to_step_matches_reference_expression(tmp_path):
+    """Md_simulator::setup converts times with (int)time / dt
+    (md_simulator.cpp:569-572): the time is truncated before the division."""
+    exe = _compile(os.path.join(ROOT, "tests", "cpp", "dropin_checks.cpp"), str(tmp_path / "dc"))
+    for t, dt, want in ((2.5, 0.005, 400), (10.0, 0.005, 2000), (0.9, 0.001, 0), (7.99, 0.5, 14)):
+        out = subprocess.run([exe, "steps", repr(t), repr(dt)], capture_output=True, text=True, check=True)
+        assert int(out.stdout) == want == int(int(t) / dt)
+
+
+def _small_fluid():
+    """256 atoms (4^3 FCC cells, rho* = 0.8442) perturbed, T* = 1.44; rc 2.5,
+    list coefficient 1.12, ghost cutoff 2.8."""
+    from cv2md.systems import lj_fluid
+    s = lj_fluid(4, temperature=1.44, seed=61)
+    rng = np.random.default_rng(61)
+    box = O.make_box(s["lo"], s["hi"])
+    s["pos"], _ = O.wrap(s["pos"] + rng.normal(0.0, 0.05, s["pos"].shape), box)
+    return s
+
+
+def _write_input(path, s):
+    L = float(s["hi"][0])
+    with open(path, "w") as f:
+        f.write(f"{len(s['pos'])} {L!r} {s['nl_cutoff']!r} {s['nl_coef']!r} {s['ghost_cutoff']!r}\n")
+        for p, v in zip(s["pos"], s["vel"]):
+            f.write(" ".join(repr(float(x)) for x in (*p, *v)) + "\n")
+
+
+def _read_series(path):
+    return np.loadtxt(path, ndmin=2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integ", ["vv", "lf"])
+def test_writers_and_leapfrog_match_oracle(integ, tmp_path):
+    """A drop-in run with every writer at a cadence of 10 over 50 steps:
+    - the files have the reference's formats ("step time value" lines,
+      dump_energy.cpp:42-53; xyz frames "N", "Atom", "type id x y z vx vy vz
+      ax ay az", dump_xyz_serial.cpp:51-64), written at setup and after the
+      steps 0, 10, ..., 40 (md_simulator.cpp:607,726);
+    - kinetic energy, temperature, pressure (kinetic_energy.cpp:206-303),
+      potential energy and positions agree with the oracle's run;
+    - Leap_frog (kick-drift-kick, md_simulator.cpp:871-980) gives the
+      velocity-Verlet trajectory."""
+    s = _small_fluid()
+    inp = tmp_path / "in.txt"
+    _write_input(inp, s)
+    exe = _compile(os.path.join(ROOT, "tests", "cpp", "dropin_checks.cpp"), str(tmp_path / "dc"))
+    r = subprocess.run([exe, "run", str(inp), integ, "50", "10"], capture_output=True, text=True,
+                       cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # the final state: lines of six hex floats (the log's info lines share stdout)
+    rows = [l.split() for l in r.stdout.split("\n")]
+    fin = np.array([[float.fromhex(x) for x in t] for t in rows
+                    if len(t) == 6 and all(x.lstrip("-").startswith("0x") for x in t)])
+    assert fin.shape == (len(s["pos"]), 6)
+    n = len(s["pos"])
+    V = float(np.prod(s["hi"] - s["lo"]))
+    o_state = {}
+    with O.OracleSim(dict(s, use_cell_list=1)) as o:
+        o.setup()
+        o_state[-1] = (o.thermo(), o.state()[0])
+        for k in range(50):
+            o.step()
+            if k % 10 == 0 or k == 49:
+                o_state[k] = (o.thermo(), o.state())
+    steps = [-1, 0, 10, 20, 30, 40]  # -1: the setup line, written as step 0
+    e = _read_series(tmp_path / "w_energy.txt")
+    t = _read_series(tmp_path / "w_temperature.txt")
+    pr = _read_series(tmp_path / "w_pressure.txt")
+    pe = _read_series(tmp_path / "w_pe.txt")
+    for arr in (e, t, pr, pe):
+        assert arr.shape == (6, 3)
+        assert list(arr[:, 0]) == [0, 0, 10, 20, 30, 40]
+        assert np.allclose(arr[:, 1], [0.0, 0.0, 0.05, 0.1, 0.15, 0.2], rtol=0, atol=1e-12)
+    # the values are printed with the stream's default 6 significant digits,
+    # as the reference's writers print them
+    for row, k in enumerate(steps):
+        th = o_state[k][0]
+        assert e[row, 2] == pytest.approx(th["ke"], rel=6e-6)
+        assert t[row, 2] == pytest.approx(th["temperature"], rel=6e-6)
+        p_ref = n * th["temperature"] / V + th["virial"] / (3.0 * V)
+        assert pr[row, 2] == pytest.approx(p_ref, rel=6e-6)
+        assert pe[row, 2] == pytest.approx(th["pe"], rel=6e-6)
+    # xyz frames
+    lines = open(tmp_path / "w_xyz.xyz").read().split("\n")
+    frames = []
+    at = 0
+    while at < len(lines) and lines[at].strip():
+        cnt = int(lines[at])
+        assert lines[at + 1] == "Atom"
+        rows = np.array([[float(x) for x in l.split()] for l in lines[at + 2:at + 2 + cnt]])
+        assert rows.shape == (cnt, 11)
+        frames.append(rows)
+        at += 2 + cnt
+    assert len(frames) == 6 and all(len(f) == n for f in frames)
+    for f, k in zip(frames[1:], steps[1:]):
+        p_o = o_state[k][1][0]
+        assert np.array_equal(f[:, 0], np.zeros(n)) and np.array_equal(f[:, 1], np.arange(n))
+        assert np.allclose(f[:, 2:5], p_o, rtol=6e-6, atol=1e-6)
+    pf, vf = fin[:, :3], fin[:, 3:]
+    po, vo, _ = o_state[49][1]
+    assert np.abs(pf - po).max() < 1e-10 and np.abs(vf - vo).max() < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("all_pairs", [0, 1])
+def test_host_neighlist_matches_oracle(all_pairs, tmp_path):
+    """Neighborlist::update_host_list after a forced build: the ghosts are the
+    reference's single-domain images in its order
+    (exchange_ghost_single_md_domain.cpp:67-215), and neighlist equals the
+    reference's Verlet list (neighborlist.cpp:247-297: owned j > i within the
+    list cutoff, then ghosts as N + g), or with all_atom_test its all-pairs
+    list (neighborlist.cpp:170-200)."""
+    s = _small_fluid()
+    inp = tmp_path / "in.txt"
+    _write_input(inp, s)
+    exe = _compile(os.path.join(ROOT, "tests", "cpp", "dropin_checks.cpp"), str(tmp_path / "dc"))
+    r = subprocess.run([exe, "neighlist", str(inp), str(all_pairs)], capture_output=True, text=True,
+                       cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = [l for l in r.stdout.split("\n") if not l.startswith("[")]  # without the log's lines
+    n, ng = (int(x) for x in out[0].split())
+    gid = np.array([int(l.split()[0]) for l in out[1:1 + ng]])
+    gpos = np.array([[float.fromhex(x) for x in l.split()[1:]] for l in out[1:1 + ng]])
+    lists = [[int(x) for x in l.split()[1:]] for l in out[1 + ng:1 + ng + n]]
+    box = O.make_box(s["lo"], s["hi"])
+    g_ref, owner, _ = O.ghosts(s["pos"], box, s["ghost_cutoff"])
+    assert np.array_equal(gpos, g_ref) and np.array_equal(gid, owner)
+    if all_pairs:
+        for i, l in enumerate(lists):
+            assert l == list(range(i + 1, n)) + list(range(n, n + ng))
+        return
+    offs, nbr = O.nlist_brute(s["pos"], g_ref, s["nl_cutoff"], s["nl_coef"])
+    for i, l in enumerate(lists):
+        assert l == sorted(int(x) for x in nbr[offs[i]:offs[i + 1]])
