@@ -170,6 +170,44 @@ def run_ensemble(steps, seed=1004):
                        "dt=0.005, one captured HIP graph per step")
 
 
+def run_share(steps, warmup, seed=11):
+    """One MPI domain's share with its halo refreshed from the host every
+    step: 1,000,000-atom Kob-Andersen mixture (rho* = 1.2, 80:20,
+    rc_ij = 2.5 sigma_ij, 2.8 sigma halo) from random positions, FP64
+    jump_fix relaxation (untimed), then Langevin quench steps (T* = 0.5,
+    gamma = 0.5, dt = 0.003), FP32 + fixed point.  A timed step includes the
+    device halo packing, the device -> host -> device copies of the packets
+    and the host's forwarding (PCIe-inclusive)."""
+    from cv2md.share import Share, ka_random
+
+    s = ka_random(1000000)
+    n = len(s["pos"])
+    with Share(s, precision=D.CV2_PREC_FP32_FIXED) as sh:
+        t0 = time.perf_counter()
+        sh.relax()
+        t_relax = time.perf_counter() - t0
+        sh.run(warmup, 0.003, True, 0.5, 0.5, seed)
+        sh.c.synchronize()
+        st0 = sh.c.stats()
+        sh.exchange_seconds, sh.exchanges = 0.0, 0
+        t0 = time.perf_counter()
+        sh.run(steps, 0.003, True, 0.5, 0.5, seed)
+        sh.c.synchronize()
+        el = time.perf_counter() - t0
+        st = sh.c.stats()
+        th = sh.c.thermo(0)
+        ex_ms = 1e3 * sh.exchange_seconds / max(sh.exchanges, 1)
+        ghosts = sh.halo_atoms
+    return dict(value=n * steps / el, unit="atom-timesteps/s", ms_per_step=1e3 * el / steps,
+                exchange_ms_per_step=ex_ms, steps=steps, warmup=warmup, natoms=n, ghosts=ghosts,
+                rebuilds=st["rebuilds"] - st0["rebuilds"], build_kind=st["build_kind"],
+                relax_seconds=t_relax, temperature_end=2.0 * th["ke"] / (3.0 * n),
+                config="Kob-Andersen 80:20, 1,000,000 owned atoms, rho*=1.2, rc_ij=2.5 sigma_ij, 2.8 sigma "
+                       "halo packed on the device and forwarded by the host every step (1x1x1 periodic "
+                       "process grid); random start, FP64 jump_fix relaxation, Langevin quench T*=0.5, "
+                       "gamma=0.5, dt=0.003; FP32 + 64-bit fixed point; PCIe-inclusive")
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lj_f32_task_lj2m_traffic.txt")
 
 
@@ -244,6 +282,10 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=2000)
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the batched-ensemble workload and print its JSON (for profiling)")
+    ap.add_argument("--no-share", action="store_true")
+    ap.add_argument("--share-steps", type=int, default=100)
+    ap.add_argument("--share-only", action="store_true",
+                    help="run only the host-halo share workload and print its JSON (for profiling)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only the force kernel and a calibration copy (for rocprofv3 --pmc)")
     a = ap.parse_args()
@@ -254,6 +296,9 @@ def main():
 
     if a.traffic_probe:
         traffic_probe(a)
+        return
+    if a.share_only:
+        print(json.dumps(run_share(a.share_steps, 10)), flush=True)
         return
     if a.ensemble_only:
         r = run_ensemble(a.ens_steps)
@@ -319,11 +364,16 @@ def main():
                      "avg_launch_ms": t_force * 1e3, **committed_counters()},
         "cpu_baseline": None,
         "ensemble": None,
+        "share": None,
     }
     if not a.no_ensemble:
         out["ensemble"] = run_ensemble(a.ens_steps)
         log(f"ensemble: {out['ensemble']['ms_per_step']:.4f} ms/step, {out['ensemble']['value']:.3e} "
             f"atom-steps/s, tuning {out['ensemble']['tuning']}")
+    if not a.no_share:
+        out["share"] = run_share(a.share_steps, 10)
+        log(f"share: {out['share']['ms_per_step']:.4f} ms/step ({out['share']['exchange_ms_per_step']:.3f} ms "
+            f"exchange), {out['share']['value']:.3e} atom-steps/s")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         log(f"cpu: {out['cpu_baseline']}")

@@ -102,6 +102,8 @@ struct cv2_ctx {
   long long *h_halo = nullptr;  // cv2_halo_pack_noalloc: the 28 results of the device scan
   int partial_blocks = 0;
   bool fuse = true;  // integrator fused into the FP32 task kernel where it runs
+  bool step_open = false;  // cv2_step_begin ran, cv2_step_end has not
+  UpdateArgs open_args{};
   int min_mpr = 0;   // task slots per cell row asked for after a row ran out of them
   int task_grid_cap = 0;  // CV2_TASK_GRID: cap of the force launch's grid (tests: tail launch)
   // HIP-event timing of kernel groups (cv2_set_timing): every group, or
@@ -224,6 +226,7 @@ int setup_box(cv2_ctx *ctx) {
     b.L[d] = ctx->hi[d] - ctx->lo[d];
     b.halfL[d] = 0.5 * b.L[d];
     b.bc[d] = ctx->host_halo ? 0 : ctx->bc[d];
+    b.wrap[d] = ctx->bc[d];
     if (ctx->bc[d] && !ctx->host_halo && b.L[d] <= 2.0 * ce)
       return fail(ctx, CV2_ERR_BAD_ARG, "periodic box must be wider than twice the list cutoff");
   }
@@ -822,6 +825,7 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
   ctx->atoms = true;
   ctx->need_build = true;
   ctx->forces_valid = false;
+  ctx->step_open = false;
   ctx->steps = 0;
   return CV2_OK;
 }
@@ -1179,6 +1183,7 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
 
 int cv2_run_vv(cv2_ctx *ctx, int64_t nsteps, double dt) {
   if (!ctx || nsteps < 0 || dt < 0) return fail(ctx, CV2_ERR_BAD_ARG, "bad step arguments");
+  if (ctx->step_open) return fail(ctx, CV2_ERR_STATE, "a step is open: cv2_step_end first");
   int rc = ensure_ready(ctx);
   if (rc) return rc;
   if (!ctx->forces_valid) {
@@ -1195,6 +1200,7 @@ int cv2_run_langevin(cv2_ctx *ctx, int64_t nsteps, double dt, double gamma, doub
                      uint64_t seed) {
   if (!ctx || nsteps < 0 || dt < 0) return fail(ctx, CV2_ERR_BAD_ARG, "bad step arguments");
   if (kbt < 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "kbt have to non-negative.");
+  if (ctx->step_open) return fail(ctx, CV2_ERR_STATE, "a step is open: cv2_step_end first");
   int rc = ensure_ready(ctx);
   if (rc) return rc;
   if (!ctx->forces_valid) {
@@ -1210,6 +1216,65 @@ int cv2_run_langevin(cv2_ctx *ctx, int64_t nsteps, double dt, double gamma, doub
   u.lc = 2.0 * dt / (2.0 + gamma * dt);
   u.seed = seed;
   return run_steps(ctx, nsteps, u);
+}
+
+int cv2_step_begin(cv2_ctx *ctx, double dt, int langevin, double gamma, double kbt, uint64_t seed) {
+  if (!ctx || dt < 0) return fail(ctx, CV2_ERR_BAD_ARG, "bad step arguments");
+  if (langevin && kbt < 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "kbt have to non-negative.");
+  if (ctx->step_open) return fail(ctx, CV2_ERR_STATE, "a step is open: cv2_step_end first");
+  int rc = ensure_ready(ctx);
+  if (rc) return rc;
+  if (!ctx->forces_valid) {
+    rc = cv2_compute_forces(ctx);
+    if (rc) return rc;
+  }
+  (void)hipSetDevice(ctx->device);
+  UpdateArgs u{};
+  u.langevin = langevin != 0;
+  u.dt = dt;
+  if (u.langevin) {  // md_simulator.cpp:398-405
+    u.la = (2.0 - gamma * dt) / (2.0 + gamma * dt);
+    u.lb = std::sqrt(kbt * gamma * 0.5 * dt);
+    u.lc = 2.0 * dt / (2.0 + gamma * dt);
+  }
+  u.seed = seed;
+  u.writeq = ctx->precision != CV2_PREC_FP64;
+  u.ref_rebuild = ref_rebuilds(ctx);
+  // the host packs the halo from the positions this half leaves: owned
+  // atoms are back in the box by then
+  u.wrap_owned = ctx->host_halo;
+  u.finish = false;
+  u.start = true;
+  u.step = (uint64_t)ctx->steps - 1;  // the start half draws the noise of step ctx->steps
+  launch_set_flag(ctx->s.flags, F_STEP, 0, ctx->stream);
+  launch_update(ctx->s, ctx->b, u, ctx->stream);
+  ctx->open_args = u;
+  ctx->step_open = true;
+  ctx->forces_valid = false;
+  return check_flags(ctx);
+}
+
+int cv2_step_end(cv2_ctx *ctx) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  if (!ctx->step_open) return fail(ctx, CV2_ERR_STATE, "no open step: cv2_step_begin first");
+  (void)hipSetDevice(ctx->device);
+  int rc = queue_build(ctx, false, false);
+  if (rc) return rc;
+  rc = forces_pass(ctx, true);
+  if (rc) return rc;
+  UpdateArgs u = ctx->open_args;
+  u.finish = true;
+  u.start = false;
+  u.step = (uint64_t)ctx->steps;
+  launch_update(ctx->s, ctx->b, u, ctx->stream);
+  ctx->step_open = false;
+  ctx->steps += 1;
+  rc = check_flags(ctx);
+  if (rc) return rc;
+  rc = fetch_energy(ctx);
+  if (rc) return rc;
+  ctx->forces_valid = true;
+  return CV2_OK;
 }
 
 int cv2_thermo(cv2_ctx *ctx, int n_r_df, double *ke, double *pe, double *virial) {
@@ -1264,7 +1329,7 @@ int cv2_download(cv2_ctx *ctx, double *pos, double *vel, double *acc) {
   // domain.cpp:443-493), which a build would apply to the same value.
   const BoxDev &b = ctx->b;
   auto wrap = [&](double c, int d) {
-    if (b.bc[d] != 1) return c;
+    if (b.wrap[d] != 1) return c;
     if (c < b.lo[d]) return c + b.L[d];
     if (c > b.hi[d]) return c - b.L[d];
     return c;

@@ -152,6 +152,7 @@ BoxDev system_box(const double lo[3], const double hi[3], double ce, double rc) 
     b.L[d] = hi[d] - lo[d];
     b.halfL[d] = 0.5 * b.L[d];
     b.bc[d] = 1;
+    b.wrap[d] = 1;
     b.glo[d] = lo[d];
     // Two cells per axis suffice: with 2, the stencil meets a cell twice,
     // under two images L apart, and as L > 2 ce at most one image of an atom

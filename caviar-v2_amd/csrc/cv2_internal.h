@@ -73,6 +73,8 @@ struct BoxDev {
   double qscale[3]; // fixed-point units per length (set_qscale)
   double qinv[3];   // length per fixed-point unit
   int bc[3];        // 1 = periodic (images generated on the device)
+  int wrap[3];      // 1 = owned atoms are wrapped into [lo, hi) at list builds: periodic
+                    // axes, also of a host-halo share that is its own neighbour
   int nc[3];
   // Coarse filter of the wave-per-cell list build (fully periodic boxes,
   // coarse = 1), on the 32-bit fixed-point coordinates q: a candidate whose
@@ -200,6 +202,11 @@ struct StepParams {
   // the last build; open axes carry infinite bounds
   int ref_rebuild;
   double wlo[3], whi[3], blo[3], bhi[3];
+  // the start half of cv2_step_begin on a host-halo box wraps owned atoms
+  // that left the box back in, on the axes with wL[d] > 0 (the owned
+  // exchange that precedes the ghost exchange, md_simulator.cpp:547-557)
+  int wrap;
+  double hi[3], wL[3];
 };
 
 // The reference's other rebuild triggers for one atom at x, built at r:
@@ -272,10 +279,23 @@ __device__ inline void integrate_store(int i, int n, const double4 &x, const dou
   if (isnan(x.x) | isnan(x.y) | isnan(x.z) | isnan(v[0]) | isnan(v[1]) | isnan(v[2]))
     raise_fatal(flags, F_NAN);
   if (START) {
-    pos[i] = x;
-    if (WRITEQ) q[i] = pack_q(x.x, x.y, x.z, (int)x.w, p.lo, p.qscale);
-    const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
-    if (dx * dx + dy * dy + dz * dz > p.thresh_sq || (p.ref_rebuild && ref_trigger(x, r, p)))
+    double4 y = x;
+    if (p.wrap) {  // same rule as the wrap at list builds (k_wrap_bin)
+      double *c[3] = {&y.x, &y.y, &y.z};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (!(p.wL[d] > 0.0)) continue;
+        if (*c[d] < p.lo[d])
+          *c[d] += p.wL[d];
+        else if (*c[d] > p.hi[d])
+          *c[d] -= p.wL[d];
+      }
+    }
+    pos[i] = y;
+    if (WRITEQ) q[i] = pack_q(y.x, y.y, y.z, (int)y.w, p.lo, p.qscale);
+    // a wrapped atom has moved by a box length: the list is rebuilt
+    const double dx = y.x - r.x, dy = y.y - r.y, dz = y.z - r.z;
+    if (dx * dx + dy * dy + dz * dz > p.thresh_sq || (p.ref_rebuild && ref_trigger(y, r, p)))
       atomicOr(&flags[F_REBUILD], 1);
   }
   vel[i] = v[0];
@@ -462,6 +482,7 @@ constexpr int kTaskBlock = 256;
 struct UpdateArgs {
   bool langevin, finish, start, writeq;
   bool ref_rebuild;  // CV2_NL_REFERENCE_REBUILDS
+  bool wrap_owned;   // StepParams::wrap
   double dt, la, lb, lc;
   uint64_t seed, step;
 };

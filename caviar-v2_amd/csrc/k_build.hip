@@ -65,7 +65,7 @@ __global__ void k_wrap_bin(double4 *__restrict__ pos, int n, BoxDev b, int *__re
     double *c[3] = {&p.x, &p.y, &p.z};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      if (b.bc[d] != 1) continue;
+      if (b.wrap[d] != 1) continue;
       if (*c[d] < b.lo[d])
         *c[d] += b.L[d];
       else if (*c[d] > b.hi[d])

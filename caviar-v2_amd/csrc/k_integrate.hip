@@ -191,8 +191,11 @@ StepParams step_params(const BoxDev &b, const UpdateArgs &u) {
     p.whi[d] = per ? b.hi[d] : inf;
     p.blo[d] = per ? b.lo[d] + b.ghost_cut : -inf;
     p.bhi[d] = per ? b.hi[d] - b.ghost_cut : inf;
+    p.hi[d] = b.hi[d];
+    p.wL[d] = b.wrap[d] == 1 ? b.L[d] : 0.0;
   }
   p.ref_rebuild = u.ref_rebuild ? 1 : 0;
+  p.wrap = u.wrap_owned ? 1 : 0;
   return p;
 }
 

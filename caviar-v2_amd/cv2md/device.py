@@ -90,6 +90,8 @@ def lib():
             "cv2_compute_forces": ([vp], C.c_int),
             "cv2_compute_accelerations": ([vp], C.c_int),
             "cv2_run_vv": ([vp, C.c_int64, C.c_double], C.c_int),
+            "cv2_step_begin": ([vp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_uint64], C.c_int),
+            "cv2_step_end": ([vp], C.c_int),
             "cv2_run_langevin": ([vp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint64], C.c_int),
             "cv2_thermo": ([vp, C.c_int, _d, _d, _d], C.c_int),
             "cv2_download": ([vp, _d, _d, _d], C.c_int),
@@ -242,6 +244,16 @@ class Context:
 
     def run_langevin(self, nsteps, dt, gamma, kbt, seed):
         self._chk(self.L.cv2_run_langevin(self.h, int(nsteps), float(dt), float(gamma), float(kbt), int(seed)))
+
+    def step_begin(self, dt, langevin=False, gamma=0.0, kbt=0.0, seed=0):
+        """First half of one step (velocities, positions); the halo may be
+        exchanged before step_end."""
+        self._chk(self.L.cv2_step_begin(self.h, float(dt), int(bool(langevin)), float(gamma), float(kbt),
+                                        int(seed)))
+
+    def step_end(self):
+        """List build when needed, forces, second velocity half."""
+        self._chk(self.L.cv2_step_end(self.h))
 
     def synchronize(self):
         self._chk(self.L.cv2_synchronize(self.h))

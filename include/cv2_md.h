@@ -113,8 +113,11 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
 /* Host-supplied ghost halo for one domain's share (the images that
  * exchange_ghost_mpi_shared_atoms.cpp:274-311 would deliver).  Switches the
  * context to host-halo mode: the box is then treated as open and only these
- * ghosts provide neighbours across its faces.  Call again each step with the
- * refreshed halo; a changed count forces a rebuild (neighborlist.cpp:118-121). */
+ * ghosts provide neighbours across its faces; owned atoms on the axes set
+ * periodic in cv2_set_box are still wrapped into the box at list builds (a
+ * share that is its own neighbour, as in a 1x1x1 process grid).  Call again
+ * each step with the refreshed halo (between cv2_step_begin and
+ * cv2_step_end); a changed count forces a rebuild (neighborlist.cpp:118-121). */
 int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *type,
                    const int64_t *id);
 
@@ -204,6 +207,22 @@ int cv2_run_vv(cv2_ctx *ctx, int64_t nsteps, double dt);
  * keyed by (seed, atom id, step) instead of the reference's mt19937 streams. */
 int cv2_run_langevin(cv2_ctx *ctx, int64_t nsteps, double dt, double gamma, double kbt,
                      uint64_t seed);
+
+/* One step in two halves, for callers that exchange the halo of a domain's
+ * share within the step, as Md_simulator does in boundary_condition between
+ * the drift and the forces (md_simulator.cpp:422-437; the MPI exchange of
+ * exchange_ghost_mpi_shared_atoms.cpp): cv2_step_begin applies the first
+ * half of integrate_velocity_verlet (langevin = 0, md_simulator.cpp:654) or
+ * integrate_velocity_verlet_langevin (langevin = 1, :778,835; a, b, c of
+ * :398-405), velocities and positions, and the rebuild test; the caller may
+ * then replace the ghosts (cv2_set_ghosts, cv2_halo_unpack) or pack its own
+ * halo (cv2_halo_pack); cv2_step_end builds the list when needed, computes
+ * the forces at the new positions (re_calculate_acc, :422-437) and applies
+ * the second velocity half (:678 / :860).  Together they give the bits of
+ * cv2_run_vv / cv2_run_langevin with nsteps = 1.  cv2_run_* refuse to run
+ * while a step is open. */
+int cv2_step_begin(cv2_ctx *ctx, double dt, int langevin, double gamma, double kbt, uint64_t seed);
+int cv2_step_end(cv2_ctx *ctx);
 
 /* Thermodynamics of the current state: kinetic energy (kinetic_energy.cpp:
  * 26-70, n_r_df as set), potential energy and virial of the last force pass

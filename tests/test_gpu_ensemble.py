@@ -163,3 +163,41 @@ def test_tuner_keeps_state():
         for a, b in zip(before, after):
             assert np.array_equal(a, b)
         e.run_vv(50, DT)
+
+
+def test_c4_workload_512_systems():
+    """The benchmark's ensemble at its real shape: 512 systems of 500 to
+    20,000 atoms (seed 1004, about 2.7M atoms), tuned by the autotuner, then
+    200 velocity-Verlet steps through the captured step graph and a list
+    rebuild of every system.  For a seeded sample of 40 systems that
+    includes the smallest and the largest: the pair set is bit-exact against
+    the oracle's cell list at the device's positions, and the FP32 +
+    fixed-point accelerations lie within 1e-6 of the FP64 oracle on the
+    metric of bench.py (per atom |a32 - a64| / max(|a64|, rms|a64|))."""
+    systems = ensemble_systems(512, seed=1004)
+    sizes = np.array([len(s["pos"]) for s in systems])
+    rng = np.random.default_rng(1004)
+    sample = sorted({int(sizes.argmin()), int(sizes.argmax())} |
+                    set(int(x) for x in rng.choice(len(systems), 38, replace=False)))
+    with D.Ensemble.from_systems(systems, rc=RC, nl_cutoff=RC, nl_coef=NL_COEF) as e:
+        tuned = e.tune(DT, trial_steps=10)
+        e.run_vv(200, DT)
+        st = e.stats()
+        e.build()
+        pos, _, acc = e.download()
+        pp, aa = split(e, pos), split(e, acc)
+        print(f"[c4] {len(systems)} systems, {int(sizes.sum())} atoms, tuning {tuned}, "
+              f"system builds {st['system_builds']}, graph nodes {st['graph_nodes']}", flush=True)
+        worst = 0.0
+        for s in sample:
+            sd = sysdef(dict(systems[s], pos=pp[s]))
+            got = sort_rows(e.pairs(s))
+            exp = oracle_pairs(pp[s], sd)
+            assert np.array_equal(got, exp), s
+            _p, ref, _vir, _pe = oracle_forces(pp[s], sd)
+            emax, erms = force_error(aa[s], ref)
+            worst = max(worst, emax)
+            assert emax <= 1e-6 and erms <= 2.5e-7, (s, sizes[s], emax, erms)
+        print(f"[c4] sample of {len(sample)} systems ({sizes[sample].min()}..{sizes[sample].max()} "
+              f"atoms): pairs bit-exact, worst force error {worst:.3e}", flush=True)
+    assert st["graph_nodes"] > 0 and st["system_builds"] > len(systems)

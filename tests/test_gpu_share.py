@@ -1,0 +1,75 @@
+"""One MPI domain's share with its halo refreshed from the host every step.
+
+1,000,000-atom Kob-Andersen mixture (rho* = 1.2, 80:20, rc_ij = 2.5 sigma_ij,
+2.8 sigma halo) from uniformly random positions, FP64 jump_fix relaxation
+(lj.cpp:453-468), then Langevin quench steps (T* = 0.5, gamma = 0.5,
+dt = 0.003) in FP32 + fixed point, each step: start half, owned wrap, halo
+packets on the device, host forwarding, ghost upload, list build, forces,
+finishing half (cv2md.share).
+
+After each of the first quench steps (every one rebuilds: the ghost count
+changes, neighborlist.cpp:117-120):
+- the halo packets are the oracle's (exchange_ghost_mpi_shared_atoms
+  restated) bit for bit, at the device's owned positions;
+- the pair set over owned atoms and the uploaded ghosts is the oracle's
+  cell-list set, bit-exact;
+- the accelerations lie within the FP32 + fixed-point tolerance of the FP64
+  oracle's LJ pass over the same lists.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from cv2md import device as D
+from cv2md.share import Share, ka_random
+from helpers import force_error, pair_keys
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_keys(pos, gpos, box, s):
+    offs, nbr = O.nlist_cell(pos, gpos, box, s["nl_cutoff"], s["nl_coef"])
+    n = len(pos)
+    k = np.repeat(np.arange(n, dtype=np.int64), np.diff(offs)) * n
+    k += nbr.astype(np.int64)
+    k *= 27
+    k += 13
+    return offs, nbr, np.sort(k)
+
+
+def test_c5_quench_halo_pairs_forces():
+    s = ka_random(1000000)
+    n = len(s["pos"])
+    box = O.make_box(s["lo"], s["hi"], (1, 1, 1))
+    t = np.asarray(s["type"], np.int32)
+    with Share(s, precision=D.CV2_PREC_FP32_FIXED) as sh:
+        sh.relax()
+        print(f"[c5] relaxed, {sh.halo_atoms} ghosts", flush=True)
+        for k in range(3):
+            r0 = sh.c.stats()["rebuilds"]
+            sh.c.step_begin(0.003, True, 0.5, 0.5, 11)
+            pos_b, vel_b, _ = sh.c.download()
+            counts, pk = sh.c.halo_pack((0, 0, 0), (1, 1, 1), s["lo"], s["hi"], sh.L, (1, 1, 1),
+                                        s["ghost_cutoff"])
+            sh.exchange()
+            sh.c.step_end()
+            assert sh.c.stats()["rebuilds"] > r0
+            # packets against the oracle's at the positions they were packed from
+            assert np.all((pos_b >= s["lo"]) & (pos_b <= s["hi"]))
+            ec, ep = O.halo_pack(pos_b, vel_b, np.arange(n), t, (0, 0, 0), (1, 1, 1), s["lo"], s["hi"],
+                                 sh.L, (1, 1, 1), s["ghost_cutoff"])
+            assert np.array_equal(counts, ec)
+            assert np.array_equal(pk, ep)
+            pos, _, acc = sh.c.download()
+            gpos, gtype, gid = sh.ghosts()
+            kg = pair_keys(sh.c.pairs(), n)
+            offs, nbr, ke = oracle_keys(pos, gpos, box, s)
+            print(f"[c5] step {k}: {len(gpos)} ghosts, {len(kg)} entries", flush=True)
+            assert len(kg) == len(ke)
+            assert np.array_equal(kg, ke)
+            del kg, ke
+            a_ref, _, _ = O.lj(pos, t, np.arange(n), gpos, gtype, gid, offs, nbr, s["eps"], s["sigma"],
+                               s["lj_cutoff"], s["mass"], s["cutoff_list"])
+            emax, erms = force_error(acc, a_ref)
+            print(f"[c5] step {k}: force error max {emax:.2e} rms {erms:.2e}", flush=True)
+            assert emax < 2e-5 and erms < 5e-6
