@@ -189,7 +189,7 @@ def run_share(steps, warmup, seed=11):
         sh.run(warmup, 0.003, True, 0.5, 0.5, seed)
         sh.c.synchronize()
         st0 = sh.c.stats()
-        sh.exchange_seconds, sh.exchanges = 0.0, 0
+        sh.exchange_seconds, sh.pack_seconds, sh.unpack_seconds, sh.exchanges = 0.0, 0.0, 0.0, 0
         t0 = time.perf_counter()
         sh.run(steps, 0.003, True, 0.5, 0.5, seed)
         sh.c.synchronize()
@@ -197,9 +197,12 @@ def run_share(steps, warmup, seed=11):
         st = sh.c.stats()
         th = sh.c.thermo(0)
         ex_ms = 1e3 * sh.exchange_seconds / max(sh.exchanges, 1)
+        pack_ms = 1e3 * sh.pack_seconds / max(sh.exchanges, 1)
+        unpack_ms = 1e3 * sh.unpack_seconds / max(sh.exchanges, 1)
         ghosts = sh.halo_atoms
     return dict(value=n * steps / el, unit="atom-timesteps/s", ms_per_step=1e3 * el / steps,
-                exchange_ms_per_step=ex_ms, steps=steps, warmup=warmup, natoms=n, ghosts=ghosts,
+                exchange_ms_per_step=ex_ms, pack_ms_per_step=pack_ms, unpack_ms_per_step=unpack_ms,
+                steps=steps, warmup=warmup, natoms=n, ghosts=ghosts,
                 rebuilds=st["rebuilds"] - st0["rebuilds"], build_kind=st["build_kind"],
                 relax_seconds=t_relax, temperature_end=2.0 * th["ke"] / (3.0 * n),
                 config="Kob-Andersen 80:20, 1,000,000 owned atoms, rho*=1.2, rc_ij=2.5 sigma_ij, 2.8 sigma "

@@ -99,7 +99,8 @@ struct cv2_ctx {
   int *h_flags = nullptr;
   double *h_red = nullptr;
   long long *h_fx = nullptr;
-  long long *h_halo = nullptr;  // cv2_halo_pack_noalloc: the 28 results of the device scan
+  long long *h_halo = nullptr;  // halo packing: the 28 results of the device scan
+  long long *halo_res = nullptr;  // cv2_halo_pack: the device scan's 28 results
   int partial_blocks = 0;
   bool fuse = true;  // integrator fused into the FP32 task kernel where it runs
   bool step_open = false;  // cv2_step_begin ran, cv2_step_end has not
@@ -194,7 +195,8 @@ void free_state(DevState &s) {
                   s.tag_alt, s.id, s.id_alt, s.cell, s.slot, s.perm, s.order, s.box_dev, s.wstat, s.cell_count, s.cell_start, s.cell_local,
                   s.gcell_count, s.gcell_start, s.gcell_items, s.tile_sum, s.nbr, s.nnbr, s.flags,
                   s.partials, s.fx_acc, s.reduce_out, s.tpiece, s.tinfo, s.ctask, s.rowtasks,
-                  s.ctinfo, s.ctpiece, s.q_alt};
+                  s.ctinfo, s.ctpiece, s.q_alt, s.gtag, s.gslot, s.gtag_alt, s.gid_alt,
+                  s.gpos_alt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   s = DevState{};
@@ -231,11 +233,17 @@ int setup_box(cv2_ctx *ctx) {
       return fail(ctx, CV2_ERR_BAD_ARG, "periodic box must be wider than twice the list cutoff");
   }
   // Fully periodic boxes with at least 5 cells of width >= ce per axis use
-  // the wave-per-cell build with the coarse filter (k_nlist_cells); every
-  // other box uses the per-atom build over cells at least ce/2 wide.
-  bool fast = !ctx->host_halo;
-  for (int d = 0; d < 3; ++d)
-    fast = fast && b.bc[d] == 1 && std::floor(b.L[d] / (ce * (1.0 + 1e-12))) >= 5.0;
+  // the wave-per-cell build with the coarse filter (k_nlist_cells), and so
+  // do host-halo boxes whose grid over the box and its ghost margin has at
+  // least 3 such cells per axis (open grid: the ghosts are sorted into it);
+  // every other box uses the per-atom build over cells at least ce/2 wide.
+  bool fast = true;
+  for (int d = 0; d < 3; ++d) {
+    if (ctx->host_halo)
+      fast = fast && std::floor((b.L[d] + 2.0 * ctx->ghost_cut) / (ce * (1.0 + 1e-12))) >= 3.0;
+    else
+      fast = fast && b.bc[d] == 1 && std::floor(b.L[d] / (ce * (1.0 + 1e-12))) >= 5.0;
+  }
   int64_t ncell = 1;
   double wmax = 0.0;
   for (int d = 0; d < 3; ++d) {
@@ -258,7 +266,9 @@ int setup_box(cv2_ctx *ctx) {
     const bool ok = !fast || coarse_filter_setup(b, ce, wmax);
     if (!ok) return fail(ctx, CV2_ERR_BAD_ARG, "box too large for the coarse list filter");
     b.coarse = fast ? 1 : 0;
-    b.ghost_ok = ctx->ghost_cut >= ce ? 1 : 0;
+    b.open = fast && ctx->host_halo ? 1 : 0;
+    // host-halo ghosts are explicit: no image needs the existence rule
+    b.ghost_ok = ctx->ghost_cut >= ce || ctx->host_halo ? 1 : 0;
   }
   if (ncell > (1 << 26)) return fail(ctx, CV2_ERR_BAD_ARG, "cell grid too large");
   {
@@ -389,7 +399,10 @@ int prepare_list_form(cv2_ctx *ctx) {
   }
   const int rows = s.tg.ny * s.tg.nz;
   if (rows != old.ny * old.nz || !s.rowtasks) HIPCHK(dalloc(&s.rowtasks, rows));
-  HIPCHK(dalloc(&s.ctask, s.ncell));
+  if (s.ncell > s.ctask_cap) {
+    HIPCHK(dalloc(&s.ctask, s.ncell));
+    s.ctask_cap = s.ncell;
+  }
   // until a build has counted the tasks: about one per 200 atoms, plus one
   // per row for the row ends
   if (s.task_grid <= 0 || s.tg.mpr != old.mpr)
@@ -434,7 +447,9 @@ int queue_build(cv2_ctx *ctx, bool force, bool keep_acc, int step = 0) {
 
 int alloc_positions(cv2_ctx *ctx, int64_t ntot) {
   if (ntot <= ctx->pos_cap) return CV2_OK;
-  const int64_t cap = ntot;
+  // room for ghosts: 1/8 more than asked, so that a halo whose size
+  // fluctuates from step to step does not reallocate on every new maximum
+  const int64_t cap = ntot > ctx->n ? ntot + (ntot - ctx->n) / 8 + 4096 : ntot;
   DevState &s = ctx->s;
   // keep the owned part when growing for ghosts
   std::vector<double4> keep;
@@ -449,7 +464,9 @@ int alloc_positions(cv2_ctx *ctx, int64_t ntot) {
   // growing: the previous arrays are released (the owned positions and ids
   // were saved above; the rest is rewritten before use)
   for (void **p : {(void **)&s.pos, (void **)&s.pos_ref, (void **)&s.q, (void **)&s.q_alt,
-                   (void **)&s.id, (void **)&s.cell, (void **)&s.slot, (void **)&s.gcell_items}) {
+                   (void **)&s.id, (void **)&s.cell, (void **)&s.slot, (void **)&s.gcell_items,
+                   (void **)&s.gtag, (void **)&s.gslot, (void **)&s.gtag_alt, (void **)&s.gid_alt,
+                   (void **)&s.gpos_alt}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
@@ -461,6 +478,12 @@ int alloc_positions(cv2_ctx *ctx, int64_t ntot) {
   HIPCHK(dalloc(&s.cell, cap));
   HIPCHK(dalloc(&s.slot, cap));
   HIPCHK(dalloc(&s.gcell_items, cap));
+  // ghost order of open wave builds (ghost counts are below cap)
+  HIPCHK(dalloc(&s.gtag, cap));
+  HIPCHK(dalloc(&s.gslot, cap));
+  HIPCHK(dalloc(&s.gtag_alt, cap));
+  HIPCHK(dalloc(&s.gid_alt, cap));
+  HIPCHK(dalloc(&s.gpos_alt, cap));
   if (!keep.empty()) {
     HIPCHK(hipMemcpyAsync(s.pos, keep.data(), sizeof(double4) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(s.id, keep_id.data(), sizeof(int) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
@@ -540,7 +563,8 @@ bool coarse_filter_setup(BoxDev &b, double ce, double wmax) {
   b.cthr_in = std::nextafter((float)((ce - e) * (ce - e) * (1.0 - 4e-6)), 0.0f);
   b.cthr_out = std::nextafter((float)((ce + e) * (ce + e) * (1.0 + 4e-6)), 1e30f);
   bool ok = ce > 2.0 * e;
-  for (int d = 0; d < 3; ++d) ok = ok && 0.5 * b.L[d] > ce + 2.0 * e;
+  // periodic axes: int32 differences are minimum-image separations
+  for (int d = 0; d < 3; ++d) ok = ok && (!b.bc[d] || 0.5 * b.L[d] > ce + 2.0 * e);
   return ok;
 }
 
@@ -598,6 +622,7 @@ int halo_scratch(cv2_ctx *ctx, int64_t n) {
   HIPCHK(dalloc(&ctx->halo_toff, nt * 27));
   if (!ctx->halo_dn) HIPCHK(dalloc(&ctx->halo_dn, 27));
   if (!ctx->halo_base) HIPCHK(dalloc(&ctx->halo_base, 3 * 64));
+  if (!ctx->halo_res) HIPCHK(dalloc(&ctx->halo_res, 28));
   ctx->halo_n = n;
   return CV2_OK;
 }
@@ -662,7 +687,7 @@ int cv2_destroy(cv2_ctx *ctx) {
   free_state(ctx->s);
   for (void *p : {(void *)ctx->halo_code, (void *)ctx->halo_src, (void *)ctx->halo_tiles,
                   (void *)ctx->halo_toff, (void *)ctx->halo_dn, (void *)ctx->halo_base,
-                  (void *)ctx->halo_buf})
+                  (void *)ctx->halo_buf, (void *)ctx->halo_res})
     if (p) (void)hipFree(p);
   if (ctx->lj_dev) (void)hipFree(ctx->lj_dev);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -854,10 +879,13 @@ int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *t
     idv[g] = id ? (int)id[g] : (int)(ctx->n + g);
   }
   DevState &s = ctx->s;
-  HIPCHK(hipMemcpyAsync(s.pos + ctx->n, p4.data(), sizeof(double4) * ng, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemcpyAsync(s.id + ctx->n, idv.data(), sizeof(int) * ng, hipMemcpyHostToDevice, ctx->stream));
+  // staged, then placed: ghost g at its slot of the last build's cell order
+  // when the count is unchanged (k_ghost_place)
+  HIPCHK(hipMemcpyAsync(s.gpos_alt, p4.data(), sizeof(double4) * ng, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(s.gid_alt, idv.data(), sizeof(int) * ng, hipMemcpyHostToDevice, ctx->stream));
   ctx->ng = ng;
   s.ng = (int)ng;
+  launch_ghost_place(s, s.gpos_alt, s.gid_alt, (int)ng, ctx->b.open && !count_changed, ctx->stream);
   if (count_changed) ctx->need_build = true;  // neighborlist.cpp:118-121
   if (s.write_q) launch_pack_q(s, ctx->b, ctx->stream);
   launch_check_disp(s, ctx->b, ref_rebuilds(ctx), ctx->stream);  // ghosts compared by index, :139-144
@@ -880,38 +908,21 @@ int cv2_halo_pack(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[27], int6
   int rc = halo_scratch(ctx, std::max(n, 1));
   if (rc) return rc;
   const HaloGeomDev h = halo_geom(g);
-  const int nt = halo_tiles(n);
-  std::vector<int> tc((size_t)nt * 27);
+  const int per_atom = g->with_velocity ? 8 : 5;
+  // tile offsets, per-direction totals and packet bases scanned on the
+  // device; the host reads the 28 totals
   launch_halo_classify(ctx->s, h, ctx->halo_code, ctx->halo_src, ctx->halo_tiles, ctx->stream);
-  if (nt > 0)
-    HIPCHK(hipMemcpyAsync(tc.data(), ctx->halo_tiles, sizeof(int) * tc.size(), hipMemcpyDeviceToHost,
-                          ctx->stream));
+  launch_halo_scan(halo_tiles(n), per_atom, ctx->halo_tiles, ctx->halo_toff, ctx->halo_dn,
+                   ctx->halo_base, ctx->halo_res, ctx->stream);
+  HIPCHK(hipMemcpyAsync(ctx->h_halo, ctx->halo_res, sizeof(long long) * 28, hipMemcpyDeviceToHost,
+                        ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  // per-direction totals, tile offsets and packet bases (exclusive scans)
-  std::vector<int> toff(tc.size());
-  int dn[27];
-  long long base[27];
-  const int64_t total = g->with_velocity ? 8 : 5;
-  int64_t at = 0;
-  for (int d = 0; d < 27; ++d) {
-    int64_t c = 0;
-    for (int t = 0; t < nt; ++t) {
-      toff[(size_t)t * 27 + d] = (int)c;
-      c += tc[(size_t)t * 27 + d];
-    }
-    counts[d] = c;
-    dn[d] = (int)c;
-    base[d] = at;
-    at += total * c;
-  }
+  for (int d = 0; d < 27; ++d) counts[d] = n > 0 ? ctx->h_halo[d] : 0;
+  const int64_t at = n > 0 ? ctx->h_halo[27] : 0;
   *need = at;
   if (at == 0 || at > cap) return CV2_OK;
   rc = halo_buffer(ctx, at);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(ctx->halo_toff, toff.data(), sizeof(int) * toff.size(), hipMemcpyHostToDevice,
-                        ctx->stream));
-  HIPCHK(hipMemcpyAsync(ctx->halo_dn, dn, sizeof(dn), hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemcpyAsync(ctx->halo_base, base, sizeof(base), hipMemcpyHostToDevice, ctx->stream));
   launch_halo_write(ctx->s, h, ctx->halo_code, ctx->halo_src, ctx->halo_toff, ctx->halo_base,
                     ctx->halo_dn, ctx->halo_buf, ctx->stream);
   HIPCHK(hipMemcpyAsync(out, ctx->halo_buf, sizeof(double) * at, hipMemcpyDeviceToHost, ctx->stream));
@@ -967,28 +978,33 @@ int cv2_halo_pack_noalloc(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[2
   return CV2_OK;
 }
 
-int cv2_halo_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const double *packets,
-                    int32_t with_velocity) {
+// cv2_halo_unpack / cv2_halo_unpack_at: packet p at packets + offsets[p]
+// (offsets null: the packets back to back).
+static int halo_unpack_impl(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const int64_t *offsets,
+                     const double *packets, int32_t with_velocity) {
   if (!ctx || npackets < 0 || npackets > 64 || (npackets > 0 && !counts))
     return fail(ctx, CV2_ERR_BAD_ARG, "bad halo unpack arguments (at most 64 packets)");
   if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload owned atoms before ghosts");
   (void)hipSetDevice(ctx->device);
   const int64_t total = with_velocity ? 8 : 5;
   long long first[64], pbase[64], pn[64];
-  int64_t ng = 0, at = 0;
+  int64_t ng = 0, at = 0;  // at: doubles of the buffer the packets lie in
   for (int p = 0; p < npackets; ++p) {
     if (counts[p] < 0) return fail(ctx, CV2_ERR_BAD_ARG, "negative packet count");
+    const int64_t o = offsets ? offsets[p] : at;
+    if (o < 0) return fail(ctx, CV2_ERR_BAD_ARG, "negative packet offset");
+    if (counts[p] > 0 && !packets) return fail(ctx, CV2_ERR_BAD_ARG, "missing packets");
     first[p] = ng;
-    pbase[p] = at;
+    pbase[p] = o;
     pn[p] = counts[p];
     // types are validated here, as cv2_set_ghosts does
     for (int64_t c = 0; c < counts[p]; ++c) {
-      const double t = packets[at + counts[p] + c];
+      const double t = packets[o + counts[p] + c];
       if (!(t >= 0.0 && t < ctx->lj.nt && t == (double)(int)t))
         return fail(ctx, CV2_ERR_BAD_ARG, "ghost type out of range");
     }
     ng += counts[p];
-    at += total * counts[p];
+    at = offsets ? std::max(at, o + total * counts[p]) : at + total * counts[p];
   }
   if (ng > 0 && !packets) return fail(ctx, CV2_ERR_BAD_ARG, "missing packets");
   const bool count_changed = ng != ctx->ng || !ctx->host_halo;
@@ -1004,7 +1020,7 @@ int cv2_halo_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const
   rc = halo_buffer(ctx, std::max<int64_t>(at, 1));
   if (rc) return rc;
   DevState &s = ctx->s;
-  if (at > 0)
+  if (at > 0 && ng > 0)
     HIPCHK(hipMemcpyAsync(ctx->halo_buf, packets, sizeof(double) * at, hipMemcpyHostToDevice,
                           ctx->stream));
   if (npackets > 0) {
@@ -1019,12 +1035,35 @@ int cv2_halo_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const
   ctx->ng = ng;
   s.ng = (int)ng;
   launch_halo_unpack(s, ctx->halo_buf, npackets, ctx->halo_base, ctx->halo_base + 64,
-                     ctx->halo_base + 128, (int)ng, ctx->stream);
+                     ctx->halo_base + 128, (int)ng, ctx->b.open && !count_changed, ctx->stream);
   if (count_changed) ctx->need_build = true;  // neighborlist.cpp:118-121
   if (s.write_q) launch_pack_q(s, ctx->b, ctx->stream);
   launch_check_disp(s, ctx->b, ref_rebuilds(ctx), ctx->stream);  // ghosts compared by index, :139-144
   HIPCHK(hipStreamSynchronize(ctx->stream));  // tab is a stack buffer
   return CV2_OK;
+}
+
+int cv2_halo_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const double *packets,
+                    int32_t with_velocity) {
+  return halo_unpack_impl(ctx, npackets, counts, nullptr, packets, with_velocity);
+}
+
+int cv2_halo_unpack_at(cv2_ctx *ctx, int32_t npackets, const int64_t *counts,
+                       const int64_t *offsets, const double *packets, int32_t with_velocity) {
+  if (npackets > 0 && !offsets) return fail(ctx, CV2_ERR_BAD_ARG, "missing packet offsets");
+  return halo_unpack_impl(ctx, npackets, counts, offsets, packets, with_velocity);
+}
+
+int cv2_host_alloc(int64_t bytes, void **ptr) {
+  if (!ptr || bytes < 0) return CV2_ERR_BAD_ARG;
+  *ptr = nullptr;
+  if (bytes == 0) return CV2_OK;
+  return hipHostMalloc(ptr, (size_t)bytes, hipHostMallocDefault) == hipSuccess ? CV2_OK : CV2_ERR_HIP;
+}
+
+int cv2_host_free(void *ptr) {
+  if (!ptr) return CV2_OK;
+  return hipHostFree(ptr) == hipSuccess ? CV2_OK : CV2_ERR_HIP;
 }
 
 int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt) {
@@ -1251,28 +1290,43 @@ int cv2_step_begin(cv2_ctx *ctx, double dt, int langevin, double gamma, double k
   ctx->open_args = u;
   ctx->step_open = true;
   ctx->forces_valid = false;
-  return check_flags(ctx);
+  // no host round trip: a NaN raised here halts the device and is reported
+  // by cv2_step_end
+  return CV2_OK;
 }
 
-int cv2_step_end(cv2_ctx *ctx) {
+int cv2_step_end(cv2_ctx *ctx, int energy) {
   if (!ctx) return CV2_ERR_BAD_ARG;
   if (!ctx->step_open) return fail(ctx, CV2_ERR_STATE, "no open step: cv2_step_begin first");
   (void)hipSetDevice(ctx->device);
   int rc = queue_build(ctx, false, false);
   if (rc) return rc;
-  rc = forces_pass(ctx, true);
-  if (rc) return rc;
   UpdateArgs u = ctx->open_args;
   u.finish = true;
   u.start = false;
   u.step = (uint64_t)ctx->steps;
-  launch_update(ctx->s, ctx->b, u, ctx->stream);
+  // forces and the finishing half with one host round trip: a build cut at
+  // the list capacity halts both (F_HALT); the lists then grow and the
+  // build, the forces and the half run again
+  for (int tries = 0;; ++tries) {
+    if (ctx->precision != CV2_PREC_FP64) launch_pack_q(ctx->s, ctx->b, ctx->stream);
+    const int t = span_begin(ctx, 0);
+    launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, energy != 0, ctx->stream);
+    span_end(ctx, t, 0);
+    launch_update(ctx->s, ctx->b, u, ctx->stream);
+    rc = check_flags(ctx);
+    if (rc != CV2_ERR_CAPACITY || !ctx->h_flags[F_OVERFLOW] || tries == 8) break;
+    rc = grow_lists(ctx);
+    if (rc) return rc;
+    launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, false);
+  }
+  if (rc) return rc;
   ctx->step_open = false;
   ctx->steps += 1;
-  rc = check_flags(ctx);
-  if (rc) return rc;
-  rc = fetch_energy(ctx);
-  if (rc) return rc;
+  if (energy) {
+    rc = fetch_energy(ctx);
+    if (rc) return rc;
+  }
   ctx->forces_valid = true;
   return CV2_OK;
 }
@@ -1364,8 +1418,13 @@ int cv2_get_pairs(cv2_ctx *ctx, int64_t cap, int64_t *npairs, int64_t *pairs) {
   // task-slot form: rebuild the 32-bit entries (atom | image) from the task
   // piece tables; the image of j follows from its cell relative to i's cell
   // (periodic grid, at least 5 cells per axis)
-  std::vector<int> cst, ctk;
+  std::vector<int> cst, ctk, gtg;
   std::vector<int4> tpc;
+  if (ctx->b.open && s.ng > 0) {  // ghosts in cell order: their host indices
+    gtg.resize(s.ng);
+    HIPCHK(hipMemcpyAsync(gtg.data(), s.gtag, sizeof(int) * s.ng, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
   if (s.slots) {
     cst.resize(s.ncell + 1);
     ctk.resize(s.ncell);
@@ -1389,6 +1448,7 @@ int cv2_get_pairs(cv2_ctx *ctx, int64_t cap, int64_t *npairs, int64_t *pairs) {
     int p = 0;
     for (int k2 = 1; k2 < kTaskPieces; ++k2) p = tp[k2].y <= sl ? k2 : p;
     const int j = tp[p].x + (sl - tp[p].y);
+    if (g.open) return (uint32_t)j | (kNoImage << 26);  // no images on open grids
     const int cj = cell_of(j);
     const int d[3] = {cj % g.nx - cx, (cj / g.nx) % g.ny - cy, cj / (g.nx * g.ny) - cz};
     uint32_t img = 0;
@@ -1404,7 +1464,7 @@ int cv2_get_pairs(cv2_ctx *ctx, int64_t cap, int64_t *npairs, int64_t *pairs) {
       const int code = (int)(img & 3u) + 3 * (int)((img >> 2) & 3u) + 9 * (int)(img >> 4);
       int64_t tj;
       if (j >= n) {
-        tj = j;  // host-halo ghost: n + host index
+        tj = gtg.empty() ? j : n + gtg[j - n];  // host-halo ghost: n + host index
       } else {
         tj = tg[j];
         if (img == kNoImage && tj < tg[i]) continue;  // half list: owned j > i

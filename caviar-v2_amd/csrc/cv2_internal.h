@@ -86,6 +86,11 @@ struct BoxDev {
   int coarse;
   int ghost_ok;  // ghost cutoff >= list cutoff: every image within it exists
   int qiso;      // one fixed-point unit on all three axes
+  // open = 1: the wave build over a host-halo box (coarse = 1, no periodic
+  // axis): the cell grid spans the box and its ghost margin, the ghosts are
+  // sorted into it at each build (ghost section of the arrays in cell order)
+  // and candidates are the owned and the ghost atoms of adjacent cells
+  int open;
 };
 
 struct LJDev {
@@ -335,6 +340,7 @@ struct TaskGeom {
   int nx, ny, nz;  // cell grid
   int mpr;         // task slots per x-row
   int nslot;       // ny * nz * mpr
+  int open;        // BoxDev::open: rows do not wrap; piece 2r + 1 holds ghosts
 };
 
 // x-cells [a0, a1) of piece p of a task over [x0, x1): the whole row once
@@ -356,12 +362,23 @@ __host__ __device__ inline void task_piece_cells(int nx, int x0, int x1, int p, 
   }
 }
 
+// Open grids: x-cells [a0, a1) of both pieces (owned, ghost) of a task
+// over [x0, x1): x0 - 1 .. x1, clipped to the grid.
+__host__ __device__ inline void task_piece_cells_open(int nx, int x0, int x1, int &a0, int &a1) {
+  a0 = x0 > 0 ? x0 - 1 : 0;
+  a1 = x1 < nx ? x1 + 1 : nx;
+}
+
 // First cell of row r (dy = r % 3 - 1, dz = r / 3 - 1, periodic) around
-// row (cy, cz).
+// row (cy, cz); on open grids -1 for a row outside the grid.
 __host__ __device__ inline int task_row_base(const TaskGeom &g, int cy, int cz, int r) {
   int y = cy + r % 3 - 1, z = cz + r / 3 - 1;
-  y += y < 0 ? g.ny : (y >= g.ny ? -g.ny : 0);
-  z += z < 0 ? g.nz : (z >= g.nz ? -g.nz : 0);
+  if (g.open) {
+    if (y < 0 || y >= g.ny || z < 0 || z >= g.nz) return -1;
+  } else {
+    y += y < 0 ? g.ny : (y >= g.ny ? -g.ny : 0);
+    z += z < 0 ? g.nz : (z >= g.nz ? -g.nz : 0);
+  }
   return g.nx * (y + g.ny * z);
 }
 
@@ -384,6 +401,11 @@ struct DevState {
   int *cell_local;               // tile-local cell starts during a build (ncell)
   int *tile_sum;                // per scan tile (ncell / 1024 + 1)
   int *gcell_count, *gcell_start, *gcell_items; // host-halo ghost cell list
+  // open wave builds sort the ghosts: gtag[r] = host index of device ghost
+  // r, gslot its inverse (where an upload of an unchanged ghost count
+  // writes host ghost g); gpos_alt / gid_alt / gtag_alt: the new order
+  int *gtag, *gslot, *gtag_alt, *gid_alt;
+  double4 *gpos_alt;
   uint32_t *nbr;                // u32 entries, or u16 slots when `slots`
   int *nnbr;
   bool slots;                   // FP32 path on a wave-build box: task-slot list
@@ -391,6 +413,7 @@ struct DevState {
   int4 *tpiece;                 // per task slot: kTaskPieces x (first atom, slot base, a0, a1)
   int4 *tinfo;                  // per task slot: first home atom, end, staged atoms S, x0 | x1 << 16
   int *ctask;                   // per cell: its task slot
+  int ctask_cap;
   int *rowtasks;                // per x-row: tasks used
   int4 *ctinfo, *ctpiece;       // the used slots' tinfo / tpiece in row order (count: flags[F_NTASK])
   int64_t task_cap;             // task slots allocated
@@ -538,6 +561,10 @@ void launch_halo_write(const DevState &s, const HaloGeomDev &g, const int *code_
                        const int *src_by_tag, const int *tile_off, const long long *dir_base,
                        const int *dir_n, double *out, hipStream_t st);
 void launch_halo_unpack(DevState &s, const double *in, int npk, const long long *pk_first,
-                        const long long *pk_base, const long long *pk_n, int ng, hipStream_t st);
+                        const long long *pk_base, const long long *pk_n, int ng, bool remap,
+                        hipStream_t st);
+// cv2_set_ghosts: ghosts staged on the device, placed as k_halo_unpack does
+void launch_ghost_place(DevState &s, const double4 *gpos_in, const int *gid_in, int ng, bool remap,
+                        hipStream_t st);
 
 }  // namespace cv2

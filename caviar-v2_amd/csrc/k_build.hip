@@ -199,7 +199,8 @@ __device__ void task_row_partition(int row, int lane, int *P, int *H,
                                    const int *__restrict__ count, const CellStarts &cs,
                                    const TaskGeom &g, int4 *__restrict__ tpiece,
                                    int4 *__restrict__ tinfo, int *__restrict__ ctask,
-                                   int *__restrict__ rowtasks, int *__restrict__ flags);
+                                   int *__restrict__ rowtasks, int *__restrict__ flags,
+                                   const int *__restrict__ gstart, int n);
 
 // Scatter atom indices to their cell slots (work-groups [0, nbs)); the
 // next nfix work-groups store the cell starts, and the rest (ntb) cut the
@@ -209,7 +210,7 @@ __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ 
                           int *__restrict__ flags, int nbs, int nfix, TaskGeom g,
                           const int *__restrict__ count, int4 *__restrict__ tpiece,
                           int4 *__restrict__ tinfo, int *__restrict__ ctask,
-                          int *__restrict__ rowtasks) {
+                          int *__restrict__ rowtasks, const int *__restrict__ gstart) {
   if (!build_pending(flags)) return;
   const int b = blockIdx.x;
   if (b >= nbs + nfix) {
@@ -218,7 +219,7 @@ __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ 
     __shared__ int P[kRowWaves][kMaxRowCells + 1], H[kRowWaves][kMaxRowCells + 1];
     const int wv = (int)(threadIdx.x >> 6);
     task_row_partition((b - nbs - nfix) * kRowWaves + wv, (int)(threadIdx.x & 63), P[wv], H[wv],
-                       count, cs, g, tpiece, tinfo, ctask, rowtasks, flags);
+                       count, cs, g, tpiece, tinfo, ctask, rowtasks, flags, gstart, n);
     return;
   }
   if (b >= nbs) {
@@ -372,7 +373,20 @@ struct CopyBack {
   const int *id_alt;
   int *id;
   int n;
+  // open wave builds: the sorted ghosts (device ghost r = atom n + r)
+  const double4 *gpos_alt;
+  const int *gid_alt, *gtag_alt;
+  int *gtag, *gslot;
+  int ng;
 };
+
+__device__ inline void copy_back_ghost(int r, const CopyBack &c) {
+  c.pos[c.n + r] = c.gpos_alt[r];
+  c.id[c.n + r] = c.gid_alt[r];
+  const int h = c.gtag_alt[r];
+  c.gtag[r] = h;
+  c.gslot[h] = r;
+}
 
 __device__ inline void copy_back_atom(int k, const CopyBack &c) {
   const int n = c.n;
@@ -389,7 +403,10 @@ __device__ inline void copy_back_atom(int k, const CopyBack &c) {
 __global__ void k_copy_back(CopyBack c, const int *__restrict__ flags) {
   if (!build_pending(flags)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < c.n) copy_back_atom(k, c);
+  if (k < c.n)
+    copy_back_atom(k, c);
+  else if (k - c.n < c.ng)
+    copy_back_ghost(k - c.n, c);
 }
 
 // Host-halo ghosts keep host order; they get their own CSR cell list and
@@ -415,6 +432,34 @@ __global__ void k_ghost_scatter(const int *__restrict__ cell, const int *__restr
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ng) return;
   items[start[cell[g]] + slot[g]] = g;
+}
+
+// Open wave builds: device ghost g goes to place r of the cell order, its
+// cell's start plus the number of the cell's ghosts with a smaller device
+// index (items lists them in arrival order), so the order does not depend
+// on the atomics of the binning; its build position and fixed-point
+// coordinates are written in the new order, the rest goes to the *_alt
+// arrays (k_copy_back).
+__global__ void k_ghost_move(const int *__restrict__ items, const int *__restrict__ gcell,
+                             const int *__restrict__ gstart, int n, int ng,
+                             const double4 *__restrict__ pos, const int *__restrict__ id,
+                             const int *__restrict__ gtag, double4 *__restrict__ gpos_alt,
+                             int *__restrict__ gid_alt, int *__restrict__ gtag_alt,
+                             double4 *__restrict__ pos_ref, int4 *__restrict__ q, BoxDev b,
+                             const int *__restrict__ flags) {
+  if (!build_pending(flags)) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int c = gcell[g], a = gstart[c], e = gstart[c + 1];
+  int rank = 0;
+  for (int m = a; m < e; ++m) rank += items[m] < g;
+  const int r = a + rank;
+  const double4 p = pos[n + g];
+  gpos_alt[r] = p;
+  pos_ref[n + r] = p;
+  q[n + r] = pack_q(p.x, p.y, p.z, (int)p.w, b.lo, b.qscale);
+  gid_alt[r] = id[n + g];
+  gtag_alt[r] = gtag[g];
 }
 
 __global__ void k_ghost_sort_cells(int *__restrict__ items, const int *__restrict__ start,
@@ -737,17 +782,37 @@ struct CellWaveBox {
   float cunit[3];
   float cthr_in, cthr_out;
   int wide;  // some axis has 3 or 4 cells: relative coordinates from the FP64 positions
+  int open;  // BoxDev::open: rows clipped, part 1 = the ghosts of the part-0 cells
+  int n;     // owned atoms (first ghost index)
 };
 
 // Range of sorted atoms of stencil row r (dy = r % 3 - 1, dz = r / 3 - 1) of
 // home cell (cx, cy, cz).  part 0: the x-cells that do not wrap; part 1: the
 // wrapped one (cx == 0: cell nx-1 seen with image +1; cx == nx-1: cell 0
 // with image -1), empty otherwise.
-__device__ inline void row_range(const int *__restrict__ cstart, const CellWaveBox &b, int cx,
-                                 int cy, int cz, int r, int part, int &j0, int &len,
-                                 uint32_t &img) {
+// Open grids: part 0 is the owned atoms of x-cells cx-1 .. cx+1 (clipped)
+// of the row, part 1 the ghosts of the same cells.
+__device__ inline void row_range(const int *__restrict__ cstart, const int *__restrict__ gstart,
+                                 const CellWaveBox &b, int cx, int cy, int cz, int r, int part,
+                                 int &j0, int &len, uint32_t &img) {
   const int nx = b.nc[0], ny = b.nc[1], nz = b.nc[2];
   int y = cy + r % 3 - 1, z = cz + r / 3 - 1, vy = 0, vz = 0;
+  if (b.open) {
+    j0 = 0;
+    len = 0;
+    img = img_bits(0, 0, 0);
+    if (y < 0 || y >= ny || z < 0 || z >= nz) return;
+    const int row = nx * (y + ny * z);
+    const int a0 = row + (cx > 0 ? cx - 1 : 0), a1 = row + (cx + 2 < nx ? cx + 2 : nx);
+    if (part == 0) {
+      j0 = cstart[a0];
+      len = cstart[a1] - j0;
+    } else if (gstart) {
+      j0 = b.n + gstart[a0];
+      len = gstart[a1] - gstart[a0];
+    }
+    return;
+  }
   if (y < 0) { y += ny; vy = 1; } else if (y >= ny) { y -= ny; vy = -1; }
   if (z < 0) { z += nz; vz = 1; } else if (z >= nz) { z -= nz; vz = -1; }
   const int row = nx * (y + ny * z);
@@ -959,7 +1024,12 @@ __device__ void cell_wave_pass(const double4 *__restrict__ pos, const int4 *__re
 // Slot offset (slot - atom index) of the atoms of segment (r, part) of home
 // cell cx in its task's staging order: the piece of row r that holds the
 // segment's first x-cell.
-__device__ inline int seg_slot_delta(const int4 *__restrict__ tp, int nx, int cx, int r, int part) {
+__device__ inline int seg_slot_delta(const int4 *__restrict__ tp, int nx, int cx, int r, int part,
+                                     int open) {
+  if (open) {  // one piece per part, holding the segment's cells
+    const int4 a = tp[2 * r + part];
+    return a.y - a.x;
+  }
   const int xs = part == 0 ? (cx == 0 ? 0 : cx - 1) : (cx == 0 ? nx - 1 : 0);
   const int4 a = tp[2 * r];
   const int4 pc = (xs >= a.z && xs < a.w) ? a : tp[2 * r + 1];
@@ -969,7 +1039,8 @@ __device__ inline int seg_slot_delta(const int4 *__restrict__ tp, int nx, int cx
 template <bool GHOST_OK, bool SLOTS>
 __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restrict__ q,
                           const CellWaveBox &b, const BoxDev *__restrict__ bdev,
-                          const int *__restrict__ cstart, uint32_t *__restrict__ nbr,
+                          const int *__restrict__ cstart, const int *__restrict__ gstart,
+                          uint32_t *__restrict__ nbr,
                           int *__restrict__ nnbr, int stride, int cap, int c, int wv, int lane,
                           uint32_t (*lds)[kPassCand + 4 + 64], uint16_t *cslot,
                           int (*cnt_sh)[kMaxHomeCell], const int *__restrict__ ctask,
@@ -991,8 +1062,8 @@ __device__ int2 cell_wave(const double4 *__restrict__ pos, const int4 *__restric
   uint32_t simg = 0u;
   if (lane < kSegs) {
     const int r = lane < kWaveRows ? lane : lane - kWaveRows, part = lane < kWaveRows ? 0 : 1;
-    row_range(cstart, b, cx, cy, cz, r, part, sj0, slen, simg);
-    if (SLOTS) sdelta = seg_slot_delta(tp, nx, cx, r, part);
+    row_range(cstart, gstart, b, cx, cy, cz, r, part, sj0, slen, simg);
+    if (SLOTS && slen > 0) sdelta = seg_slot_delta(tp, nx, cx, r, part, b.open);
   }
   int sat = slen;  // inclusive prefix sum over lanes (kSegs <= 32)
 #pragma unroll
@@ -1086,7 +1157,7 @@ template <bool GHOST_OK, bool SLOTS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
     k_nlist_cells(const double4 *__restrict__ pos, const int4 *__restrict__ q, CellWaveBox b,
                   const BoxDev *__restrict__ bdev, const int *__restrict__ cstart,
-                  uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
+                  const int *__restrict__ gstart, uint32_t *__restrict__ nbr, int *__restrict__ nnbr, int stride, int cap,
                   const int *__restrict__ flags, int nblk_real, int2 *__restrict__ bstat,
                   const int *__restrict__ ctask, const int4 *__restrict__ tpiece,
                   const int4 *__restrict__ tinfo, int ncopy, CopyBack cpb) {
@@ -1118,8 +1189,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
   const int c = __builtin_amdgcn_readfirstlane(blk * 4 + wv);
   int2 st = make_int2(0, 0);
   if (c < b.nc[0] * b.nc[1] * b.nc[2])
-    st = cell_wave<GHOST_OK, SLOTS>(pos, q, b, bdev, cstart, nbr, nnbr, stride, cap, c, wv, lane,
-                                    lds, cslot_sh[wv], cnt_sh, ctask, tpiece, tinfo);
+    st = cell_wave<GHOST_OK, SLOTS>(pos, q, b, bdev, cstart, gstart, nbr, nnbr, stride, cap, c, wv,
+                                    lane, lds, cslot_sh[wv], cnt_sh, ctask, tpiece, tinfo);
   if (lane == 0) st_sh[wv] = st;
   __syncthreads();
   if (threadIdx.x == 0) {  // per-block statistics, reduced by k_build_stats
@@ -1193,7 +1264,8 @@ __device__ void task_row_partition(int row, int lane, int *P, int *H,
                                    const int *__restrict__ count, const CellStarts &cs,
                                    const TaskGeom &g, int4 *__restrict__ tpiece,
                                    int4 *__restrict__ tinfo, int *__restrict__ ctask,
-                                   int *__restrict__ rowtasks, int *__restrict__ flags) {
+                                   int *__restrict__ rowtasks, int *__restrict__ flags,
+                                   const int *__restrict__ gstart, int n) {
   if (row >= g.ny * g.nz) return;  // whole wavefronts
   const int nx = g.nx, cy = row % g.ny, cz = row / g.ny;
   const int home = task_row_base(g, cy, cz, 4);  // dy = dz = 0
@@ -1203,7 +1275,12 @@ __device__ void task_row_partition(int row, int lane, int *P, int *H,
     int col = 0, hm = 0;
     if (x < nx) {
 #pragma unroll
-      for (int r = 0; r < 9; ++r) col += count[task_row_base(g, cy, cz, r) + x];
+      for (int r = 0; r < 9; ++r) {
+        const int rb = task_row_base(g, cy, cz, r);
+        if (rb < 0) continue;  // open grids: outside the grid
+        col += count[rb + x];
+        if (g.open && gstart) col += gstart[rb + x + 1] - gstart[rb + x];
+      }
       hm = count[home + x];
     }
     int ic = col, ih = hm;  // inclusive scans over the wavefront
@@ -1232,6 +1309,7 @@ __device__ void task_row_partition(int row, int lane, int *P, int *H,
   // staged atoms of [x0, x1): x-cells x0-1 .. x1 (periodic) of the 9 rows,
   // the whole row once when that covers it (task_piece_cells)
   auto staged = [&](int x0, int x1) {
+    if (g.open) return P[x1 < nx ? x1 + 1 : nx] - P[x0 > 0 ? x0 - 1 : 0];
     if (x1 - x0 + 2 >= nx) return P[nx];
     if (x0 == 0) return P[x1 + 1] + (P[nx] - P[nx - 1]);
     if (x1 == nx) return P[nx] - P[x0 - 1] + P[1];
@@ -1239,6 +1317,8 @@ __device__ void task_row_partition(int row, int lane, int *P, int *H,
   };
   int x0 = 0, t = 0;
   bool ovf = false;
+  // open grids: a row without owned atoms (the ghost margin) has no task
+  if (g.open && H[nx] == 0) x0 = nx;
   while (x0 < nx) {  // uniform over the wavefront
     int x1 = x0 + 1;
     if (t + 1 < g.mpr) {
@@ -1259,9 +1339,22 @@ __device__ void task_row_partition(int row, int lane, int *P, int *H,
     int j0 = 0, len = 0, a0 = 0, a1 = 0;
     if (lane < kTaskPieces) {
       const int rb = task_row_base(g, cy, cz, lane >> 1);
-      task_piece_cells(nx, x0, x1, lane & 1, a0, a1);
-      j0 = cs(rb + a0);
-      len = a1 > a0 ? cs(rb + a1) - j0 : 0;
+      if (!g.open) {
+        task_piece_cells(nx, x0, x1, lane & 1, a0, a1);
+        j0 = cs(rb + a0);
+        len = a1 > a0 ? cs(rb + a1) - j0 : 0;
+      } else if (rb >= 0) {
+        // piece 2r: owned atoms, 2r + 1: ghosts (sorted section after the
+        // owned atoms), of the same x-cells
+        task_piece_cells_open(nx, x0, x1, a0, a1);
+        if ((lane & 1) == 0) {
+          j0 = cs(rb + a0);
+          len = cs(rb + a1) - j0;
+        } else if (gstart) {
+          j0 = n + gstart[rb + a0];
+          len = gstart[rb + a1] - gstart[rb + a0];
+        }
+      }
     }
     int base = len;
 #pragma unroll
@@ -1609,8 +1702,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
       cb.cthr_in = bd->cthr_in;
       cb.cthr_out = bd->cthr_out;
       const int off = e.cell_off[sy];
-      st = cell_wave<true, false>(pos, q, cb, bd, cstart + off, nbr, nnbr, stride, cap, c - off, wv,
-                                  lane, lds, nullptr, cnt_sh, nullptr, nullptr, nullptr);
+      st = cell_wave<true, false>(pos, q, cb, bd, cstart + off, nullptr, nbr, nnbr, stride, cap,
+                                  c - off, wv, lane, lds, nullptr, cnt_sh, nullptr, nullptr, nullptr);
     }
   }
   if (lane == 0) st_sh[wv] = st;
@@ -1719,8 +1812,25 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
     k_build_begin<<<nblk(nc, 256), 256, 0, st>>>(s.flags, s.cell_count, nc, s.gcell_count);
   const bool coarse = n > 0 && b.coarse;
   const bool copy_in_list = coarse && !ghosts;
+  // open wave builds sort the ghosts into the cell grid first: the task
+  // partition counts them, and the list kernel reads them in cell order
+  const bool gsort = ghosts && coarse && b.open;
   const CopyBack cpb{s.pos_alt,  s.pos,     s.vel_alt, s.vel, keep_acc ? s.acc_alt : nullptr,
-                     keep_acc ? s.acc : nullptr, s.tag_alt, s.tag, s.id_alt,  s.id,  n};
+                     keep_acc ? s.acc : nullptr, s.tag_alt, s.tag, s.id_alt,  s.id,  n,
+                     s.gpos_alt, s.gid_alt, s.gtag_alt, s.gtag, s.gslot, gsort ? s.ng : 0};
+  auto ghost_cells = [&]() {
+    k_ghost_bin<<<nblk(s.ng, 256), 256, 0, st>>>(s.pos, s.pos_ref, n, s.ng, b, s.cell + n,
+                                                  s.slot + n, s.gcell_count, s.flags);
+    launch_scan(s.gcell_count, s.gcell_start, s.tile_sum, nc, s.flags, st);
+    k_ghost_scatter<<<nblk(s.ng, 256), 256, 0, st>>>(s.cell + n, s.slot + n, s.gcell_start,
+                                                      s.gcell_items, s.ng, s.flags);
+  };
+  if (gsort) {
+    ghost_cells();
+    k_ghost_move<<<nblk(s.ng, 256), 256, 0, st>>>(s.gcell_items, s.cell + n, s.gcell_start, n, s.ng,
+                                                   s.pos, s.id, s.gtag, s.gpos_alt, s.gid_alt,
+                                                   s.gtag_alt, s.pos_ref, s.q, b, s.flags);
+  }
   if (n > 0) {
     k_wrap_bin<<<nblk(n, 256), 256, 0, st>>>(s.pos, n, b, s.cell, s.slot, s.cell_count, s.flags);
     k_scan_local<<<nblk(nc, kScanTile), kScanTile, 0, st>>>(s.cell_count, s.cell_local, s.tile_sum,
@@ -1731,7 +1841,7 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
     const int rows = tasks ? s.tg.ny * s.tg.nz : 0;
     k_scatter<<<nbs + nfix + nblk(rows, kRowWaves), 256, 0, st>>>(
         s.cell, s.slot, cs, s.cell_start, s.perm, n, s.flags, nbs, nfix, s.tg, s.cell_count,
-        s.tpiece, s.tinfo, s.ctask, s.rowtasks);
+        s.tpiece, s.tinfo, s.ctask, s.rowtasks, gsort ? s.gcell_start : nullptr);
     // accelerations are carried along only when no force pass follows; the
     // extra last work-groups (tasks) copy the task tables to compact order
     const TaskMapArgs tm{s.rowtasks, s.tinfo, s.tpiece, s.ctinfo, s.ctpiece, rows, s.tg.mpr,
@@ -1742,14 +1852,10 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
         (s.write_q || b.coarse) ? s.q : nullptr, b, s.flags, tm);
     // The host cannot know whether the device ran the build, so the new
     // order is copied back instead of swapping buffer pointers.
-    if (!copy_in_list) k_copy_back<<<nblk(n, 256), 256, 0, st>>>(cpb, s.flags);
+    if (!copy_in_list) k_copy_back<<<nblk(n + cpb.ng, 256), 256, 0, st>>>(cpb, s.flags);
   }
-  if (ghosts) {
-    k_ghost_bin<<<nblk(s.ng, 256), 256, 0, st>>>(s.pos, s.pos_ref, n, s.ng, b, s.cell + n,
-                                                  s.slot + n, s.gcell_count, s.flags);
-    launch_scan(s.gcell_count, s.gcell_start, s.tile_sum, nc, s.flags, st);
-    k_ghost_scatter<<<nblk(s.ng, 256), 256, 0, st>>>(s.cell + n, s.slot + n, s.gcell_start,
-                                                      s.gcell_items, s.ng, s.flags);
+  if (ghosts && !gsort) {  // per-atom build: the ghost lists of the cells, by device index
+    ghost_cells();
     k_ghost_sort_cells<<<nblk(nc, 256), 256, 0, st>>>(s.gcell_items, s.gcell_start, nc, s.flags);
   }
   int nw = 0;  // per-work-group statistics written by the list kernel
@@ -1761,6 +1867,8 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
     }
     cb.cthr_in = b.cthr_in;
     cb.cthr_out = b.cthr_out;
+    cb.open = b.open;
+    cb.n = n;
     const int real = nblk(nc, 4), grid = (real + 7) / 8 * 8;
     const int ncopy = copy_in_list ? (nblk(n, 256) + 7) / 8 * 8 : 0;
     // the list kernel reads the new-order positions where they were moved
@@ -1768,7 +1876,9 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
 #define CV2_NLC(G, S)                                                                            \
   if (b.ghost_ok == G && s.slots == S)                                                           \
     k_nlist_cells<G, S><<<ncopy + grid, 256, 0, st>>>(pos_new, s.q, cb, s.box_dev, s.cell_start, \
-                                                      s.nbr, s.nnbr, s.stride, s.cap, s.flags,   \
+                                                      gsort ? s.gcell_start : nullptr, s.nbr,    \
+                                                      s.nnbr, s.stride,                          \
+                                                      s.cap, s.flags,                            \
                                                       real, s.wstat, s.ctask, s.tpiece, s.tinfo, \
                                                       ncopy, cpb);
     CV2_NLC(true, true)
@@ -1799,6 +1909,7 @@ TaskGeom task_geom(int n, const BoxDev &b, int min_mpr) {
   const int want = (int)std::ceil(1.5 * row / 230.0) + 2;
   g.mpr = std::max(1, std::min(g.nx, std::max(min_mpr, want)));
   g.nslot = g.ny * g.nz * g.mpr;
+  g.open = b.open;
   return g;
 }
 

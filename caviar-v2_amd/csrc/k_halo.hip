@@ -134,65 +134,96 @@ __global__ void __launch_bounds__(kHaloTile)
   }
 }
 
-// Tile offsets, per-direction totals and packet bases on the device (the
-// host does this scan in cv2_halo_pack): one work-group; direction d's
-// exclusive scan over the tiles' counts gives tile_off, its total N_d goes
-// to dir_n and res[d], and the packets follow each other in direction order
-// (dir_base, in doubles, res[27] = their total).
+// Tile offsets, per-direction totals and packet bases on the device: one
+// work-group per direction d.  It scans direction d's counts over the tiles
+// into tile_off, stores its total N_d (dir_n, res[d]), and sums the counts
+// of the directions before it for its packet base (the packets follow each
+// other in direction order; dir_base in doubles, res[27] = their total).
 __global__ void __launch_bounds__(1024)
     k_halo_scan(const int *__restrict__ tile_counts, int ntiles, int per_atom,
                 int *__restrict__ tile_off, int *__restrict__ dir_n, long long *__restrict__ dir_base,
                 long long *__restrict__ res) {
   __shared__ int sh[1024];
-  __shared__ long long tot[27];
-  const int t = threadIdx.x;
-  for (int d = 0; d < 27; ++d) {
-    int carry = 0;
-    for (int t0 = 0; t0 < ntiles; t0 += 1024) {
-      const int v = t0 + t < ntiles ? tile_counts[(t0 + t) * 27 + d] : 0;
-      sh[t] = v;
+  __shared__ long long red[16];
+  const int t = threadIdx.x, d = blockIdx.x;
+  int carry = 0;
+  for (int t0 = 0; t0 < ntiles; t0 += 1024) {
+    const int v = t0 + t < ntiles ? tile_counts[(t0 + t) * 27 + d] : 0;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int a = t >= off ? sh[t - off] : 0;
       __syncthreads();
-      for (int off = 1; off < 1024; off <<= 1) {
-        const int a = t >= off ? sh[t - off] : 0;
-        __syncthreads();
-        sh[t] += a;
-        __syncthreads();
-      }
-      if (t0 + t < ntiles) tile_off[(t0 + t) * 27 + d] = carry + sh[t] - v;
-      carry += sh[1023];
+      sh[t] += a;
       __syncthreads();
     }
-    if (t == 0) tot[d] = carry;
+    if (t0 + t < ntiles) tile_off[(t0 + t) * 27 + d] = carry + sh[t] - v;
+    carry += sh[1023];
+    __syncthreads();
   }
+  // atoms of the directions before d
+  long long before = 0;
+  for (int k = t; k < ntiles; k += 1024)
+    for (int e = 0; e < d; ++e) before += tile_counts[k * 27 + e];
+  for (int o = 32; o > 0; o >>= 1) before += __shfl_xor(before, o, 64);
+  if ((t & 63) == 0) red[t >> 6] = before;
   __syncthreads();
   if (t == 0) {
-    long long at = 0;
-    for (int d = 0; d < 27; ++d) {
-      dir_n[d] = (int)tot[d];
-      dir_base[d] = at;
-      res[d] = tot[d];
-      at += (long long)per_atom * tot[d];
-    }
-    res[27] = at;
+    long long b = 0;
+    for (int w = 0; w < 16; ++w) b += red[w];
+    dir_n[d] = carry;
+    dir_base[d] = (long long)per_atom * b;
+    res[d] = carry;
+    if (d == 26) res[27] = (long long)per_atom * (b + carry);
   }
 }
 
 // Ghost g of the concatenated received packets (pk_first[p] = first ghost of
-// packet p, pk_base[p] = its first double, pk_n[p] = its atoms).
+// packet p, pk_base[p] = its first double, pk_n[p] = its atoms).  Where the
+// list build keeps the ghosts in cell order (gslot given, the ghost count
+// unchanged) ghost g goes to device slot gslot[g], so that each ghost is
+// compared with its own build position; otherwise to slot g, and gtag /
+// gslot (when allocated) restart as the identity.
 __global__ void k_halo_unpack(const double *__restrict__ in, int npk,
                               const long long *__restrict__ pk_first,
                               const long long *__restrict__ pk_base,
                               const long long *__restrict__ pk_n, int ng, int n,
-                              double4 *__restrict__ pos, int *__restrict__ id) {
+                              double4 *__restrict__ pos, int *__restrict__ id,
+                              int *__restrict__ gtag, int *__restrict__ gslot, int remap) {
   const int gi = blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= ng) return;
   int p = 0;
   while (p + 1 < npk && pk_first[p + 1] <= gi) ++p;
   const long long N = pk_n[p], c = gi - pk_first[p];
   const double *pk = in + pk_base[p];
-  pos[n + gi] = make_double4(pk[2 * N + 3 * c], pk[2 * N + 3 * c + 1], pk[2 * N + 3 * c + 2],
-                             pk[N + c]);
-  id[n + gi] = (int)pk[c];
+  int k = gi;
+  if (remap) {
+    k = gslot[gi];
+  } else if (gtag) {
+    gtag[gi] = gi;
+    gslot[gi] = gi;
+  }
+  pos[n + k] = make_double4(pk[2 * N + 3 * c], pk[2 * N + 3 * c + 1], pk[2 * N + 3 * c + 2],
+                            pk[N + c]);
+  id[n + k] = (int)pk[c];
+}
+
+// The same placement for ghosts uploaded as arrays (cv2_set_ghosts), staged
+// in gpos_in / gid_in.
+__global__ void k_ghost_place(const double4 *__restrict__ gpos_in, const int *__restrict__ gid_in,
+                              int ng, int n, double4 *__restrict__ pos, int *__restrict__ id,
+                              int *__restrict__ gtag, int *__restrict__ gslot, int remap) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= ng) return;
+  int k = gi;
+  if (remap) {
+    k = gslot[gi];
+  } else {
+    gtag[gi] = gi;
+    gslot[gi] = gi;
+  }
+  pos[n + k] = gpos_in[gi];
+  id[n + k] = gid_in[gi];
 }
 
 }  // namespace
@@ -208,7 +239,7 @@ void launch_halo_classify(const DevState &s, const HaloGeomDev &g, int *code_by_
 
 void launch_halo_scan(int ntiles, int per_atom, const int *tile_counts, int *tile_off, int *dir_n,
                       long long *dir_base, long long *res, hipStream_t st) {
-  k_halo_scan<<<1, 1024, 0, st>>>(tile_counts, ntiles, per_atom, tile_off, dir_n, dir_base, res);
+  k_halo_scan<<<27, 1024, 0, st>>>(tile_counts, ntiles, per_atom, tile_off, dir_n, dir_base, res);
 }
 
 void launch_halo_write(const DevState &s, const HaloGeomDev &g, const int *code_by_tag,
@@ -220,10 +251,18 @@ void launch_halo_write(const DevState &s, const HaloGeomDev &g, const int *code_
 }
 
 void launch_halo_unpack(DevState &s, const double *in, int npk, const long long *pk_first,
-                        const long long *pk_base, const long long *pk_n, int ng, hipStream_t st) {
+                        const long long *pk_base, const long long *pk_n, int ng, bool remap,
+                        hipStream_t st) {
   if (ng == 0) return;
   k_halo_unpack<<<(ng + 255) / 256, 256, 0, st>>>(in, npk, pk_first, pk_base, pk_n, ng, s.n, s.pos,
-                                                   s.id);
+                                                   s.id, s.gtag, s.gslot, remap ? 1 : 0);
+}
+
+void launch_ghost_place(DevState &s, const double4 *gpos_in, const int *gid_in, int ng, bool remap,
+                        hipStream_t st) {
+  if (ng == 0) return;
+  k_ghost_place<<<(ng + 255) / 256, 256, 0, st>>>(gpos_in, gid_in, ng, s.n, s.pos, s.id, s.gtag,
+                                                   s.gslot, remap ? 1 : 0);
 }
 
 }  // namespace cv2

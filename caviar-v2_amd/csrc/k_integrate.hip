@@ -57,12 +57,15 @@ __global__ void __launch_bounds__(kBlock)
     k_check_disp(const double4 *__restrict__ pos, const double4 *__restrict__ pos_ref, int n,
                  int ntot, StepParams p, int *__restrict__ flags) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= ntot) return;
-  const double4 x = pos[i], r = pos_ref[i];
-  const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
-  // the wrap / ghost-band triggers concern owned atoms
-  if (dx * dx + dy * dy + dz * dz > p.thresh_sq || (p.ref_rebuild && i < n && ref_trigger(x, r, p)))
-    atomicOr(&flags[F_REBUILD], 1);
+  bool t = false;
+  if (i < ntot) {
+    const double4 x = pos[i], r = pos_ref[i];
+    const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
+    // the wrap / ghost-band triggers concern owned atoms
+    t = dx * dx + dy * dy + dz * dz > p.thresh_sq || (p.ref_rebuild && i < n && ref_trigger(x, r, p));
+  }
+  // one atomic per wavefront: after a new halo every ghost may trigger
+  if (__ballot(t) != 0ull && (threadIdx.x & 63) == 0) atomicOr(&flags[F_REBUILD], 1);
 }
 
 // Per-block sums of (m v.v, m vx, m vy, m vz, m).

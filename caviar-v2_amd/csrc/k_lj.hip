@@ -410,8 +410,11 @@ __device__ inline void task_atom(const int4 *__restrict__ q, const uint32_t *__r
                                  double u1, double u2, const int *sx, const int *sy,
                                  const int *sz, const unsigned char *stype, const int *sj,
                                  const int4 *pc, int S, const int4 &sent, AccF32 &a,
-                                 float &fcmax) {
+                                 float &fcmax, const int *__restrict__ hid, int n) {
   if (cp == 0) return;
+  // host-halo ghosts (hid set): a pair with a ghost counts once by id, as
+  // lj.cpp:487 (the sentinel, j = 0x7fffffff, is never within the cutoff)
+  const int id_i = (ENERGY && hid) ? hid[i] : 0;
   // the next group is loaded while this one is processed
   u32x4 gn = load_group8(nbr, (size_t)stride, i, 0);
   for (int k = 0; k < cp; k += 8) {
@@ -432,7 +435,9 @@ __device__ inline void task_atom(const int4 *__restrict__ q, const uint32_t *__r
         int j = 0;
         if (ENERGY) j = sl[v] == S ? 0x7fffffff : (STAGED ? sj[sl[v]] : slot_to_j(pc, sl[v]));
         const PairF32 c = MULTI ? pair_f32_table(lj, ti * nt + (qj[v].w & 7)) : c0;
-        pair_f32_entry<ISO, ENERGY, MULTI>(qi, qj[v], u0, u1, u2, c, ENERGY && i < j, a);
+        bool once = ENERGY && i < j;
+        if (ENERGY && hid && j >= n && j != 0x7fffffff) once = id_i < hid[j];
+        pair_f32_entry<ISO, ENERGY, MULTI>(qi, qj[v], u0, u1, u2, c, once, a);
       } else {
         pair_f32_flat<ISO>(qi, qj[v], u0, u1, u2, c0, a.sx, a.sy, a.sz, fcmax);
       }
@@ -570,10 +575,12 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
       float fcmax = 0.0f;
       if (staged)
         task_atom<MULTI, ENERGY, ISO, true>(q, nbr, stride, i, cp, qi, ti, nt, lj, c0, u0, u1, u2, sx,
-                                            sy, sz, stype, sj, pc, S, sent, a, fcmax);
+                                            sy, sz, stype, sj, pc, S, sent, a, fcmax,
+                                            FUSE ? nullptr : fz.id, n);
       else
         task_atom<MULTI, ENERGY, ISO, false>(q, nbr, stride, i, cp, qi, ti, nt, lj, c0, u0, u1, u2,
-                                             sx, sy, sz, stype, sj, pc, S, sent, a, fcmax);
+                                             sx, sy, sz, stype, sj, pc, S, sent, a, fcmax,
+                                             FUSE ? nullptr : fz.id, n);
       // the flat form biases every entry, the branched form counts its terms
       const int terms = (ENERGY || MULTI) ? a.terms : cp;
       const bool ovf = (ENERGY || MULTI) ? a.ovf : !((double)fcmax * fc_rc < 0.9 * 2251799813685248.0);
@@ -766,14 +773,17 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
     long long *part = reinterpret_cast<long long *>(s.partials);
     if (s.slots) {
       const int tgrid = s.task_grid;
+      // unfused launches carry the ids of the host-halo ghosts' pair rule
+      FuseArgs fa{};
+      fa.id = (ENERGY && HALO) ? s.id : nullptr;
 #define CV2_LJT(I)                                                                              \
   do {                                                                                           \
   k_lj_f32_task<MULTI, ENERGY, I, false, false><<<tgrid, kTaskBlock, 0, st>>>(                   \
       s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,    \
-      part, s.flags, FuseArgs{}, tgrid / 8);                                                     \
+      part, s.flags, fa, tgrid / 8);                                                             \
   k_lj_f32_task<MULTI, ENERGY, I, false, true><<<kTailGrid, kTaskBlock, 0, st>>>(                \
       s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,    \
-      part, s.flags, FuseArgs{}, tgrid / 8);                                                     \
+      part, s.flags, fa, tgrid / 8);                                                             \
   } while (0)
       if (!MULTI && b.qiso)
         CV2_LJT(!MULTI);

@@ -85,13 +85,16 @@ def lib():
             "cv2_set_ghosts": ([vp, C.c_int64, _d, _i32, _i64], C.c_int),
             "cv2_halo_pack": ([vp, C.POINTER(HaloGeom), _i64, C.c_int64, _d, _i64], C.c_int),
             "cv2_halo_unpack": ([vp, C.c_int32, _i64, _d, C.c_int32], C.c_int),
+            "cv2_halo_unpack_at": ([vp, C.c_int32, _i64, _i64, _d, C.c_int32], C.c_int),
+            "cv2_host_alloc": ([C.c_int64, C.POINTER(vp)], C.c_int),
+            "cv2_host_free": ([vp], C.c_int),
             "cv2_halo_pack_noalloc": ([vp, C.POINTER(HaloGeom), _i64, C.c_int64, _d, _i64], C.c_int),
             "cv2_build_neighbors": ([vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
             "cv2_compute_forces": ([vp], C.c_int),
             "cv2_compute_accelerations": ([vp], C.c_int),
             "cv2_run_vv": ([vp, C.c_int64, C.c_double], C.c_int),
             "cv2_step_begin": ([vp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_uint64], C.c_int),
-            "cv2_step_end": ([vp], C.c_int),
+            "cv2_step_end": ([vp, C.c_int], C.c_int),
             "cv2_run_langevin": ([vp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint64], C.c_int),
             "cv2_thermo": ([vp, C.c_int, _d, _d, _d], C.c_int),
             "cv2_download": ([vp, _d, _d, _d], C.c_int),
@@ -126,6 +129,21 @@ def lib():
 
 def _p(a, t):
     return None if a is None else a.ctypes.data_as(t)
+
+
+def pinned_empty(n: int) -> np.ndarray:
+    """float64 array of n elements in page-locked host memory
+    (cv2_host_alloc), released when the array is collected."""
+    import weakref
+
+    L = lib()
+    ptr = C.c_void_p()
+    rc = L.cv2_host_alloc(8 * max(int(n), 1), C.byref(ptr))
+    if rc:
+        raise Cv2Error(rc, "cv2_host_alloc failed")
+    a = np.ctypeslib.as_array(C.cast(ptr, _d), shape=(max(int(n), 1),))
+    weakref.finalize(a, L.cv2_host_free, ptr)
+    return a[:int(n)] if n else a[:0]
 
 
 def _f64(a):
@@ -220,6 +238,34 @@ class Context:
         self._chk(fn(self.h, C.byref(g), _p(counts, _i64), need.value, _p(out, _d), C.byref(need)))
         return counts, out[:need.value]
 
+    def halo_pack_into(self, out, grid_index, nprocs, lo_local, hi_local, size_global, bc, ghost_cutoff,
+                       with_velocity=False):
+        """cv2_halo_pack into a caller's float64 buffer in one call: returns
+        (counts, need); the packets are out[:need] when need <= len(out),
+        otherwise out is untouched and the caller grows it."""
+        g = HaloGeom()
+        for a in range(3):
+            g.grid_index[a] = int(grid_index[a]); g.nprocs[a] = int(nprocs[a]); g.bc[a] = int(bc[a])
+            g.lo_local[a] = float(lo_local[a]); g.hi_local[a] = float(hi_local[a])
+            g.size_global[a] = float(size_global[a])
+        g.ghost_cutoff = float(ghost_cutoff)
+        g.with_velocity = int(bool(with_velocity))
+        counts = np.zeros(27, np.int64)
+        need = C.c_int64()
+        assert out.dtype == np.float64 and out.flags.c_contiguous
+        self._chk(self.L.cv2_halo_pack(self.h, C.byref(g), _p(counts, _i64), len(out), _p(out, _d),
+                                       C.byref(need)))
+        return counts, need.value
+
+    def halo_unpack_at(self, counts, offsets, buf, with_velocity=False):
+        """Ghosts from packets at offsets (doubles) of one buffer, appended in
+        the order given (cv2_halo_unpack_at)."""
+        cnt = np.ascontiguousarray(counts, np.int64)
+        off = np.ascontiguousarray(offsets, np.int64)
+        assert buf.dtype == np.float64 and buf.flags.c_contiguous and len(off) == len(cnt)
+        self._chk(self.L.cv2_halo_unpack_at(self.h, len(cnt), _p(cnt, _i64), _p(off, _i64),
+                                            _p(buf, _d) if buf.size else None, int(bool(with_velocity))))
+
     def halo_unpack(self, counts, packets, with_velocity=False):
         """Ghosts from received packets, appended in the order given."""
         cnt = np.ascontiguousarray(counts, np.int64)
@@ -251,9 +297,9 @@ class Context:
         self._chk(self.L.cv2_step_begin(self.h, float(dt), int(bool(langevin)), float(gamma), float(kbt),
                                         int(seed)))
 
-    def step_end(self):
+    def step_end(self, energy=True):
         """List build when needed, forces, second velocity half."""
-        self._chk(self.L.cv2_step_end(self.h))
+        self._chk(self.L.cv2_step_end(self.h, int(bool(energy))))
 
     def synchronize(self):
         self._chk(self.L.cv2_synchronize(self.h))

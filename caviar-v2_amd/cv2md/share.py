@@ -13,8 +13,9 @@ boundary_condition, md_simulator.cpp:422-437,731-869):
   (host)           the packet sent in direction d arrives from direction
                    26 - d; the share appends the received packets in the
                    order they arrive
-  cv2_halo_unpack  the packets become the ghosts (host -> device; a changed
-                   ghost count forces a list rebuild, neighborlist.cpp:117-120)
+  cv2_halo_unpack_at  the packets become the ghosts (host -> device, from
+                   the same page-locked buffer; a changed ghost count forces
+                   a list rebuild, neighborlist.cpp:117-120)
   cv2_step_end     list build when needed, forces, second velocity half
 
 Owned atoms that leave the box are wrapped back by cv2_step_begin, before
@@ -66,50 +67,63 @@ class Share:
         c.set_precision(precision)
         c.upload(s["pos"], s.get("vel"), s.get("type"))
         self.c = c
-        self.exchange_seconds = 0.0
+        self.exchange_seconds = 0.0  # pack (with the device work queued before it) + unpack
+        self.pack_seconds = 0.0
+        self.unpack_seconds = 0.0
         self.exchanges = 0
         self.halo_atoms = 0
+        # the packets: one page-locked buffer, sent and received in place
+        self.buf = D.pinned_empty(0)
         self.exchange()
 
     def exchange(self):
-        """One halo exchange: pack on the device, forward, unpack."""
+        """One halo exchange: pack on the device into the host buffer, then
+        hand the packets back in receive order (slot k receives the packet
+        sent towards 26 - k) without moving them on the host."""
         t0 = time.perf_counter()
         s = self.s
-        counts, pk = self.c.halo_pack((0, 0, 0), (1, 1, 1), s["lo"], s["hi"], self.L, (1, 1, 1),
-                                      s["ghost_cutoff"])
+        geom = ((0, 0, 0), (1, 1, 1), s["lo"], s["hi"], self.L, (1, 1, 1), s["ghost_cutoff"])
+        counts, need = self.c.halo_pack_into(self.buf, *geom)
+        if need > len(self.buf):
+            self.buf = D.pinned_empty(int(need * 1.25) + 1024)
+            counts, need = self.c.halo_pack_into(self.buf, *geom)
+        t1 = time.perf_counter()
         per = 5  # doubles per atom: id, type, x, y, z
-        starts = np.concatenate([[0], np.cumsum(per * counts)])
-        order = [26 - k for k in range(27)]  # slot k receives the packet sent towards 26 - k
-        rc = counts[order]
-        rp = np.concatenate([pk[starts[d]:starts[d + 1]] for d in order]) if len(pk) else pk
-        self.c.halo_unpack(rc, rp)
-        self._recv = (rc, rp)
+        starts = np.concatenate([[0], np.cumsum(per * counts)[:-1]])
+        order = np.arange(26, -1, -1)
+        rc, ro = counts[order], starts[order]
+        self.c.halo_unpack_at(rc, ro, self.buf)
+        self._recv = (rc, ro)
         self.halo_atoms = int(counts.sum())
-        self.exchange_seconds += time.perf_counter() - t0
+        t2 = time.perf_counter()
+        self.pack_seconds += t1 - t0
+        self.unpack_seconds += t2 - t1
+        self.exchange_seconds += t2 - t0
         self.exchanges += 1
 
-    def step(self, dt: float, langevin: bool = True, gamma: float = 0.0, kbt: float = 0.0, seed: int = 0):
+    def step(self, dt: float, langevin: bool = True, gamma: float = 0.0, kbt: float = 0.0, seed: int = 0,
+             energy: bool = True):
         self.c.step_begin(dt, langevin, gamma, kbt, seed)
         self.exchange()
-        self.c.step_end()
+        self.c.step_end(energy)
 
     def run(self, nsteps: int, dt: float, langevin: bool = True, gamma: float = 0.0, kbt: float = 0.0,
             seed: int = 0):
-        for _ in range(nsteps):
-            self.step(dt, langevin, gamma, kbt, seed)
+        """nsteps steps; the energy and virial are summed on the last one
+        only, as cv2_run_* do."""
+        for k in range(nsteps):
+            self.step(dt, langevin, gamma, kbt, seed, energy=k == nsteps - 1)
 
     def ghosts(self):
         """The ghosts of the last exchange as the device holds them (ghost g
         is atom n + g of the pair list): positions (G, 3), types, ids."""
-        rc, rp = self._recv
+        rc, ro = self._recv
         pos, typ, ids = [], [], []
-        at = 0
-        for N in (int(x) for x in rc):
-            pk = rp[at:at + 5 * N]
-            ids.append(pk[:N])
-            typ.append(pk[N:2 * N])
-            pos.append(pk[2 * N:5 * N].reshape(N, 3))
-            at += 5 * N
+        for N, at in zip((int(x) for x in rc), (int(x) for x in ro)):
+            pk = self.buf[at:at + 5 * N]
+            ids.append(pk[:N].copy())
+            typ.append(pk[N:2 * N].copy())
+            pos.append(pk[2 * N:5 * N].reshape(N, 3).copy())
         return (np.concatenate(pos), np.concatenate(typ).astype(np.int32),
                 np.concatenate(ids).astype(np.int64))
 

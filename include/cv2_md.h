@@ -177,6 +177,22 @@ int cv2_halo_pack_noalloc(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[2
 int cv2_halo_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const double *packets,
                     int32_t with_velocity);
 
+/* cv2_halo_unpack for packets that lie anywhere in one host buffer, packet p
+ * at packets + offsets[p] (doubles), appended in the order given: the
+ * receive buffers of an MPI exchange posted at offsets of one allocation
+ * (the reference keeps one recv_data buffer per direction,
+ * exchange_ghost_mpi_shared_atoms.cpp:405-470), forwarded without a host
+ * copy.  The buffer is copied to the device once, up to the end of the
+ * farthest packet. */
+int cv2_halo_unpack_at(cv2_ctx *ctx, int32_t npackets, const int64_t *counts,
+                       const int64_t *offsets, const double *packets, int32_t with_velocity);
+
+/* Page-locked host memory for packet buffers (hipHostMalloc): device copies
+ * from and to it run at the link's full rate.  No reference counterpart
+ * (the reference's buffers are std::vector).  *ptr = NULL for bytes = 0. */
+int cv2_host_alloc(int64_t bytes, void **ptr);
+int cv2_host_free(void *ptr);
+
 /* Neighbour-list build.  Neighborlist::build(bool) (neighborlist.cpp:77-98):
  * wraps positions into the box (exchange_owned_single_md_domain.cpp:29-57),
  * re-sorts atoms by cell and rebuilds the list when force != 0 or when
@@ -220,9 +236,13 @@ int cv2_run_langevin(cv2_ctx *ctx, int64_t nsteps, double dt, double gamma, doub
  * the forces at the new positions (re_calculate_acc, :422-437) and applies
  * the second velocity half (:678 / :860).  Together they give the bits of
  * cv2_run_vv / cv2_run_langevin with nsteps = 1.  cv2_run_* refuse to run
- * while a step is open. */
+ * while a step is open.  energy != 0: the force pass also sums the potential
+ * energy and virial that cv2_thermo reports (the last step of cv2_run_*);
+ * energy = 0: the force pass of the other steps of a run, and cv2_thermo's
+ * energy and virial stay those of the last energy pass.  A NaN raised by
+ * cv2_step_begin's half is reported by cv2_step_end. */
 int cv2_step_begin(cv2_ctx *ctx, double dt, int langevin, double gamma, double kbt, uint64_t seed);
-int cv2_step_end(cv2_ctx *ctx);
+int cv2_step_end(cv2_ctx *ctx, int energy);
 
 /* Thermodynamics of the current state: kinetic energy (kinetic_energy.cpp:
  * 26-70, n_r_df as set), potential energy and virial of the last force pass

@@ -121,3 +121,30 @@ def test_halo_exchange_forces_and_pairs(domains):
         assert np.array_equal(pairs, exp), gi
         ref = a_ref[own]
         assert np.abs(acc - ref).max() < 1e-12 * max(np.abs(ref).max(), 1.0), gi
+
+
+def test_unpack_at_offsets_equals_concatenated(domains):
+    """cv2_halo_unpack_at with packets left where they lie in one page-locked
+    buffer (received in reverse direction order) gives the ghosts, pairs and
+    forces of cv2_halo_unpack over the same packets concatenated."""
+    s, pos, vel, L, doms = domains
+    gi, dom = next(iter(doms.items()))
+    with make_ctx(s, dom, pos, vel) as c:
+        cnt, pk = c.halo_pack(gi, GRID, dom["lo"], dom["hi"], L, (1, 1, 1), GC)
+    starts = np.concatenate([[0], np.cumsum(5 * cnt)[:-1]])
+    order = np.arange(26, -1, -1)
+    flat = np.concatenate([pk[starts[d]:starts[d] + 5 * cnt[d]] for d in order])
+    buf = D.pinned_empty(len(pk) + 7)
+    buf[:len(pk)] = pk
+    out = []
+    for use_at in (False, True):
+        with make_ctx(s, dom, pos, vel) as c:
+            if use_at:
+                c.halo_unpack_at(cnt[order], starts[order], buf)
+            else:
+                c.halo_unpack(cnt[order], flat)
+            c.compute_forces()
+            out.append((sort_rows(c.pairs()), c.download()[2], c.stats()["nghost"]))
+    assert out[0][2] == out[1][2] == int(cnt.sum())
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])

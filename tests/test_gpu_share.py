@@ -14,7 +14,11 @@ changes, neighborlist.cpp:117-120):
 - the pair set over owned atoms and the uploaded ghosts is the oracle's
   cell-list set, bit-exact;
 - the accelerations lie within the FP32 + fixed-point tolerance of the FP64
-  oracle's LJ pass over the same lists.
+  oracle's LJ pass over the same lists: max 1e-5, rms 1.5e-6 in the metric
+  of the 1e-6 bar (|a32 - a64| / max(|a64|, rms|a64|)).  The quenched
+  mixture at rho* = 1.2 holds close A-B pairs whose forces are tens of
+  times an atom's net force, so the per-pair FP32 rounding weighs more
+  here than in the LJ liquid that bar is stated for.
 """
 import numpy as np
 import pytest
@@ -72,4 +76,30 @@ def test_c5_quench_halo_pairs_forces():
                                s["lj_cutoff"], s["mass"], s["cutoff_list"])
             emax, erms = force_error(acc, a_ref)
             print(f"[c5] step {k}: force error max {emax:.2e} rms {erms:.2e}", flush=True)
-            assert emax < 2e-5 and erms < 5e-6
+            assert emax < 1e-5 and erms < 1.5e-6
+
+
+@pytest.mark.parametrize("prec", [D.CV2_PREC_FP64, D.CV2_PREC_FP32_FIXED])
+def test_split_step_matches_run(prec):
+    """cv2_step_begin + cv2_step_end (energy on the last step only) give the
+    bits of cv2_run_langevin over the same steps, rebuilds included."""
+    from cv2md.systems import lj_fluid
+
+    s = lj_fluid(12, temperature=1.2, seed=77, integrator=1, gamma=1.0)
+    out = []
+    for split in (False, True):
+        with D.Context.from_sysdef(s, precision=prec) as c:
+            c.compute_forces()
+            if split:
+                for k in range(40):
+                    c.step_begin(s["dt"], True, 1.0, 1.2, 5)
+                    c.step_end(energy=k == 39)
+            else:
+                c.run_langevin(40, s["dt"], 1.0, 1.2, 5)
+            st = c.stats()
+            th = c.thermo()
+            out.append((c.download(), st["rebuilds"], th["pe"], th["virial"]))
+    assert out[0][1] == out[1][1] and out[0][1] > 1
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert out[0][2] == out[1][2] and out[0][3] == out[1][3]
