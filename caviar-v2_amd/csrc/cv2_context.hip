@@ -306,13 +306,13 @@ int ensure_ready(cv2_ctx *ctx) {
   return CV2_OK;
 }
 
-// Launch grid of the force task kernel for ntask tasks: a multiple of 8
-// (one share per XCD) with 6% + 16 spare work-groups, so that the task
-// count of later builds rarely exceeds it (the tail launch then takes the
-// rest); at most cap (> 0) when set.
-int task_grid_for(int64_t ntask, int64_t nslot, int cap) {
-  int64_t g = ntask + ntask / 16 + 16;
-  g = std::min(g, std::max<int64_t>(nslot, 8));
+// Launch grid of the force task kernel: one work-group per task slot, a
+// multiple of 8 (one share per XCD), so that every build's tasks fit
+// whatever their count (the work-groups beyond it return at once, which
+// costs less than a second launch); at most cap (> 0) when set, the tail
+// launch then taking the rest.
+int task_grid_for(int64_t nslot, int cap) {
+  int64_t g = std::max<int64_t>(nslot, 8);
   if (cap > 0) g = std::min<int64_t>(g, cap);
   return (int)((g + 7) / 8 * 8);
 }
@@ -323,7 +323,6 @@ int check_flags(cv2_ctx *ctx) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
   const int *f = ctx->h_flags;
   DevState &s = ctx->s;
-  if (s.slots && f[F_NTASK] > 0) s.task_grid = task_grid_for(f[F_NTASK], s.tg.nslot, ctx->task_grid_cap);
   if (f[F_TASKOVF]) {
     // some cell row wanted more task slots: its last task took the rest of
     // the row (correct, slower); widen the rows at the next build
@@ -405,8 +404,7 @@ int prepare_list_form(cv2_ctx *ctx) {
   }
   // until a build has counted the tasks: about one per 200 atoms, plus one
   // per row for the row ends
-  if (s.task_grid <= 0 || s.tg.mpr != old.mpr)
-    s.task_grid = task_grid_for((int64_t)s.n / 200 + rows, s.tg.nslot, ctx->task_grid_cap);
+  s.task_grid = task_grid_for(s.tg.nslot, ctx->task_grid_cap);
   const int nb = s.tg.nslot + 80;  // per-block energy partials (main grid <= nslot + 7, tail 64)
   if (nb > ctx->partial_blocks) {
     HIPCHK(dalloc(&s.partials, (size_t)nb * 8));
@@ -887,8 +885,9 @@ int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *t
   s.ng = (int)ng;
   launch_ghost_place(s, s.gpos_alt, s.gid_alt, (int)ng, ctx->b.open && !count_changed, ctx->stream);
   if (count_changed) ctx->need_build = true;  // neighborlist.cpp:118-121
-  if (s.write_q) launch_pack_q(s, ctx->b, ctx->stream);
-  launch_check_disp(s, ctx->b, ref_rebuilds(ctx), ctx->stream);  // ghosts compared by index, :139-144
+  // ghosts compared by index (:139-144) unless the build is forced anyway;
+  // the fixed-point copy is written by the build or the next force pass
+  if (!count_changed) launch_check_disp(s, ctx->b, ref_rebuilds(ctx), ctx->stream);
   HIPCHK(hipStreamSynchronize(ctx->stream));  // p4/idv are stack buffers
   return CV2_OK;
 }
@@ -1037,8 +1036,9 @@ static int halo_unpack_impl(cv2_ctx *ctx, int32_t npackets, const int64_t *count
   launch_halo_unpack(s, ctx->halo_buf, npackets, ctx->halo_base, ctx->halo_base + 64,
                      ctx->halo_base + 128, (int)ng, ctx->b.open && !count_changed, ctx->stream);
   if (count_changed) ctx->need_build = true;  // neighborlist.cpp:118-121
-  if (s.write_q) launch_pack_q(s, ctx->b, ctx->stream);
-  launch_check_disp(s, ctx->b, ref_rebuilds(ctx), ctx->stream);  // ghosts compared by index, :139-144
+  // ghosts compared by index (:139-144) unless the build is forced anyway;
+  // the fixed-point copy is written by the build or the next force pass
+  if (!count_changed) launch_check_disp(s, ctx->b, ref_rebuilds(ctx), ctx->stream);
   HIPCHK(hipStreamSynchronize(ctx->stream));  // tab is a stack buffer
   return CV2_OK;
 }

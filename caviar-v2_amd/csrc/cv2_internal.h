@@ -276,7 +276,9 @@ __device__ inline void integrate_values(double4 &x, double v[3], int id, const d
 // fixed-point coordinates (WRITEQ) and the rebuild check of
 // Neighborlist::update_is_needed (neighborlist.cpp:100-148) against the
 // build position r, raising F_REBUILD.
-template <bool START, bool WRITEQ>
+// EXTRA = false leaves out the owned wrap and the reference rebuild
+// triggers (callers that never set them: fewer live kernel parameters).
+template <bool START, bool WRITEQ, bool EXTRA = true>
 __device__ inline void integrate_store(int i, int n, const double4 &x, const double v[3],
                                        const double4 &r, double4 *__restrict__ pos,
                                        double *__restrict__ vel, int4 *__restrict__ q,
@@ -285,7 +287,7 @@ __device__ inline void integrate_store(int i, int n, const double4 &x, const dou
     raise_fatal(flags, F_NAN);
   if (START) {
     double4 y = x;
-    if (p.wrap) {  // same rule as the wrap at list builds (k_wrap_bin)
+    if (EXTRA && p.wrap) {  // same rule as the wrap at list builds (k_wrap_bin)
       double *c[3] = {&y.x, &y.y, &y.z};
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -300,7 +302,12 @@ __device__ inline void integrate_store(int i, int n, const double4 &x, const dou
     if (WRITEQ) q[i] = pack_q(y.x, y.y, y.z, (int)y.w, p.lo, p.qscale);
     // a wrapped atom has moved by a box length: the list is rebuilt
     const double dx = y.x - r.x, dy = y.y - r.y, dz = y.z - r.z;
-    if (dx * dx + dy * dy + dz * dz > p.thresh_sq || (p.ref_rebuild && ref_trigger(y, r, p)))
+    const bool trig =
+        dx * dx + dy * dy + dz * dz > p.thresh_sq || (EXTRA && p.ref_rebuild && ref_trigger(y, r, p));
+    // one atomic per wavefront (many atoms trigger together at a melt or a
+    // wrap): the lowest triggering lane
+    const unsigned long long m = __ballot(trig);
+    if (trig && __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u)) == 0)
       atomicOr(&flags[F_REBUILD], 1);
   }
   vel[i] = v[0];

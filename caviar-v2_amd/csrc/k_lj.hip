@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
       integrate_values<true, true, true, true>(x, v, idv, aa, fz.p);
     else
       integrate_values<false, true, true, true>(x, v, idv, aa, fz.p);
-    integrate_store<true, true>(ia, n, x, v, r, fz.pos, fz.vel, fz.q_out, fz.p, flags);
+    integrate_store<true, true, false>(ia, n, x, v, r, fz.pos, fz.vel, fz.q_out, fz.p, flags);
   };
 
   auto run_task = [&](int k, double u0, double u1, double u2, const PairF32 &c0, double fc_rc, int nt) {
@@ -509,6 +509,12 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
     const int4 ti4 = tinfo[k];
     const int h0 = ti4.x, h1 = ti4.y, S = ti4.z;
     const bool staged = S + 1 <= stage_limit;  // uniform over the block (sentinel included)
+    const int nh = h1 - h0;
+    const bool sorted = nh <= kTaskBlock;
+    // the list length of this thread's home atom (sort key), loaded while
+    // the neighbourhood is staged
+    const int my_nn = (int)threadIdx.x < nh ? nnbr[h0 + threadIdx.x] : 0;
+    if (threadIdx.x <= kSortBins) lhist[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
       int4 v = make_int4(0, 0, 0, 0);
       if (h0 < h1) {
@@ -547,19 +553,24 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
     // home atoms of similar list length (counting sort on the group count,
     // longest first).  Which thread computes an atom does not change its
     // result.
-    const int nh = h1 - h0;
-    const bool sorted = nh <= kTaskBlock;
     if (sorted) {
-      for (int kk = threadIdx.x; kk <= kSortBins; kk += kTaskBlock) lhist[kk] = 0;
-      __syncthreads();
       int bin = 0, rank = 0;
       if ((int)threadIdx.x < nh) {
-        bin = kSortBins - 1 - min((nnbr[h0 + threadIdx.x] + 7) >> 3, kSortBins - 1);
+        bin = kSortBins - 1 - min((my_nn + 7) >> 3, kSortBins - 1);
         rank = atomicAdd(&lhist[bin + 1], 1);
       }
       __syncthreads();
-      if (threadIdx.x == 0)
-        for (int kk = 1; kk <= kSortBins; ++kk) lhist[kk] += lhist[kk - 1];
+      // bin offsets: an inclusive scan of the 64 counts by the first wavefront
+      static_assert(kSortBins == 64, "one wavefront scans the bins");
+      if (threadIdx.x < 64) {
+        int c = lhist[threadIdx.x + 1];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const int v = __shfl_up(c, d, 64);
+          c += (int)threadIdx.x >= d ? v : 0;
+        }
+        lhist[threadIdx.x + 1] = c;
+      }
       __syncthreads();
       if ((int)threadIdx.x < nh) lorder[lhist[bin] + rank] = (unsigned char)threadIdx.x;
     }
@@ -773,6 +784,7 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
     long long *part = reinterpret_cast<long long *>(s.partials);
     if (s.slots) {
       const int tgrid = s.task_grid;
+      const bool tail = tgrid < (s.tg.nslot + 7) / 8 * 8;  // a capped grid: the rest in a tail launch
       // unfused launches carry the ids of the host-halo ghosts' pair rule
       FuseArgs fa{};
       fa.id = (ENERGY && HALO) ? s.id : nullptr;
@@ -781,16 +793,17 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
   k_lj_f32_task<MULTI, ENERGY, I, false, false><<<tgrid, kTaskBlock, 0, st>>>(                   \
       s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,    \
       part, s.flags, fa, tgrid / 8);                                                             \
-  k_lj_f32_task<MULTI, ENERGY, I, false, true><<<kTailGrid, kTaskBlock, 0, st>>>(                \
-      s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,    \
-      part, s.flags, fa, tgrid / 8);                                                             \
+  if (tail)                                                                                      \
+    k_lj_f32_task<MULTI, ENERGY, I, false, true><<<kTailGrid, kTaskBlock, 0, st>>>(              \
+        s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,  \
+        part, s.flags, fa, tgrid / 8);                                                           \
   } while (0)
       if (!MULTI && b.qiso)
         CV2_LJT(!MULTI);
       else
         CV2_LJT(false);
 #undef CV2_LJT
-      if (ENERGY) k_reduce_i64<<<1, 1024, 0, st>>>(part, tgrid + kTailGrid, s.fx_acc);
+      if (ENERGY) k_reduce_i64<<<1, 1024, 0, st>>>(part, tgrid + (tail ? kTailGrid : 0), s.fx_acc);
       return;
     }
     if (!MULTI && b.qiso)
@@ -843,15 +856,19 @@ void launch_forces_ens(DevState &s, const EnsDev &e, const LJDev *lj_dev, int bl
 
 bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                          const UpdateArgs &u, hipStream_t st) {
+  // (the fused epilogue has no owned wrap and no reference rebuild triggers)
   if (!s.slots || s.n == 0 || s.ng > 0 || !s.q_alt || !u.finish || !u.start) return false;
+  if (u.ref_rebuild || u.wrap_owned) return false;
   FuseArgs fz{s.pos, s.pos_ref, s.vel, s.id, s.q_alt, step_params(b, u), u.langevin ? 1 : 0};
   long long *part = reinterpret_cast<long long *>(s.partials);
   const int tgrid = s.task_grid;
+  const bool tail = tgrid < (s.tg.nslot + 7) / 8 * 8;
 #define CV2_LJF(M, I)                                                                           \
   do {                                                                                           \
   k_lj_f32_task<M, false, I, true, false><<<tgrid, kTaskBlock, 0, st>>>(                          \
       s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,    \
       part, s.flags, fz, tgrid / 8);                                                             \
+  if (tail)                                                                                      \
   k_lj_f32_task<M, false, I, true, true><<<kTailGrid, kTaskBlock, 0, st>>>(                       \
       s.q, s.nbr, s.nnbr, s.stride, s.n, s.ctinfo, s.ctpiece, s.stage_limit, lj_dev, b, s.acc,    \
       part, s.flags, fz, tgrid / 8);                                                             \
