@@ -483,6 +483,7 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
   __shared__ unsigned char stype[MULTI ? kStage : 1];
   __shared__ int sj[ENERGY ? kStage : 1];
   __shared__ int4 pc[kTaskPieces];
+  __shared__ unsigned char pstart[kStage / 64 + 2];  // piece holding slot 64 c
   __shared__ long long sh[kTaskBlock / 64];
   __shared__ int lhist[kSortBins + 1];
   __shared__ unsigned char lorder[kTaskBlock];
@@ -505,7 +506,17 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
   };
 
   auto run_task = [&](int k, double u0, double u1, double u2, const PairF32 &c0, double fc_rc, int nt) {
-    if (threadIdx.x < kTaskPieces) pc[threadIdx.x] = tpiece[(size_t)k * kTaskPieces + threadIdx.x];
+    const int4 *tpk = tpiece + (size_t)k * kTaskPieces;
+    if (threadIdx.x < kTaskPieces) pc[threadIdx.x] = tpk[threadIdx.x];
+    // the piece of slot 64 c, counted from the piece bases (uniform loads),
+    // so that staging starts each 64-slot chunk at its piece
+    if (threadIdx.x <= kStage / 64 + 1) {
+      const int sl0 = (int)threadIdx.x * 64;
+      int p = 0;
+#pragma unroll
+      for (int j = 1; j < kTaskPieces; ++j) p += tpk[j].y <= sl0 ? 1 : 0;
+      pstart[threadIdx.x] = (unsigned char)p;
+    }
     const int4 ti4 = tinfo[k];
     const int h0 = ti4.x, h1 = ti4.y, S = ti4.z;
     const bool staged = S + 1 <= stage_limit;  // uniform over the block (sentinel included)
@@ -529,11 +540,14 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
       // slots threadIdx.x + k * kTaskBlock; their pieces are non-decreasing in k
       constexpr int K = kStage / kTaskBlock;
       int jj[K];
-      int p = 0;
 #pragma unroll
       for (int kk = 0; kk < K; ++kk) {
         const int sl = kk * kTaskBlock + (int)threadIdx.x;
-        while (p + 1 < kTaskPieces && pc[p + 1].y <= sl) ++p;
+        // the chunk's first piece; most chunks lie in one piece (no search),
+        // the rest cross few boundaries (uniform over the wavefront)
+        int p = pstart[sl >> 6];
+        if (p != (int)pstart[(sl >> 6) + 1])
+          while (p + 1 < kTaskPieces && pc[p + 1].y <= sl) ++p;
         jj[kk] = pc[p].x + (sl - pc[p].y);
       }
 #pragma unroll
