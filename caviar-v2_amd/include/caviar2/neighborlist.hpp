@@ -32,6 +32,10 @@ class Neighborlist {
   void set_build_cell_list(bool v) { build_cell_list = v; }
   void set_all_atom_test(bool v) { all_atom_test = v; }
   void set_rebuild_test(bool v) { rebuild_test = v; }
+  // Addition (default false): also rebuild on the reference's other
+  // triggers, an atom's wrap and a changed ghost set, so that the rebuild
+  // steps are the reference's (the device's images need neither)
+  void set_reference_rebuilds(bool v) { reference_rebuilds = v; }
   void set_dt(double v);
 
   // neighborlist.cpp:303-390 (cutoff_extra and the rebuild threshold)
@@ -46,6 +50,7 @@ class Neighborlist {
   bool build_cell_list = false;
   bool all_atom_test = false;
   bool rebuild_test = false;
+  bool reference_rebuilds = false;
   double cutoff = 0.0;
   double cutoff_extra = 0.0;
   double cutoff_extra_coef = 1.12347;

@@ -117,7 +117,8 @@ void configure(DeviceSession &s, Precision precision, int device) {
   s.check(cv2_set_jump_tol(s.ctx, lj->device_jump_tol()), "cv2_set_jump_tol");
   Neighborlist &nl = cv->neighborlist;
   s.check(cv2_set_neighbor(s.ctx, nl.cutoff, nl.cutoff_extra_coef,
-                           nl.rebuild_test ? CV2_NL_REBUILD_EVERY_STEP : 0u),
+                           (nl.rebuild_test ? CV2_NL_REBUILD_EVERY_STEP : 0u) |
+                               (nl.reference_rebuilds ? CV2_NL_REFERENCE_REBUILDS : 0u)),
           "cv2_set_neighbor");
   s.check(cv2_set_precision(s.ctx, precision == Precision::fp64 ? CV2_PREC_FP64 : CV2_PREC_FP32_FIXED),
           "cv2_set_precision");
