@@ -410,11 +410,22 @@ __device__ inline void task_atom(const int4 *__restrict__ q, const uint32_t *__r
                                  double u1, double u2, const int *sx, const int *sy,
                                  const int *sz, const unsigned char *stype, const int *sj,
                                  const int4 *pc, int S, const int4 &sent, AccF32 &a,
-                                 float &fcmax, const int *__restrict__ hid, int n) {
+                                 float &fcmax, const int *__restrict__ hid, int n, bool flat2,
+                                 double &rc_flat) {
   if (cp == 0) return;
   // host-halo ghosts (hid set): a pair with a ghost counts once by id, as
   // lj.cpp:487 (the sentinel, j = 0x7fffffff, is never within the cutoff)
   const int id_i = (ENERGY && hid) ? hid[i] : 0;
+  // flat2 (two species, no energy, no jump_fix): the branchless entry of the
+  // single-species kernel, with the pair constants of the home atom's two
+  // type pairs held in registers and selected per entry by the neighbour's
+  // type instead of gathered from the table
+  PairF32 cA{}, cB{};
+  if (MULTI && flat2) {
+    cA = pair_f32_table(lj, ti * 2);
+    cB = pair_f32_table(lj, ti * 2 + 1);
+    rc_flat = sqrt(fmax(cA.rc2, cB.rc2));
+  }
   // the next group is loaded while this one is processed
   u32x4 gn = load_group8(nbr, (size_t)stride, i, 0);
   for (int k = 0; k < cp; k += 8) {
@@ -431,7 +442,16 @@ __device__ inline void task_atom(const int4 *__restrict__ q, const uint32_t *__r
     }
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-      if (ENERGY || MULTI) {
+      if (MULTI && !ENERGY && flat2) {  // uniform over the launch
+        const bool b = (qj[v].w & 1) != 0;
+        PairF32 c;
+        c.sig2 = b ? cB.sig2 : cA.sig2;
+        c.rc2if = b ? cB.rc2if : cA.rc2if;
+        c.e6 = b ? cB.e6 : cA.e6;
+        c.e12 = b ? cB.e12 : cA.e12;
+        c.fsh = b ? cB.fsh : cA.fsh;
+        pair_f32_flat<ISO>(qi, qj[v], u0, u1, u2, c, a.sx, a.sy, a.sz, fcmax);
+      } else if (ENERGY || MULTI) {
         int j = 0;
         if (ENERGY) j = sl[v] == S ? 0x7fffffff : (STAGED ? sj[sl[v]] : slot_to_j(pc, sl[v]));
         const PairF32 c = MULTI ? pair_f32_table(lj, ti * nt + (qj[v].w & 7)) : c0;
@@ -505,6 +525,8 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
     integrate_store<true, true, false>(ia, n, x, v, r, fz.pos, fz.vel, fz.q_out, fz.p, flags);
   };
 
+  // two species without jump_fix: the branchless entry (task_atom)
+  const bool flat2 = MULTI && !ENERGY && lj->nt == 2 && lj->jump_tol == 0.0;
   auto run_task = [&](int k, double u0, double u1, double u2, const PairF32 &c0, double fc_rc, int nt) {
     const int4 *tpk = tpiece + (size_t)k * kTaskPieces;
     if (threadIdx.x < kTaskPieces) pc[threadIdx.x] = tpk[threadIdx.x];
@@ -598,17 +620,19 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
       const int cp = (nnbr[i] + 7) & ~7;
       AccF32 a;
       float fcmax = 0.0f;
+      double rc_flat = fc_rc;
       if (staged)
         task_atom<MULTI, ENERGY, ISO, true>(q, nbr, stride, i, cp, qi, ti, nt, lj, c0, u0, u1, u2, sx,
                                             sy, sz, stype, sj, pc, S, sent, a, fcmax,
-                                            FUSE ? nullptr : fz.id, n);
+                                            FUSE ? nullptr : fz.id, n, flat2, rc_flat);
       else
         task_atom<MULTI, ENERGY, ISO, false>(q, nbr, stride, i, cp, qi, ti, nt, lj, c0, u0, u1, u2,
                                              sx, sy, sz, stype, sj, pc, S, sent, a, fcmax,
-                                             FUSE ? nullptr : fz.id, n);
+                                             FUSE ? nullptr : fz.id, n, flat2, rc_flat);
       // the flat form biases every entry, the branched form counts its terms
-      const int terms = (ENERGY || MULTI) ? a.terms : cp;
-      const bool ovf = (ENERGY || MULTI) ? a.ovf : !((double)fcmax * fc_rc < 0.9 * 2251799813685248.0);
+      const bool flat = !ENERGY && (!MULTI || flat2);
+      const int terms = flat ? cp : a.terms;
+      const bool ovf = flat ? !((double)fcmax * rc_flat < 0.9 * 2251799813685248.0) : a.ovf;
       const double mi = lj->mass_inv[ti] * (1.0 / kForceScale);
       const double ai[3] = {fx_value(a.sx, terms) * mi, fx_value(a.sy, terms) * mi,
                             fx_value(a.sz, terms) * mi};
