@@ -379,8 +379,10 @@ __device__ inline void pair_f32_flat(const int4 qi, const int4 qj, double u0, do
   // the cutoff, where the shifted force is zero to first order
   // (forceCoef(rc) = 0, lj.cpp:475-484).  SELF: the list pads with the atom
   // itself (r^2 = 0), which adds nothing.
-  float fsel = (SELF ? (r2 <= c.rc2 && r2 > 0.0) : (r2if >= c.rc2if)) ? fc : 0.0f;
-  __asm__ volatile("" : "+v"(fsel));  // select in FP32, before the conversion
+  // (an empty asm here had pinned the select before the conversion; it also
+  // kept the scheduler from filling the compare's hazard slots: 21 s_nop per
+  // 8 entries with it, 15 without, same results)
+  const float fsel = (SELF ? (r2 <= c.rc2 && r2 > 0.0) : (r2if >= c.rc2if)) ? fc : 0.0f;
   fcmax = fmaxf(fcmax, fabsf(fsel));
   const double fcs = (double)fsel;
   sx += (unsigned long long)__double_as_longlong(fma(fcs, dx, kMagic));
