@@ -448,7 +448,7 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
   EHIP(ralloc(&ens->d_box, nsys));
   EHIP(ralloc(&ens->d_sflag, nsys));
   EHIP(ralloc(&ens->d_srebuilds, nsys));
-  EHIP(ralloc(&ens->d_act, 2 * nsys + 4));
+  EHIP(ralloc(&ens->d_act, 4 * nsys + 8));  // systems and cells of a build (EnsDev::act)
   EHIP(ralloc(&ens->eatom, 2 * n));
   EHIP(ralloc(&ens->d_thermo, 3 * nsys));
   EHIP(hipMemcpyAsync(ens->d_sys, sys.data(), sizeof(int) * n, hipMemcpyHostToDevice, ens->stream));

@@ -448,9 +448,13 @@ struct EnsDev {
   int *srebuilds;       // per system: builds done
   // the flagged systems of a build (k_ens_begin): act[0] = their number m,
   // act[1] = their atoms, act[2 .. 2+m] = atom prefix, act[3+nsys ..] their
-  // first atoms; the per-atom build kernels walk only these atoms
+  // first atoms; the per-atom build kernels walk only these atoms.  From
+  // act[ens_act_cells(nsys)]: the same for the flagged systems on the wave
+  // build and their cells (count, total cells, cell prefix, first cells).
   int *act;
 };
+// first index of the cell part of EnsDev::act (4 nsys + 8 ints in all)
+__host__ __device__ inline int ens_act_cells(int nsys) { return 3 + 2 * nsys; }
 
 // One ensemble atom's velocity-Verlet pass (k_ens_update, and the epilogue of
 // the fused ensemble force kernel): FINISH closes step s with a (v += a dt/2),

@@ -1421,6 +1421,7 @@ __global__ void k_ens_policy(EnsDev e, const int *__restrict__ flags) {
 // Clear the flagged systems' cell counts; the last work-group also lists
 // the flagged systems (EnsDev::act) in system order.
 constexpr int kEnsBegin = 1024;
+constexpr int kEnsListGrid = 8192;  // work-groups of the ensemble's wave build
 __global__ void __launch_bounds__(kEnsBegin)
     k_ens_begin(EnsDev e, int *__restrict__ cell_count, int ncell, const int *__restrict__ flags) {
   if (flags[F_REBUILD] == 0) return;
@@ -1429,41 +1430,60 @@ __global__ void __launch_bounds__(kEnsBegin)
     if (c < ncell && e.sflag[e.cell_sys[c]]) cell_count[c] = 0;
     return;
   }
-  __shared__ int wsum[2][kEnsBegin / 64];
-  __shared__ int carry[2];
+  __shared__ int wsum[4][kEnsBegin / 64];
+  __shared__ int carry[4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (threadIdx.x < 2) carry[threadIdx.x] = 0;
+  if (threadIdx.x < 4) carry[threadIdx.x] = 0;
   __syncthreads();
   int *pre = e.act + 2, *first = e.act + 3 + e.nsys;
+  // the flagged systems on the wave build and their cells (k_ens_nlist_cells)
+  const int cb = ens_act_cells(e.nsys);
+  int *cpre = e.act + cb + 2, *cfirst = e.act + cb + 3 + e.nsys;
   for (int s0 = 0; s0 < e.nsys; s0 += kEnsBegin) {
     const int s = s0 + (int)threadIdx.x;
     const bool on = s < e.nsys && e.sflag[s] != 0;
     const int na = on ? e.atom_off[s + 1] - e.atom_off[s] : 0;
-    int x = on ? 1 : 0, y = na;  // inclusive scans: flagged systems, their atoms
+    const bool onc = on && e.box[s].coarse;
+    const int nc = onc ? e.box[s].nc[0] * e.box[s].nc[1] * e.box[s].nc[2] : 0;
+    // inclusive scans: flagged systems, their atoms; wave-build ones, their cells
+    int x = on ? 1 : 0, y = na, xc = onc ? 1 : 0, z = nc;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const int xv = __shfl_up(x, d, 64), yv = __shfl_up(y, d, 64);
+      const int xcv = __shfl_up(xc, d, 64), zv = __shfl_up(z, d, 64);
       x += lane >= d ? xv : 0;
       y += lane >= d ? yv : 0;
+      xc += lane >= d ? xcv : 0;
+      z += lane >= d ? zv : 0;
     }
     if (lane == 63) {
       wsum[0][w] = x;
       wsum[1][w] = y;
+      wsum[2][w] = xc;
+      wsum[3][w] = z;
     }
     __syncthreads();
-    int bx = carry[0], by = carry[1];
+    int bx = carry[0], by = carry[1], bxc = carry[2], bz = carry[3];
     for (int k = 0; k < w; ++k) {
       bx += wsum[0][k];
       by += wsum[1][k];
+      bxc += wsum[2][k];
+      bz += wsum[3][k];
     }
     if (on) {  // the (bx + x - 1)-th flagged system
       pre[bx + x - 1] = by + y - na;
       first[bx + x - 1] = e.atom_off[s];
     }
+    if (onc) {
+      cpre[bxc + xc - 1] = bz + z - nc;
+      cfirst[bxc + xc - 1] = e.cell_off[s];
+    }
     __syncthreads();
     if (threadIdx.x == kEnsBegin - 1) {
       carry[0] = bx + x;
       carry[1] = by + y;
+      carry[2] = bxc + xc;
+      carry[3] = bz + z;
     }
     __syncthreads();
   }
@@ -1471,6 +1491,9 @@ __global__ void __launch_bounds__(kEnsBegin)
     e.act[0] = carry[0];
     e.act[1] = carry[1];
     pre[carry[0]] = carry[1];
+    e.act[cb] = carry[2];
+    e.act[cb + 1] = carry[3];
+    cpre[carry[2]] = carry[3];
   }
 }
 
@@ -1677,41 +1700,64 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
     k_ens_nlist_cells(const double4 *__restrict__ pos, const int4 *__restrict__ q, EnsDev e,
                       int ncell, const int *__restrict__ cstart, uint32_t *__restrict__ nbr,
                       int *__restrict__ nnbr, int stride, int cap, const int *__restrict__ flags,
-                      int nblk_real, int2 *__restrict__ bstat) {
+                      int2 *__restrict__ bstat) {
   if (flags[F_REBUILD] == 0) return;
-  const int per = gridDim.x >> 3;
-  const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  if (blk >= nblk_real) return;
   __shared__ __attribute__((aligned(16))) uint32_t lds[4][kPassCand + 4 + 64];
   __shared__ int cnt_sh[4][kMaxHomeCell];
   __shared__ int2 st_sh[4];
+  __shared__ int cpre_sh[kActLds + 1];
+  // the cells of the flagged wave-build systems (k_ens_begin), in groups of
+  // four (one per wavefront); each XCD takes an eighth of the groups and its
+  // work-groups stride through them, so that a step whose few systems
+  // rebuild launches no idle work-groups for the others
+  const int cb = ens_act_cells(e.nsys);
+  const int cm = e.act[cb], ctot = e.act[cb + 1];
+  const int *cpre = e.act + cb + 2, *cfirst = e.act + cb + 3 + e.nsys;
+  if (cm + 1 <= kActLds) {
+    for (int k = threadIdx.x; k <= cm; k += blockDim.x) cpre_sh[k] = cpre[k];
+    __syncthreads();
+    cpre = cpre_sh;
+  }
   const int wv = (int)(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int c = __builtin_amdgcn_readfirstlane(blk * 4 + wv);
+  const int ng = (ctot + 3) / 4;
+  const int xcd = (int)(blockIdx.x & 7), per = (int)(gridDim.x >> 3);
+  const int g1 = (int)((long long)ng * (xcd + 1) / 8);
   int2 st = make_int2(0, 0);
-  if (c < ncell) {
+  for (int g = (int)((long long)ng * xcd / 8) + (int)(blockIdx.x >> 3); g < g1; g += per) {
+    const int t = g * 4 + wv;  // uniform over the wavefront
+    if (t >= ctot) continue;
+    int a = 0, z = cm - 1;  // the system of active cell t
+    while (a < z) {
+      const int mid = (a + z + 1) >> 1;
+      if (cpre[mid] <= t)
+        a = mid;
+      else
+        z = mid - 1;
+    }
+    const int c = __builtin_amdgcn_readfirstlane(cfirst[a] + (t - cpre[a]));
     const int sy = __builtin_amdgcn_readfirstlane(e.cell_sys[c]);
     const BoxDev *bd = e.box + sy;
-    if (e.sflag[sy] && bd->coarse) {
-      CellWaveBox cb{};
-      for (int d = 0; d < 3; ++d) {
-        cb.nc[d] = bd->nc[d];
-        cb.cunit[d] = bd->cunit[d];
-        if (bd->nc[d] < 5) cb.wide |= 1 << d;
-      }
-      cb.cthr_in = bd->cthr_in;
-      cb.cthr_out = bd->cthr_out;
-      const int off = e.cell_off[sy];
-      st = cell_wave<true, false>(pos, q, cb, bd, cstart + off, nullptr, nbr, nnbr, stride, cap,
-                                  c - off, wv, lane, lds, nullptr, cnt_sh, nullptr, nullptr, nullptr);
+    CellWaveBox cbx{};
+    for (int d = 0; d < 3; ++d) {
+      cbx.nc[d] = bd->nc[d];
+      cbx.cunit[d] = bd->cunit[d];
+      if (bd->nc[d] < 5) cbx.wide |= 1 << d;
     }
+    cbx.cthr_in = bd->cthr_in;
+    cbx.cthr_out = bd->cthr_out;
+    const int off = e.cell_off[sy];
+    const int2 r = cell_wave<true, false>(pos, q, cbx, bd, cstart + off, nullptr, nbr, nnbr, stride,
+                                          cap, c - off, wv, lane, lds, nullptr, cnt_sh, nullptr,
+                                          nullptr, nullptr);
+    st = make_int2(max(st.x, r.x), st.y + r.y);
   }
   if (lane == 0) st_sh[wv] = st;
   __syncthreads();
   if (threadIdx.x == 0) {
     int2 t = st_sh[0];
     for (int w = 1; w < 4; ++w) t = make_int2(max(t.x, st_sh[w].x), t.y + st_sh[w].y);
-    bstat[blk] = t;
+    bstat[blockIdx.x] = t;
   }
 }
 
@@ -1781,10 +1827,11 @@ void launch_build_ens(DevState &s, const EnsDev &e, int ncell, int block, bool p
   if (per_atom)
     k_ens_nlist<<<nb, block, 0, st>>>(s.pos, n, e, s.cell_start, s.nbr, s.nnbr, s.stride, s.cap,
                                        s.flags, s.wstat);
-  const int nbc = nblk(ncell, 4), gridc = (nbc + 7) / 8 * 8;
+  // a fixed grid (captured in the step graph) striding over the active cells
+  const int gridc = std::min((nblk(ncell, 4) + 7) / 8 * 8, kEnsListGrid);
   k_ens_nlist_cells<<<gridc, 256, 0, st>>>(s.pos, s.q, e, ncell, s.cell_start, s.nbr, s.nnbr,
-                                           s.stride, s.cap, s.flags, nbc, s.wstat + nb);
-  k_ens_done<<<1, 1024, 0, st>>>(s.flags, s.wstat, nb + nbc, s.cap, e);
+                                           s.stride, s.cap, s.flags, s.wstat + nb);
+  k_ens_done<<<1, 1024, 0, st>>>(s.flags, s.wstat, nb + gridc, s.cap, e);
 }
 
 void launch_set_flag(int *flags, int which, int value, hipStream_t st) {
