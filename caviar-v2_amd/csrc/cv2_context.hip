@@ -196,7 +196,7 @@ void free_state(DevState &s) {
                   s.gcell_count, s.gcell_start, s.gcell_items, s.tile_sum, s.nbr, s.nnbr, s.flags,
                   s.partials, s.fx_acc, s.reduce_out, s.tpiece, s.tinfo, s.ctask, s.rowtasks,
                   s.ctinfo, s.ctpiece, s.q_alt, s.gtag, s.gslot, s.gtag_alt, s.gid_alt,
-                  s.gpos_alt};
+                  s.gpos_alt, s.mol};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   s = DevState{};
@@ -384,8 +384,9 @@ int prepare_list_form(cv2_ctx *ctx) {
   // the task sentinel is more than sqrt(2) list cutoffs from its home atoms
   double rc2max = 0.0;
   for (int k = 0; k < ctx->lj.nt * ctx->lj.nt; ++k) rc2max = std::max(rc2max, ctx->lj.rc2[k]);
+  // (ignore_intra_molecule runs on the gather kernels, 32-bit entries)
   s.slots = ctx->precision != CV2_PREC_FP64 && ctx->b.coarse && s.n > 0 &&
-            rc2max < 1.9 * ctx->b.ce_sq;
+            rc2max < 1.9 * ctx->b.ce_sq && !s.mol;
   if (!s.slots) return CV2_OK;
   const TaskGeom old = s.tg;
   s.tg = task_geom(s.n, ctx->b, ctx->min_mpr);
@@ -753,6 +754,34 @@ int cv2_set_jump_tol(cv2_ctx *ctx, double jump_tol) {
   return CV2_OK;
 }
 
+int cv2_set_molecules(cv2_ctx *ctx, int64_t n, const int32_t *molecule_index,
+                      int ignore_intra_molecule) {
+  if (!ctx) return CV2_ERR_BAD_ARG;
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload owned atoms before their molecule indices");
+  DevState &s = ctx->s;
+  (void)hipSetDevice(ctx->device);
+  if (!ignore_intra_molecule) {
+    if (s.mol) (void)hipFree(s.mol);
+    s.mol = nullptr;
+  } else {
+    if (n != ctx->n) return fail(ctx, CV2_ERR_BAD_ARG, "one molecule index per owned atom");
+    if (ctx->host_halo)
+      return fail(ctx, CV2_ERR_STATE,
+                  "ignore_intra_molecule on a host-halo share: the halo packets carry no molecule "
+                  "indices (exchange_ghost_mpi_shared_atoms.cpp:405-470)");
+    std::vector<int> m((size_t)n, -1);  // atom_data.cpp:625: -1 outside molecules
+    if (molecule_index)
+      for (int64_t i = 0; i < n; ++i) m[i] = molecule_index[i];
+    if (!s.mol) HIPCHK(dalloc(&s.mol, std::max<int64_t>(n, 1)));
+    HIPCHK(hipMemcpyAsync(s.mol, m.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  // the list form follows (32-bit entries while the flag is on)
+  ctx->need_build = true;
+  ctx->forces_valid = false;
+  return CV2_OK;
+}
+
 int cv2_set_neighbor(cv2_ctx *ctx, double cutoff, double extra_coef, unsigned flags) {
   if (!ctx) return CV2_ERR_BAD_ARG;
   if (cutoff <= 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "cutoff have to non-negative.");
@@ -803,6 +832,8 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
   ctx->pos_cap = 0;
   ctx->n = n;
   s.n = ni;
+  if (s.mol) (void)hipFree(s.mol);  // molecule indices belong to the previous atoms
+  s.mol = nullptr;
   {
     int rc = alloc_positions(ctx, n + ctx->ng);
     if (rc) return rc;
@@ -857,6 +888,8 @@ int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *t
                    const int64_t *id) {
   if (!ctx || ng < 0 || (ng > 0 && !pos)) return fail(ctx, CV2_ERR_BAD_ARG, "bad ghost arrays");
   if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload owned atoms before ghosts");
+  if (ctx->s.mol)
+    return fail(ctx, CV2_ERR_STATE, "host-supplied ghosts carry no molecule indices (ignore_intra_molecule is on)");
   (void)hipSetDevice(ctx->device);
   const bool count_changed = ng != ctx->ng || !ctx->host_halo;
   if (!ctx->host_halo) {
@@ -984,6 +1017,8 @@ static int halo_unpack_impl(cv2_ctx *ctx, int32_t npackets, const int64_t *count
   if (!ctx || npackets < 0 || npackets > 64 || (npackets > 0 && !counts))
     return fail(ctx, CV2_ERR_BAD_ARG, "bad halo unpack arguments (at most 64 packets)");
   if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload owned atoms before ghosts");
+  if (ctx->s.mol)
+    return fail(ctx, CV2_ERR_STATE, "halo packets carry no molecule indices (ignore_intra_molecule is on)");
   (void)hipSetDevice(ctx->device);
   const int64_t total = with_velocity ? 8 : 5;
   long long first[64], pbase[64], pn[64];

@@ -400,6 +400,9 @@ struct DevState {
   double *vel, *vel_alt;  // 3 * n each (SoA: x block, y block, z block)
   double *acc, *acc_alt;
   int *tag, *tag_alt, *id, *id_alt;
+  // Lj::ignore_intra_molecule (lj.cpp:446): molecule index of each owned
+  // atom by upload index (read through tag); null when the flag is off
+  int *mol;
   BoxDev *box_dev;              // device copy of the box constants
   int2 *wstat;                  // per-block build statistics (longest list, entries)
   int64_t wstat_cap;

@@ -79,6 +79,7 @@ template <bool MULTI, bool ENERGY, bool HALO>
 __global__ void __launch_bounds__(kBlock)
     k_lj_f64(const double4 *__restrict__ pos, const uint32_t *__restrict__ nbr,
              const int *__restrict__ nnbr, int stride, int n, const int *__restrict__ id,
+             const int *__restrict__ tag, const int *__restrict__ mol,
              const LJDev *__restrict__ lj, BoxDev b, double *__restrict__ acc,
              double *__restrict__ partials, int *__restrict__ flags, int nblk_real) {
   __shared__ double sh[kBlock / 64];
@@ -96,6 +97,9 @@ __global__ void __launch_bounds__(kBlock)
     const int cp = (nnbr[i] + 3) & ~3;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int id_i = (ENERGY && HALO) ? id[i] : 0;
+    // ignore_intra_molecule: pairs of one molecule index are skipped,
+    // atoms outside molecules (-1) included, as lj.cpp:446 compares them
+    const int mol_i = (MULTI && mol) ? mol[tag[i]] : 0;
     for (int k = 0; k < cp; k += 4) {
       const u32x4 g = load_group(nbr, (size_t)stride, i, k);
       const uint32_t e[4] = {g.x, g.y, g.z, g.w};
@@ -116,7 +120,8 @@ __global__ void __launch_bounds__(kBlock)
         double r2 = dx * dx + dy * dy + dz * dz;
         const int tp = MULTI ? ti * nt + (int)pj[u].w : 0;
         const double rc2 = MULTI ? lj->rc2[tp] : rc2_0;
-        if (r2 <= rc2 && r2 > 0.0) {
+        const bool same_mol = MULTI && mol && mol[tag[j]] == mol_i;
+        if (r2 <= rc2 && r2 > 0.0 && !same_mol) {
           if (MULTI && r2 < jt) r2 = jt;  // lj.cpp:459-468
           // 1/r^2: hardware reciprocal and two Newton steps (about 1 ulp)
           double r2i = __builtin_amdgcn_rcp(r2);
@@ -267,6 +272,7 @@ template <bool MULTI, bool ENERGY, bool HALO, bool ISO>
 __global__ void __launch_bounds__(kBlock)
     k_lj_f32fx(const int4 *__restrict__ q, const uint32_t *__restrict__ nbr,
                const int *__restrict__ nnbr, int stride, int n, const int *__restrict__ id,
+               const int *__restrict__ tag, const int *__restrict__ mol,
                const LJDev *__restrict__ lj, BoxDev b, double *__restrict__ acc,
                long long *__restrict__ partials, int *__restrict__ flags, int nblk_real) {
   static_assert(!(MULTI && ISO), "species tables are in length units");
@@ -283,6 +289,7 @@ __global__ void __launch_bounds__(kBlock)
     const PairF32 c0 = pair_f32(lj, 0, ISO ? u0 : 1.0);
     const int cp = (nnbr[i] + 3) & ~3;
     const int id_i = (ENERGY && HALO) ? id[i] : 0;
+    const int mol_i = (MULTI && mol) ? mol[tag[i]] : 0;  // as in k_lj_f64
     for (int k = 0; k < cp; k += 4) {
       const u32x4 g = load_group(nbr, (size_t)stride, i, k);
       const uint32_t e[4] = {g.x, g.y, g.z, g.w};
@@ -293,11 +300,13 @@ __global__ void __launch_bounds__(kBlock)
       for (int v = 0; v < 4; ++v) {
         const int j = (int)(e[v] & kIndexMask);
         const bool once = ENERGY && (HALO && j >= n ? (id_i < id[j]) : (i < j));
-        if (MULTI)
-          pair_f32_entry<false, ENERGY, true>(qi, qj[v], u0, u1, u2,
-                                              pair_f32_table(lj, ti * nt + (qj[v].w & 7)), once, a);
-        else
+        if (MULTI) {
+          if (!(mol && mol[tag[j]] == mol_i))
+            pair_f32_entry<false, ENERGY, true>(qi, qj[v], u0, u1, u2,
+                                                pair_f32_table(lj, ti * nt + (qj[v].w & 7)), once, a);
+        } else {
           pair_f32_entry<ISO, ENERGY>(qi, qj[v], u0, u1, u2, c0, once, a);
+        }
       }
     }
     // fc * dx and fc' * dq are both the force component times 2^32
@@ -817,8 +826,8 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
   const int grid = ((real + 7) / 8) * 8;
   if (precision == 0) {
     k_lj_f64<MULTI, ENERGY, HALO><<<grid, kBlock, 0, st>>>(s.pos, s.nbr, s.nnbr, s.stride, s.n,
-                                                           s.id, lj_dev, b, s.acc, s.partials,
-                                                           s.flags, real);
+                                                           s.id, s.tag, s.mol, lj_dev, b, s.acc,
+                                                           s.partials, s.flags, real);
     if (ENERGY) k_reduce_partials<<<1, 1024, 0, st>>>(s.partials, real, 2, s.reduce_out);
   } else {
     long long *part = reinterpret_cast<long long *>(s.partials);
@@ -848,10 +857,12 @@ void lj_dispatch(DevState &s, const BoxDev &b, const LJDev *lj_dev, int precisio
     }
     if (!MULTI && b.qiso)
       k_lj_f32fx<false, ENERGY, HALO, true><<<grid, kBlock, 0, st>>>(
-          s.q, s.nbr, s.nnbr, s.stride, s.n, s.id, lj_dev, b, s.acc, part, s.flags, real);
+          s.q, s.nbr, s.nnbr, s.stride, s.n, s.id, s.tag, s.mol, lj_dev, b, s.acc, part, s.flags,
+          real);
     else
       k_lj_f32fx<MULTI, ENERGY, HALO, false><<<grid, kBlock, 0, st>>>(
-          s.q, s.nbr, s.nnbr, s.stride, s.n, s.id, lj_dev, b, s.acc, part, s.flags, real);
+          s.q, s.nbr, s.nnbr, s.stride, s.n, s.id, s.tag, s.mol, lj_dev, b, s.acc, part, s.flags,
+          real);
     if (ENERGY) k_reduce_i64<<<1, 1024, 0, st>>>(part, real, s.fx_acc);
   }
 }
@@ -926,8 +937,8 @@ bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, cons
 void launch_forces(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                    int precision, bool energy, hipStream_t st) {
   if (s.n == 0) return;
-  // jump_fix runs on the species-table kernels
-  const bool multi = lj.nt > 1 || lj.jump_tol > 0.0;
+  // jump_fix and ignore_intra_molecule run on the species-table kernels
+  const bool multi = lj.nt > 1 || lj.jump_tol > 0.0 || s.mol != nullptr;
   const bool halo = s.ng > 0;
 #define CV2_LJ(M, E, H)                                      \
   if (multi == M && energy == E && halo == H) {              \

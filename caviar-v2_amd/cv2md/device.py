@@ -83,6 +83,7 @@ def lib():
             "cv2_set_precision": ([vp, C.c_int], C.c_int),
             "cv2_upload_atoms": ([vp, C.c_int64, _d, _d, _i32, _i64], C.c_int),
             "cv2_set_ghosts": ([vp, C.c_int64, _d, _i32, _i64], C.c_int),
+            "cv2_set_molecules": ([vp, C.c_int64, _i32, C.c_int], C.c_int),
             "cv2_halo_pack": ([vp, C.POINTER(HaloGeom), _i64, C.c_int64, _d, _i64], C.c_int),
             "cv2_halo_unpack": ([vp, C.c_int32, _i64, _d, C.c_int32], C.c_int),
             "cv2_halo_unpack_at": ([vp, C.c_int32, _i64, _i64, _d, C.c_int32], C.c_int),
@@ -197,6 +198,12 @@ class Context:
         i = np.ascontiguousarray(id_, np.int64) if id_ is not None else None
         self.n = len(p)
         self._chk(self.L.cv2_upload_atoms(self.h, self.n, _p(p, _d), _p(v, _d), _p(t, _i32), _p(i, _i64)))
+
+    def set_molecules(self, mol, ignore=True):
+        """Owned molecule indices (upload order) and Lj::ignore_intra_molecule."""
+        m = np.ascontiguousarray(mol, np.int32) if mol is not None else None
+        n = len(m) if m is not None else self.n
+        self._chk(self.L.cv2_set_molecules(self.h, n, _p(m, _i32), int(ignore)))
 
     def set_ghosts(self, pos, type_=None, id_=None):
         p = _f64(pos)

@@ -39,6 +39,7 @@ class Lj : public Force_field {
               std::vector<double> &cutoff_list, bool &has_cutoff_list) const;
   // The jump_fix clamp of r^2 the device applies (0 when jump_fix is off).
   double device_jump_tol() const { return jump_fix ? jump_tol : 0.0; }
+  bool ignores_intra_molecule() const { return ignore_intra_molecule; }
 
  private:
   std::vector<std::vector<double>> epsilon, sigma, cutoff_list, lambda_s, lambda_e;

@@ -136,6 +136,12 @@ void upload(DeviceSession &s) {
   }
   s.check(cv2_upload_atoms(s.ctx, (int64_t)n, &o.position[0].x, &o.velocity[0].x, type.data(), id.data()),
           "cv2_upload_atoms");
+  // lj.cpp:446 compares the owned molecule indices (ghosts: their owners')
+  if (find_lj(s.cv)->ignores_intra_molecule()) {
+    std::vector<int32_t> mol(n, -1);
+    for (size_t i = 0; i < n && i < o.molecule_index.size(); ++i) mol[i] = (int32_t)o.molecule_index[i];
+    s.check(cv2_set_molecules(s.ctx, (int64_t)n, mol.data(), 1), "cv2_set_molecules");
+  }
   s.device_current = true;
 }
 
@@ -414,10 +420,7 @@ void Lj::set_jump_tol(double v) {
   jump_fix = true;
   jump_tol = v;
 }
-void Lj::ignore_intra_molecule_flag(bool f) {
-  ignore_intra_molecule = f;
-  if (f) caviar_->log.warning("Lj: molecules are not modelled on the device path; flag ignored");
-}
+void Lj::ignore_intra_molecule_flag(bool f) { ignore_intra_molecule = f; }
 void Lj::enable_wca(bool f) {
   wca = f;
   if (f) {

@@ -110,6 +110,21 @@ int cv2_set_precision(cv2_ctx *ctx, int precision);
 int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *vel,
                      const int32_t *type, const int64_t *id);
 
+/* Molecule indices and force_field::Lj::ignore_intra_molecule_flag
+ * (lj.hpp; applied at lj.cpp:446).  molecule_index holds n = owned-atom
+ * entries in upload order (Atom_struct::molecule_index, -1 outside
+ * molecules, atom_data.cpp:625; NULL = all -1).  With the flag on, a listed
+ * pair whose two molecule indices are equal adds no force, virial or
+ * energy; as at lj.cpp:446 this includes two atoms that are both outside
+ * molecules (-1 == -1).  Periodic images carry their owner's index (the
+ * single-domain ghost copy, exchange_ghost_single_md_domain.cpp:54-60).
+ * The pair set is unchanged.  The flag runs on the species-table gather
+ * kernels (32-bit lists); a host-halo share refuses it, because its
+ * ghosts' indices do not travel in the halo packets.  Cleared by
+ * cv2_upload_atoms; flag 0 turns it off. */
+int cv2_set_molecules(cv2_ctx *ctx, int64_t n, const int32_t *molecule_index,
+                      int ignore_intra_molecule);
+
 /* Host-supplied ghost halo for one domain's share (the images that
  * exchange_ghost_mpi_shared_atoms.cpp:274-311 would deliver).  Switches the
  * context to host-halo mode: the box is then treated as open and only these

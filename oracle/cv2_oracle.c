@@ -360,6 +360,8 @@ void orc_lj(int64_t n, const double *pos, const int32_t *type, const int64_t *id
         tj = type[j];
         id_j = (int)id[j];
       }
+      /* :446: equal molecule indices (-1 == -1 included) add nothing */
+      if (p->ignore_intra_molecule && p->mol[i] == (is_ghost ? p->gmol[j] : p->mol[j])) continue;
       const double drx = pj[0] - pi[0], dry = pj[1] - pi[1], drz = pj[2] - pi[2]; /* :449 */
       double dr_sq = dot3(drx, dry, drz);                                       /* :451 */
       if (p->cutoff_list) {                                                      /* :453-454 */
@@ -462,6 +464,11 @@ struct orc_sim {
   double virial, pe, ke;
   /* Langevin coefficients, md_simulator.cpp:398-405 */
   double la, lb, lc;
+  /* Lj::ignore_intra_molecule with the owned molecule indices; the ghosts
+   * carry their owners' (exchange_ghost_single_md_domain.cpp:54-60) */
+  int ignore_mol;
+  int32_t *mol, *gmol;
+  int64_t gmol_cap;
 };
 
 static void sim_ghosts(orc_sim *s) {
@@ -525,7 +532,18 @@ static int sim_update_is_needed(const orc_sim *s) {
 
 static void sim_forces(orc_sim *s) {
   memset(s->acc, 0, sizeof(double) * 3 * (size_t)s->n); /* reset_owned_acceleration */
-  orc_lj_params lj = {s->p.ntypes, s->eps, s->sigma, s->cutoff_list, s->p.lj_cutoff, s->mass_inv, 0.0};
+  orc_lj_params lj = {s->p.ntypes, s->eps, s->sigma, s->cutoff_list, s->p.lj_cutoff, s->mass_inv, 0.0, 0, NULL, NULL};
+  if (s->ignore_mol) {
+    if (s->ng > s->gmol_cap) {
+      free(s->gmol);
+      s->gmol_cap = s->ng;
+      s->gmol = (int32_t *)malloc(sizeof(int32_t) * (size_t)s->gmol_cap);
+    }
+    for (int64_t g = 0; g < s->ng; ++g) s->gmol[g] = s->mol[s->gowner[g]];
+    lj.ignore_intra_molecule = 1;
+    lj.mol = s->mol;
+    lj.gmol = s->gmol;
+  }
   s->pe = 0.0;
   orc_lj(s->n, s->pos, s->type, s->id, s->ng, s->gpos, s->gtype, s->gid, s->offs, s->nbr, &lj,
          s->acc, &s->virial, &s->pe);
@@ -604,8 +622,15 @@ void orc_sim_destroy(orc_sim *s) {
   free(s->pos); free(s->vel); free(s->acc); free(s->acc_old); free(s->pos_old_nl);
   free(s->gpos_old_nl); free(s->type); free(s->id); free(s->eps); free(s->sigma);
   free(s->cutoff_list); free(s->mass); free(s->mass_inv); free(s->offs); free(s->nbr);
-  free(s->gpos); free(s->gowner); free(s->gtype); free(s->gid);
+  free(s->gpos); free(s->gowner); free(s->gtype); free(s->gid); free(s->mol); free(s->gmol);
   free(s);
+}
+
+void orc_sim_set_molecules(orc_sim *s, const int32_t *mol, int ignore) {
+  free(s->mol);
+  s->mol = (int32_t *)malloc(sizeof(int32_t) * (size_t)(s->n > 0 ? s->n : 1));
+  for (int64_t i = 0; i < s->n; ++i) s->mol[i] = mol ? mol[i] : -1;
+  s->ignore_mol = ignore;
 }
 
 static void sim_ke(orc_sim *s) {

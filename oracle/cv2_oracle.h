@@ -35,6 +35,10 @@ typedef struct {
   double cutoff;             /* Lj::cutoff */
   const double *mass_inv;    /* ntypes */
   double jump_tol;           /* <= 0: jump_fix disabled */
+  int ignore_intra_molecule; /* Lj::ignore_intra_molecule (lj.cpp:446) */
+  const int32_t *mol;        /* owned molecule indices (-1 outside molecules); used with the flag */
+  const int32_t *gmol;       /* ghost molecule indices (copies of the owners', */
+                             /* exchange_ghost_single_md_domain.cpp:54-60) */
 } orc_lj_params;
 
 /* domain.cpp:443-493 via exchange_owned_single_md_domain.cpp:29-57.
@@ -112,6 +116,9 @@ typedef struct {
 orc_sim *orc_sim_create(const orc_sim_params *p, int64_t n, const double *pos,
                         const double *vel, const int32_t *type, const int64_t *id);
 void orc_sim_destroy(orc_sim *s);
+/* Atom_struct::molecule_index of every atom (NULL: all -1, atom_data.cpp:625)
+ * and Lj::ignore_intra_molecule_flag (lj.cpp:93-96, applied at :446). */
+void orc_sim_set_molecules(orc_sim *s, const int32_t *mol, int ignore);
 /* Md_simulator::setup(), md_simulator.cpp:565-611. */
 void orc_sim_setup(orc_sim *s);
 /* One step; eta = 3n Gaussian numbers (AoS) for Langevin, ignored for VV.

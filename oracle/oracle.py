@@ -34,7 +34,8 @@ class Box(C.Structure):
 
 class LjParams(C.Structure):
     _fields_ = [("ntypes", C.c_int), ("eps", _d), ("sigma", _d), ("cutoff_list", _d),
-                ("cutoff", C.c_double), ("mass_inv", _d), ("jump_tol", C.c_double)]
+                ("cutoff", C.c_double), ("mass_inv", _d), ("jump_tol", C.c_double),
+                ("ignore_intra_molecule", C.c_int), ("mol", _i32), ("gmol", _i32)]
 
 
 class SimParams(C.Structure):
@@ -106,6 +107,7 @@ def oracle_lib():
         L.orc_nve_lambda.argtypes = [C.c_double, C.c_int] + [C.c_double] * 5
         L.orc_nve_lambda.restype = C.c_double
         L.orc_sim_scale_velocities.argtypes = [_vp, C.c_double]
+        L.orc_sim_set_molecules.argtypes = [_vp, _i32, C.c_int]
         _orc = L
     return _orc
 
@@ -178,8 +180,9 @@ def nlist_cell(pos, gpos, box, cutoff, coef):
 
 
 def lj(pos, type_, id_, gpos, gtype, gid, offs, nbr, eps, sigma, cutoff, mass,
-       cutoff_list=None, jump_tol=0.0):
-    """Reference LJ pass: returns (acc[N,3], virial, pe)."""
+       cutoff_list=None, jump_tol=0.0, mol=None, gmol=None):
+    """Reference LJ pass: returns (acc[N,3], virial, pe).  With mol (owned
+    molecule indices) and gmol (the ghosts'), Lj::ignore_intra_molecule is on."""
     p = _f64(pos); n = len(p)
     t = np.ascontiguousarray(type_, np.int32); i = np.ascontiguousarray(id_, np.int64)
     g = _f64(gpos) if gpos is not None and len(gpos) else np.zeros((0, 3))
@@ -188,7 +191,10 @@ def lj(pos, type_, id_, gpos, gtype, gid, offs, nbr, eps, sigma, cutoff, mass,
     eps = _f64(np.atleast_2d(eps)); sig = _f64(np.atleast_2d(sigma))
     cl = _f64(np.atleast_2d(cutoff_list)) if cutoff_list is not None else None
     minv = _f64(1.0 / np.atleast_1d(np.asarray(mass, np.float64)))
-    prm = LjParams(eps.shape[0], _p(eps, _d), _p(sig, _d), _p(cl, _d), cutoff, _p(minv, _d), jump_tol)
+    m = np.ascontiguousarray(mol, np.int32) if mol is not None else None
+    gm = np.ascontiguousarray(gmol if gmol is not None else np.zeros(len(g)), np.int32)
+    prm = LjParams(eps.shape[0], _p(eps, _d), _p(sig, _d), _p(cl, _d), cutoff, _p(minv, _d), jump_tol,
+                   1 if m is not None else 0, _p(m, _i32), _p(gm, _i32))
     acc = np.zeros((n, 3)); vir = np.zeros(1); pe = np.zeros(1)
     oracle_lib().orc_lj(n, _p(p, _d), _p(t, _i32), _p(i, _i64), len(g), _p(g, _d), _p(gt, _i32),
                         _p(gi, _i64), _p(np.ascontiguousarray(offs, np.int64), _i64),
@@ -255,6 +261,10 @@ class OracleSim(_SimBase):
     def setup(self):
         self.L.orc_sim_setup(self.h)
 
+    def set_molecules(self, mol, ignore=True):
+        m = np.ascontiguousarray(mol, np.int32)
+        self.L.orc_sim_set_molecules(self.h, _p(m, _i32), int(ignore))
+
     def step(self, eta=None):
         e = _f64(eta) if eta is not None else None
         return bool(self.L.orc_sim_step(self.h, _p(e, _d)))
@@ -312,6 +322,7 @@ def ref_lib():
         L.ref_sim_nghost.argtypes = [_vp]; L.ref_sim_nghost.restype = C.c_int64
         L.ref_sim_ghosts.argtypes = [_vp, _d, _i64]
         L.ref_sim_nlist.argtypes = [_vp, C.c_int64, _i64, _u32]; L.ref_sim_nlist.restype = C.c_int64
+        L.ref_sim_set_molecules.argtypes = [_vp, _i32, C.c_int]
         L.ref_random_velocity.argtypes = [C.c_int64, C.c_uint, C.c_double, _d]
         _ref = L
     return _ref
@@ -341,6 +352,10 @@ class RefSim(_SimBase):
 
     def setup(self):
         self.L.ref_sim_setup(self.h)
+
+    def set_molecules(self, mol, ignore=True):
+        m = np.ascontiguousarray(mol, np.int32)
+        self.L.ref_sim_set_molecules(self.h, _p(m, _i32), int(ignore))
 
     def step(self):
         return bool(self.L.ref_sim_step(self.h))

@@ -124,6 +124,25 @@ void *ref_sim_create(const double lo[3], const double hi[3], const int bc[3], do
   return s;
 }
 
+// Atom_struct::molecule_index of every atom (add_atom sets -1,
+// atom_data.cpp:625) and Lj::ignore_intra_molecule_flag (lj.cpp:93-96).
+// The molecules themselves (molecule_struct_owned, read by
+// set_atoms_mpi_rank, atom_data.cpp:817-838) list their atoms' ids.
+void ref_sim_set_molecules(void *h, const int32_t *mol, int ignore) {
+  RefSim *s = static_cast<RefSim *>(h);
+  caviar2::Atom_data &ad = s->cv->atom_data;
+  auto &mi = ad.atom_struct_owned.molecule_index;
+  int nmol = 0;
+  for (size_t i = 0; i < mi.size(); ++i) {
+    mi[i] = mol[i];
+    if (mol[i] + 1 > nmol) nmol = mol[i] + 1;
+  }
+  ad.molecule_struct_owned.resize((size_t)nmol);
+  for (size_t i = 0; i < mi.size(); ++i)
+    if (mol[i] > -1) ad.molecule_struct_owned[(size_t)mol[i]].atom_list.push_back((int)ad.atom_struct_owned.id[i]);
+  s->lj->ignore_intra_molecule_flag(ignore != 0);
+}
+
 void ref_sim_destroy(void *h) {
   RefSim *s = static_cast<RefSim *>(h);
   delete s->lj;

@@ -5,7 +5,9 @@ Runs only where the reference tree exists (it drives
 fixture is an ``.npz`` of inputs and the reference's outputs; the manifest
 ``golden.json`` records a SHA-256 per file.  Regenerate with
 
-    make -C oracle oracle ref && python tests/golden/make_golden.py
+    make -C oracle oracle ref && python tests/golden/make_golden.py [fixture ...]
+
+(naming fixtures regenerates only those).
 """
 from __future__ import annotations
 
@@ -53,12 +55,16 @@ def setup_fixture(name, s, use_cell_list=False):
     return name, d
 
 
-def traj_fixture(name, s, nsteps, keep=(1, 10, 100), use_cell_list=False, with_eta=False):
+def traj_fixture(name, s, nsteps, keep=(1, 10, 100), use_cell_list=False, with_eta=False, mol=None):
     s = dict(s, use_cell_list=int(use_cell_list))
     d = sys_arrays(s)
     d["in_use_cell_list"] = np.asarray(int(use_cell_list))
+    if mol is not None:  # Lj::ignore_intra_molecule on, with these molecule indices
+        d["in_mol"] = np.asarray(mol, np.int32)
     rebuilt, kes, etas = [], [], []
     with O.RefSim(s) as r:
+        if mol is not None:
+            r.set_molecules(mol, True)
         r.setup()
         for k in range(1, nsteps + 1):
             rebuilt.append(r.step())
@@ -94,8 +100,30 @@ def grid512():
     return s
 
 
-def main():
+def molecules(n):
+    """Molecule indices: runs of 4 consecutive atoms share one index, every
+    fifth run stays outside molecules (-1)."""
+    m = np.arange(n) // 4
+    return np.where(m % 5 == 4, -1, m).astype(np.int32)
+
+
+def main(only=None):
     O.ref_lib()
+    if only:  # regenerate the named fixtures, keep the others and their hashes
+        gen = {"mol_traj": lambda: traj_fixture("mol_traj", lj_fluid(5, temperature=1.0, seed=1006), 20,
+                                                keep=(1, 10, 20), use_cell_list=True,
+                                                mol=molecules(500))}
+        mpath = os.path.join(HERE, "golden.json")
+        meta = json.load(open(mpath))
+        for name in only:
+            name_, d = gen[name]()
+            path = os.path.join(HERE, name_ + ".npz")
+            np.savez_compressed(path, **d)
+            meta["files"][name_ + ".npz"] = hashlib.sha256(open(path, "rb").read()).hexdigest()
+            print(f"{name_}: {os.path.getsize(path) / 1024:.0f} KiB")
+        with open(mpath, "w") as f:
+            json.dump(meta, f, indent=1)
+        return
     fixtures = []
     c1 = lj_fluid(10, rho=0.8442, temperature=1.44, seed=1001)
     fixtures.append(setup_fixture("c1_setup_brute", c1, use_cell_list=False))
@@ -108,6 +136,8 @@ def main():
                                                            integrator=1, gamma=1.0),
                                  20, keep=(1, 10, 20), with_eta=True))
     fixtures.append(traj_fixture("grid512_traj", grid512(), 50, keep=(1, 50)))
+    fixtures.append(traj_fixture("mol_traj", lj_fluid(5, temperature=1.0, seed=1006), 20, keep=(1, 10, 20),
+                                 use_cell_list=True, mol=molecules(500)))
     fixtures.append(("random_velocity", dict(n=np.asarray(512), seed=np.asarray(1),
                                              amplitude=np.asarray(0.5),
                                              vel=O.ref_random_velocity(512, 1, 0.5))))
@@ -124,4 +154,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -67,9 +67,11 @@ def oracle_forces(pos, s):
     gpos, owner, _ = O.ghosts(p, box, s["ghost_cutoff"])
     offs, nbr = O.nlist_cell(p, gpos, box, s["nl_cutoff"], s["nl_coef"])
     t = np.asarray(s["type"], np.int32)
+    mol = s.get("mol")  # ignore_intra_molecule on: ghosts carry their owners' indices
     acc, vir, pe = O.lj(p, t, np.arange(len(p)), gpos, t[owner], owner, offs, nbr, s["eps"],
                         s["sigma"], s["lj_cutoff"], s["mass"], s.get("cutoff_list"),
-                        s.get("jump_tol", 0.0))
+                        s.get("jump_tol", 0.0), mol=mol,
+                        gmol=None if mol is None else np.asarray(mol)[owner])
     return p, acc, vir, pe
 
 

@@ -46,12 +46,14 @@ def test_setup_matches_reference(golden, name):
 
 
 @pytest.mark.parametrize("name", ["c1_traj_brute", "c1_traj_cell", "dimer_traj", "grid512_traj",
-                                  "langevin_traj"])
+                                  "langevin_traj", "mol_traj"])
 def test_trajectory_matches_reference(golden, name):
     d = golden(name)
     s = sysdef_from(d)
     keep = set(int(k) for k in d["keep"])
     with O.OracleSim(s) as o:
+        if "mol" in s:  # Lj::ignore_intra_molecule on (mol_traj)
+            o.set_molecules(s["mol"])
         o.setup()
         for k in range(1, int(d["nsteps"]) + 1):
             eta = d["eta"][k - 1] if "eta" in d else None
@@ -287,3 +289,38 @@ def test_nve_rescale_reaches_target(mode, n_r_df):
     assert ke1 == pytest.approx(target, rel=1e-13)
     assert np.allclose(v, lam * s["vel"], rtol=0, atol=1e-15 * np.abs(s["vel"]).max())
     assert O.nve_lambda(0.0, dof, kbt=0.9) == 1.0  # the ignored step
+
+
+def test_ignore_intra_molecule_drops_exactly_the_same_molecule_pairs():
+    """lj.cpp:446: with the flag on, a pair adds nothing when its two
+    molecule indices are equal (two atoms outside molecules, -1 and -1,
+    included).  Pinned against the unflagged force over the list with those
+    pairs removed (a different route to the same sum), and by the two
+    extremes: one molecule per atom changes nothing, all atoms outside
+    molecules leaves no force at all."""
+    s = lj_fluid(4, temperature=1.0, seed=7)
+    box = _box(s)
+    pos, _ = O.wrap(s["pos"] + np.random.default_rng(7).normal(0, 0.05, s["pos"].shape), box)
+    gpos, owner, _ = O.ghosts(pos, box, s["ghost_cutoff"])
+    offs, nbr = O.nlist_cell(pos, gpos, box, s["nl_cutoff"], s["nl_coef"])
+    n = len(pos)
+    ids = np.arange(n)
+    args = (pos, s["type"], ids, gpos, s["type"][owner], owner)
+    prm = (s["eps"], s["sigma"], s["lj_cutoff"], s["mass"])
+    mol = (np.arange(n) // 4).astype(np.int32)
+    mol[(np.arange(n) // 4) % 5 == 4] = -1
+    gmol = mol[owner]
+    acc_m, vir_m, pe_m = O.lj(*args, offs, nbr, *prm, mol=mol, gmol=gmol)
+    # the same list without the same-molecule pairs, flag off
+    jm = np.where(nbr >= n, gmol[np.minimum(nbr.astype(np.int64) - n, len(gmol) - 1)], mol[np.minimum(nbr, n - 1)])
+    im = np.repeat(mol, np.diff(offs))
+    keep = jm != im
+    offs2 = np.concatenate([[0], np.cumsum(np.add.reduceat(keep.astype(np.int64), offs[:-1]) * (np.diff(offs) > 0))])
+    acc_f, vir_f, pe_f = O.lj(*args, offs2, nbr[keep], *prm)
+    assert keep.sum() < len(nbr)
+    assert np.array_equal(acc_m, acc_f) and vir_m == vir_f and pe_m == pe_f
+    acc_0, vir_0, pe_0 = O.lj(*args, offs, nbr, *prm)
+    acc_u, vir_u, pe_u = O.lj(*args, offs, nbr, *prm, mol=np.arange(n), gmol=owner)
+    assert np.array_equal(acc_u, acc_0) and vir_u == vir_0 and pe_u == pe_0
+    acc_n, vir_n, pe_n = O.lj(*args, offs, nbr, *prm, mol=np.full(n, -1), gmol=np.full(len(owner), -1))
+    assert not acc_n.any() and vir_n == 0.0 and pe_n == 0.0
