@@ -95,6 +95,14 @@ struct cv2_ctx {
   long long *halo_base = nullptr;
   double *halo_buf = nullptr;
   int64_t halo_n = 0, halo_buf_cap = 0;
+  // owned-atom exchange (cv2_owned_pack / cv2_owned_unpack)
+  int64_t own_cap = 0;       // owned atoms the per-atom arrays hold
+  bool ids_sorted = false;   // owned order is ascending id (merge arrivals by id)
+  int *type_by_id = nullptr; // cv2_set_type_table
+  int64_t ntype_ids = 0;
+  int *mig_moved = nullptr;  // device flag: some atom was shifted back into the domain
+  void *mig_aux = nullptr;   // received ids, sources and packet tables
+  size_t mig_aux_bytes = 0;
   double virial = 0.0, pe = 0.0;
   int *h_flags = nullptr;
   double *h_red = nullptr;
@@ -492,6 +500,77 @@ int alloc_positions(cv2_ctx *ctx, int64_t ntot) {
   return CV2_OK;
 }
 
+// Reallocate *p for count elements keeping its first keep elements.
+template <class T>
+hipError_t grow_keep(T **p, size_t keep, size_t count, hipStream_t st) {
+  T *q = nullptr;
+  hipError_t e = hipMalloc((void **)&q, sizeof(T) * std::max<size_t>(count, 1));
+  if (e != hipSuccess) return e;
+  if (*p && keep > 0) {
+    e = hipMemcpyAsync(q, *p, sizeof(T) * keep, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      (void)hipFree(q);
+      return e;
+    }
+  }
+  if (*p) (void)hipFree(*p);
+  *p = q;
+  return hipSuccess;
+}
+
+// Per-owned-atom arrays for at least n owned atoms (an owned exchange that
+// brings atoms in): grown with 1/8 slack, keeping the state of the current
+// owned atoms (positions and ids, velocities and accelerations, SoA with
+// stride ctx->n), the list stride following the capacity.
+int ensure_owned_cap(cv2_ctx *ctx, int64_t n) {
+  if (n + ctx->ng > ctx->pos_cap) {  // positions and ids: the owned part is kept
+    const int rc = alloc_positions(ctx, n + ctx->ng);
+    if (rc) return rc;
+  }
+  if (n <= ctx->own_cap) return CV2_OK;
+  DevState &s = ctx->s;
+  const int64_t cap = n + n / 8 + 1024, n0 = ctx->n;
+  HIPCHK(grow_keep(&s.vel, 3 * n0, 3 * cap, ctx->stream));
+  HIPCHK(grow_keep(&s.acc, 3 * n0, 3 * cap, ctx->stream));
+  HIPCHK(grow_keep(&s.tag, n0, cap, ctx->stream));
+  HIPCHK(dalloc(&s.pos_alt, cap));
+  HIPCHK(dalloc(&s.vel_alt, 3 * cap));
+  HIPCHK(dalloc(&s.acc_alt, 3 * cap));
+  HIPCHK(dalloc(&s.tag_alt, cap));
+  HIPCHK(dalloc(&s.id_alt, cap));
+  HIPCHK(dalloc(&s.perm, cap));
+  HIPCHK(dalloc(&s.order, cap));
+  HIPCHK(dalloc(&s.nnbr, cap));
+  s.stride = (int)((cap + 63) / 64 * 64);
+  HIPCHK(dalloc(&s.nbr, (size_t)s.cap * s.stride));
+  const int nb = (int)((cap + 255) / 256) + 8;
+  if (nb > ctx->partial_blocks) {
+    HIPCHK(dalloc(&s.partials, (size_t)nb * 8));
+    ctx->partial_blocks = nb;
+  }
+  ctx->own_cap = cap;
+  return CV2_OK;
+}
+
+// The owned set after an exchange: n atoms in owned order in the state
+// arrays.  The ghosts of the old halo go (the next ghost exchange brings the
+// new one) and the next force pass rebuilds the list.
+int adopt_owned(cv2_ctx *ctx, int64_t n) {
+  DevState &s = ctx->s;
+  launch_mig_adopt(s, (int)n, ctx->stream);
+  ctx->n = n;
+  s.n = (int)n;
+  ctx->ng = 0;
+  s.ng = 0;
+  int rc = ensure_wstat(ctx);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->need_build = true;
+  if (!ctx->step_open) ctx->forces_valid = false;
+  return CV2_OK;
+}
+
 }  // namespace
 
 namespace cv2 {
@@ -611,9 +690,7 @@ HaloGeomDev halo_geom(const cv2_halo_geom *g) {
 // Device scratch of the halo packing for n owned atoms.
 int halo_scratch(cv2_ctx *ctx, int64_t n) {
   if (n <= ctx->halo_n && ctx->halo_code) return CV2_OK;
-  for (void *p : {(void *)ctx->halo_code, (void *)ctx->halo_src, (void *)ctx->halo_tiles,
-                  (void *)ctx->halo_toff})
-    if (p) (void)hipFree(p);
+  // (dalloc releases the smaller buffers)
   const int64_t nt = halo_tiles((int)n);
   HIPCHK(dalloc(&ctx->halo_code, n));
   HIPCHK(dalloc(&ctx->halo_src, n));
@@ -688,6 +765,9 @@ int cv2_destroy(cv2_ctx *ctx) {
                   (void *)ctx->halo_toff, (void *)ctx->halo_dn, (void *)ctx->halo_base,
                   (void *)ctx->halo_buf, (void *)ctx->halo_res})
     if (p) (void)hipFree(p);
+  if (ctx->type_by_id) (void)hipFree(ctx->type_by_id);
+  if (ctx->mig_moved) (void)hipFree(ctx->mig_moved);
+  if (ctx->mig_aux) (void)hipFree(ctx->mig_aux);
   if (ctx->lj_dev) (void)hipFree(ctx->lj_dev);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
@@ -881,6 +961,10 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
   ctx->forces_valid = false;
   ctx->step_open = false;
   ctx->steps = 0;
+  ctx->own_cap = n;
+  ctx->ids_sorted = true;
+  for (int64_t i = 1; i < n; ++i)
+    if (idv[i] <= idv[i - 1]) ctx->ids_sorted = false;
   return CV2_OK;
 }
 
@@ -1087,6 +1171,178 @@ int cv2_halo_unpack_at(cv2_ctx *ctx, int32_t npackets, const int64_t *counts,
                        const int64_t *offsets, const double *packets, int32_t with_velocity) {
   if (npackets > 0 && !offsets) return fail(ctx, CV2_ERR_BAD_ARG, "missing packet offsets");
   return halo_unpack_impl(ctx, npackets, counts, offsets, packets, with_velocity);
+}
+
+namespace {
+// Bit d: direction d leads back to this domain (one process on every axis
+// the direction moves along), as Domain::all[i][j][k] == me (domain.cpp:70-94).
+unsigned self_directions(const cv2_halo_geom *g) {
+  unsigned m = 0;
+  for (int d = 0; d < 27; ++d) {
+    if (d == 13) continue;
+    const int c[3] = {d / 9, (d / 3) % 3, d % 3};
+    bool self = true;
+    for (int a = 0; a < 3; ++a)
+      if (c[a] != 1 && g->nprocs[a] != 1) self = false;
+    if (self) m |= 1u << d;
+  }
+  return m;
+}
+}  // namespace
+
+int cv2_set_type_table(cv2_ctx *ctx, int64_t nid, const int32_t *type_by_id) {
+  if (!ctx || nid < 0 || (nid > 0 && !type_by_id)) return fail(ctx, CV2_ERR_BAD_ARG, "bad type table");
+  if (!ctx->lj_set) return fail(ctx, CV2_ERR_STATE, "set the LJ types before the type table");
+  for (int64_t i = 0; i < nid; ++i)
+    if (type_by_id[i] < 0 || type_by_id[i] >= ctx->lj.nt)
+      return fail(ctx, CV2_ERR_BAD_ARG, "atom type out of range");
+  (void)hipSetDevice(ctx->device);
+  std::vector<int> t(type_by_id, type_by_id + nid);
+  HIPCHK(dalloc(&ctx->type_by_id, std::max<int64_t>(nid, 1)));
+  if (nid > 0)
+    HIPCHK(hipMemcpyAsync(ctx->type_by_id, t.data(), sizeof(int) * nid, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->ntype_ids = nid;
+  return CV2_OK;
+}
+
+int cv2_owned_pack(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[27], int64_t cap,
+                   double *out, int64_t *need) {
+  if (!ctx || !g || !counts || !need || cap < 0 || (cap > 0 && !out))
+    return fail(ctx, CV2_ERR_BAD_ARG, "bad owned pack arguments");
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload atoms before exchanging them");
+  if (ctx->s.mol)
+    return fail(ctx, CV2_ERR_STATE,
+                "atoms in molecules move with their molecule's centre (exchange_owned_mpi_shared_atoms.cpp:"
+                "208-300); the device exchange moves atoms outside molecules");
+  for (int a = 0; a < 3; ++a)
+    if (g->nprocs[a] < 1 || g->grid_index[a] < 0 || g->grid_index[a] >= g->nprocs[a] ||
+        !(g->hi_local[a] > g->lo_local[a]))
+      return fail(ctx, CV2_ERR_BAD_ARG, "bad process grid or local box");
+  (void)hipSetDevice(ctx->device);
+  const int n = (int)ctx->n;
+  int rc = halo_scratch(ctx, std::max(n, 1));
+  if (rc) return rc;
+  if (!ctx->mig_moved) HIPCHK(dalloc(&ctx->mig_moved, 1));
+  const HaloGeomDev h = halo_geom(g);
+  const unsigned selfmask = self_directions(g);
+  const int per_atom = mig_per_atom();
+  HIPCHK(hipMemsetAsync(ctx->mig_moved, 0, sizeof(int), ctx->stream));
+  launch_mig_classify(ctx->s, h, selfmask, ctx->halo_code, ctx->halo_src, ctx->halo_tiles,
+                      ctx->mig_moved, ctx->stream);
+  launch_halo_scan(halo_tiles(n), per_atom, ctx->halo_tiles, ctx->halo_toff, ctx->halo_dn,
+                   ctx->halo_base, ctx->halo_res, ctx->stream, true);
+  int moved = 0;
+  HIPCHK(hipMemcpyAsync(ctx->h_halo, ctx->halo_res, sizeof(long long) * 28, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(&moved, ctx->mig_moved, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int64_t stay = n > 0 ? ctx->h_halo[13] : 0;
+  for (int d = 0; d < 27; ++d) counts[d] = (n > 0 && d != 13) ? ctx->h_halo[d] : 0;
+  const int64_t at = n > 0 ? ctx->h_halo[27] : 0;
+  *need = at;
+  if (at > cap) return CV2_OK;  // sizing: nothing changes until the packets fit
+  if (at == 0) {
+    if (moved) {  // only shifts back into the domain: in place, the list is rebuilt
+      launch_mig_shift(ctx->s, h, selfmask, ctx->stream);
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      ctx->need_build = true;
+    }
+    return CV2_OK;
+  }
+  rc = halo_buffer(ctx, at);
+  if (rc) return rc;
+  launch_mig_write(ctx->s, h, selfmask, ctx->halo_code, ctx->halo_src, ctx->halo_toff,
+                   ctx->halo_base, ctx->halo_dn, ctx->halo_buf, ctx->stream);
+  HIPCHK(hipMemcpyAsync(out, ctx->halo_buf, sizeof(double) * at, hipMemcpyDeviceToHost, ctx->stream));
+  return adopt_owned(ctx, stay);
+}
+
+int cv2_owned_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const double *packets) {
+  if (!ctx || npackets < 0 || npackets > 64 || (npackets > 0 && !counts))
+    return fail(ctx, CV2_ERR_BAD_ARG, "bad owned unpack arguments (at most 64 packets)");
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload atoms before exchanging them");
+  const int per_atom = mig_per_atom();
+  long long pbase[64], pn[64];
+  int64_t nr = 0, at = 0;
+  for (int p = 0; p < npackets; ++p) {
+    if (counts[p] < 0) return fail(ctx, CV2_ERR_BAD_ARG, "negative packet count");
+    pbase[p] = at;
+    pn[p] = counts[p];
+    nr += counts[p];
+    at += per_atom * counts[p];
+  }
+  if (nr == 0) return CV2_OK;
+  if (!packets) return fail(ctx, CV2_ERR_BAD_ARG, "missing packets");
+  if (!ctx->type_by_id)
+    return fail(ctx, CV2_ERR_STATE, "cv2_set_type_table first: received atoms take their type by id");
+  if (ctx->s.mol) return fail(ctx, CV2_ERR_STATE, "the device exchange moves atoms outside molecules");
+  if (ctx->n + nr + ctx->ng > (int64_t)kIndexMask) return fail(ctx, CV2_ERR_BAD_ARG, "too many atoms (2^26 max)");
+  // received atoms by id (a stable sort: equal ids keep packet order)
+  std::vector<int> rid((size_t)nr);
+  std::vector<int2> src((size_t)nr);
+  {
+    std::vector<std::pair<int, int2>> e((size_t)nr);
+    int64_t k = 0;
+    for (int p = 0; p < npackets; ++p)
+      for (int64_t c = 0; c < counts[p]; ++c, ++k) {
+        const double v = packets[pbase[p] + c];
+        if (!(v >= 0.0 && v < (double)ctx->ntype_ids && v == (double)(int64_t)v))
+          return fail(ctx, CV2_ERR_BAD_ARG, "received atom id outside the type table");
+        if (packets[pbase[p] + 10 * counts[p] + c] > -1.0)
+          return fail(ctx, CV2_ERR_STATE, "the device exchange moves atoms outside molecules");
+        e[(size_t)k] = {(int)v, make_int2(p, (int)c)};
+      }
+    std::stable_sort(e.begin(), e.end(),
+                     [](const std::pair<int, int2> &a, const std::pair<int, int2> &b) { return a.first < b.first; });
+    for (size_t i = 0; i < e.size(); ++i) {
+      rid[i] = e[i].first;
+      src[i] = e[i].second;
+    }
+  }
+  (void)hipSetDevice(ctx->device);
+  const int64_t ns = ctx->n, n = ns + nr;
+  launch_mig_to_owned(ctx->s, ctx->stream);  // the merge reads the owned atoms in owned order
+  int rc = ensure_owned_cap(ctx, n);
+  if (rc) return rc;
+  rc = halo_buffer(ctx, at);
+  if (rc) return rc;
+  // ids, sources and the packet tables in one device buffer
+  const size_t b_rid = 0, b_src = (sizeof(int) * nr + 15) / 16 * 16,
+               b_pb = b_src + (sizeof(int2) * nr + 15) / 16 * 16, b_pn = b_pb + 8 * 64,
+               bytes = b_pn + 8 * 64;
+  if (bytes > ctx->mig_aux_bytes) {
+    if (ctx->mig_aux) (void)hipFree(ctx->mig_aux);
+    ctx->mig_aux = nullptr;
+    HIPCHK(hipMalloc(&ctx->mig_aux, bytes));
+    ctx->mig_aux_bytes = bytes;
+  }
+  unsigned char *aux = static_cast<unsigned char *>(ctx->mig_aux);
+  HIPCHK(hipMemcpyAsync(ctx->halo_buf, packets, sizeof(double) * at, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(aux + b_rid, rid.data(), sizeof(int) * nr, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(aux + b_src, src.data(), sizeof(int2) * nr, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(aux + b_pb, pbase, 8 * npackets, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(aux + b_pn, pn, 8 * npackets, hipMemcpyHostToDevice, ctx->stream));
+  launch_mig_merge(ctx->s, (int)ns, (int)nr, ctx->ids_sorted, reinterpret_cast<const int *>(aux + b_rid),
+                   reinterpret_cast<const int2 *>(aux + b_src), ctx->halo_buf,
+                   reinterpret_cast<const long long *>(aux + b_pb),
+                   reinterpret_cast<const long long *>(aux + b_pn), ctx->type_by_id, ctx->stream);
+  return adopt_owned(ctx, n);
+}
+
+int cv2_get_ids(cv2_ctx *ctx, int64_t *ids) {
+  if (!ctx || (ctx->n > 0 && !ids)) return fail(ctx, CV2_ERR_BAD_ARG, "null argument");
+  if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "no atoms");
+  (void)hipSetDevice(ctx->device);
+  const int64_t n = ctx->n;
+  std::vector<int> id((size_t)n), tg((size_t)n);
+  if (n > 0) {
+    HIPCHK(hipMemcpyAsync(id.data(), ctx->s.id, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(tg.data(), ctx->s.tag, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int64_t k = 0; k < n; ++k) ids[tg[k]] = id[k];
+  return CV2_OK;
 }
 
 int cv2_host_alloc(int64_t bytes, void **ptr) {

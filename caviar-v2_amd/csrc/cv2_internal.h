@@ -569,8 +569,9 @@ void launch_halo_classify(const DevState &s, const HaloGeomDev &g, int *code_by_
 // Device-side scan of the tile counts: tile offsets, per-direction totals
 // (dir_n, res[0..26]), packet bases (dir_base, doubles) and their total
 // (res[27]); per_atom = doubles per atom (5, or 8 with velocities).
+// skip13: direction 13 (the owned exchange's staying atoms) holds no packet.
 void launch_halo_scan(int ntiles, int per_atom, const int *tile_counts, int *tile_off, int *dir_n,
-                      long long *dir_base, long long *res, hipStream_t st);
+                      long long *dir_base, long long *res, hipStream_t st, bool skip13 = false);
 void launch_halo_write(const DevState &s, const HaloGeomDev &g, const int *code_by_tag,
                        const int *src_by_tag, const int *tile_off, const long long *dir_base,
                        const int *dir_n, double *out, hipStream_t st);
@@ -580,5 +581,28 @@ void launch_halo_unpack(DevState &s, const double *in, int npk, const long long 
 // cv2_set_ghosts: ghosts staged on the device, placed as k_halo_unpack does
 void launch_ghost_place(DevState &s, const double4 *gpos_in, const int *gid_in, int ng, bool remap,
                         hipStream_t st);
+
+// Owned-atom exchange (k_migrate.hip).  selfmask bit d: direction d leads
+// back to this domain.  classify: direction codes by upload index (13 =
+// staying), per-tile counts, *moved |= 1 when an atom is shifted back into
+// the domain instead of leaving; write: the leavers' packets (mig_per_atom()
+// doubles each) and the staying atoms, compacted and shifted, into the alt
+// arrays with stride dir_n[13]; adopt: the alt arrays into the state arrays
+// for n atoms in owned order; shift: the in-place shifts alone (nothing
+// leaves); merge: owned (ns, stride ns) and received atoms (nr, sorted by
+// id) into the alt arrays with stride ns + nr.
+int mig_per_atom();
+void launch_mig_classify(DevState &s, const HaloGeomDev &g, unsigned selfmask, int *code_by_tag,
+                         int *src_by_tag, int *tile_counts, int *moved, hipStream_t st);
+void launch_mig_write(DevState &s, const HaloGeomDev &g, unsigned selfmask, const int *code_by_tag,
+                      const int *src_by_tag, const int *tile_off, const long long *dir_base,
+                      const int *dir_n, double *out, hipStream_t st);
+void launch_mig_shift(DevState &s, const HaloGeomDev &g, unsigned selfmask, hipStream_t st);
+void launch_mig_adopt(DevState &s, int n, hipStream_t st);
+// the state arrays of the s.n owned atoms put in owned order (tag = index)
+void launch_mig_to_owned(DevState &s, hipStream_t st);
+void launch_mig_merge(DevState &s, int ns, int nr, bool merge, const int *rid, const int2 *src,
+                      const double *in, const long long *pbase, const long long *pn,
+                      const int *type_by_id, hipStream_t st);
 
 }  // namespace cv2

@@ -139,8 +139,10 @@ __global__ void __launch_bounds__(kHaloTile)
 // into tile_off, stores its total N_d (dir_n, res[d]), and sums the counts
 // of the directions before it for its packet base (the packets follow each
 // other in direction order; dir_base in doubles, res[27] = their total).
+// skip13: direction 13 holds no packet (the owned exchange counts its
+// staying atoms there) and is left out of the bases and the total.
 __global__ void __launch_bounds__(1024)
-    k_halo_scan(const int *__restrict__ tile_counts, int ntiles, int per_atom,
+    k_halo_scan(const int *__restrict__ tile_counts, int ntiles, int per_atom, int skip13,
                 int *__restrict__ tile_off, int *__restrict__ dir_n, long long *__restrict__ dir_base,
                 long long *__restrict__ res) {
   __shared__ int sh[1024];
@@ -164,7 +166,8 @@ __global__ void __launch_bounds__(1024)
   // atoms of the directions before d
   long long before = 0;
   for (int k = t; k < ntiles; k += 1024)
-    for (int e = 0; e < d; ++e) before += tile_counts[k * 27 + e];
+    for (int e = 0; e < d; ++e)
+      if (!(skip13 && e == 13)) before += tile_counts[k * 27 + e];
   for (int o = 32; o > 0; o >>= 1) before += __shfl_xor(before, o, 64);
   if ((t & 63) == 0) red[t >> 6] = before;
   __syncthreads();
@@ -175,6 +178,7 @@ __global__ void __launch_bounds__(1024)
     dir_base[d] = (long long)per_atom * b;
     res[d] = carry;
     if (d == 26) res[27] = (long long)per_atom * (b + carry);
+    if (skip13 && d == 13) dir_base[d] = 0;
   }
 }
 
@@ -238,8 +242,9 @@ void launch_halo_classify(const DevState &s, const HaloGeomDev &g, int *code_by_
 }
 
 void launch_halo_scan(int ntiles, int per_atom, const int *tile_counts, int *tile_off, int *dir_n,
-                      long long *dir_base, long long *res, hipStream_t st) {
-  k_halo_scan<<<27, 1024, 0, st>>>(tile_counts, ntiles, per_atom, tile_off, dir_n, dir_base, res);
+                      long long *dir_base, long long *res, hipStream_t st, bool skip13) {
+  k_halo_scan<<<27, 1024, 0, st>>>(tile_counts, ntiles, per_atom, skip13 ? 1 : 0, tile_off, dir_n,
+                                   dir_base, res);
 }
 
 void launch_halo_write(const DevState &s, const HaloGeomDev &g, const int *code_by_tag,

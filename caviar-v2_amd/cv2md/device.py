@@ -84,6 +84,10 @@ def lib():
             "cv2_upload_atoms": ([vp, C.c_int64, _d, _d, _i32, _i64], C.c_int),
             "cv2_set_ghosts": ([vp, C.c_int64, _d, _i32, _i64], C.c_int),
             "cv2_set_molecules": ([vp, C.c_int64, _i32, C.c_int], C.c_int),
+            "cv2_owned_pack": ([vp, C.POINTER(HaloGeom), _i64, C.c_int64, _d, _i64], C.c_int),
+            "cv2_owned_unpack": ([vp, C.c_int32, _i64, _d], C.c_int),
+            "cv2_set_type_table": ([vp, C.c_int64, _i32], C.c_int),
+            "cv2_get_ids": ([vp, _i64], C.c_int),
             "cv2_halo_pack": ([vp, C.POINTER(HaloGeom), _i64, C.c_int64, _d, _i64], C.c_int),
             "cv2_halo_unpack": ([vp, C.c_int32, _i64, _d, C.c_int32], C.c_int),
             "cv2_halo_unpack_at": ([vp, C.c_int32, _i64, _i64, _d, C.c_int32], C.c_int),
@@ -244,6 +248,43 @@ class Context:
         out = np.empty(max(need.value, 1))
         self._chk(fn(self.h, C.byref(g), _p(counts, _i64), need.value, _p(out, _d), C.byref(need)))
         return counts, out[:need.value]
+
+    def owned_pack(self, grid_index, nprocs, lo_local, hi_local, size_global, bc, cap=None):
+        """Owned-atom exchange, sending half: (counts[27], packets concatenated
+        in direction order).  The leavers leave this context.  cap: packet
+        doubles offered in one call (None: size first, then pack)."""
+        g = HaloGeom()
+        for a in range(3):
+            g.grid_index[a] = int(grid_index[a]); g.nprocs[a] = int(nprocs[a]); g.bc[a] = int(bc[a])
+            g.lo_local[a] = float(lo_local[a]); g.hi_local[a] = float(hi_local[a])
+            g.size_global[a] = float(size_global[a])
+        counts = np.zeros(27, np.int64)
+        need = C.c_int64()
+        if cap is None:
+            self._chk(self.L.cv2_owned_pack(self.h, C.byref(g), _p(counts, _i64), 0, None, C.byref(need)))
+            cap = need.value
+        out = np.empty(max(int(cap), 1))
+        self._chk(self.L.cv2_owned_pack(self.h, C.byref(g), _p(counts, _i64), int(cap), _p(out, _d),
+                                        C.byref(need)))
+        self.n = self.stats()["natoms"]
+        return counts, out[:need.value]
+
+    def owned_unpack(self, counts, packets):
+        """Owned-atom exchange, receiving half: the atoms of the packets join."""
+        cnt = np.ascontiguousarray(counts, np.int64)
+        pk = np.ascontiguousarray(packets, np.float64)
+        self._chk(self.L.cv2_owned_unpack(self.h, len(cnt), _p(cnt, _i64), _p(pk, _d) if pk.size else None))
+        self.n = self.stats()["natoms"]
+
+    def set_type_table(self, type_by_id):
+        t = np.ascontiguousarray(type_by_id, np.int32)
+        self._chk(self.L.cv2_set_type_table(self.h, len(t), _p(t, _i32)))
+
+    def ids(self):
+        """Ids of the owned atoms in owned order (the order of download())."""
+        out = np.empty(max(self.n, 1), np.int64)
+        self._chk(self.L.cv2_get_ids(self.h, _p(out, _i64)))
+        return out[:self.n]
 
     def halo_pack_into(self, out, grid_index, nprocs, lo_local, hi_local, size_global, bc, ghost_cutoff,
                        with_velocity=False):

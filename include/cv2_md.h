@@ -202,6 +202,53 @@ int cv2_halo_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const
 int cv2_halo_unpack_at(cv2_ctx *ctx, int32_t npackets, const int64_t *counts,
                        const int64_t *offsets, const double *packets, int32_t with_velocity);
 
+/* Owned-atom exchange: the device side of
+ * Atom_data::exchange_owned_mpi_shared_atoms (exchange_owned_mpi_shared_atoms.cpp:44-602),
+ * called where the reference calls it, between the drift and the ghost
+ * exchange of a step (Md_simulator::boundary_condition, md_simulator.cpp:547-557),
+ * with the geometry of cv2_halo_pack (ghost_cutoff and with_velocity unused).
+ *
+ * cv2_owned_pack: every owned atom strictly outside [lo_local, hi_local]
+ * gets the direction d = i*9 + j*3 + k of its three components (-1 below,
+ * +1 above, zeroed at a non-periodic grid edge, :136-190).  Where direction
+ * d leads back to this domain (one process on every axis it moves along)
+ * the atom stays and is shifted by the local box length (:571-583);
+ * otherwise it leaves this domain in direction d's packet:
+ * [N ids][3N positions][3N velocities][3N accelerations][N molecule
+ * indices][N bond counts] as doubles (:330-369, none of the optional
+ * blocks), atoms in owned order, positions shifted by size_global across a
+ * periodic grid edge (:402-431).  counts[27] receives the atoms per
+ * direction (counts[13] = 0) and *need the doubles of the packets,
+ * concatenated in direction order.  The state changes only when the
+ * packets fit in cap (out written): the leavers go, the others keep their
+ * owned order, the host-supplied ghosts are dropped (the ghost exchange
+ * that follows brings the new halo) and the list is rebuilt at the next
+ * force pass.  A call with cap 0 sizes the packets and changes nothing.
+ * Atoms in molecules, which the reference moves with their molecule's
+ * centre (:208-300), are refused (cv2_set_molecules). */
+int cv2_owned_pack(cv2_ctx *ctx, const cv2_halo_geom *g, int64_t counts[27], int64_t cap,
+                   double *out, int64_t *need);
+
+/* The receiving half (:534-569): the atoms of npackets received packets
+ * (packet p with counts[p] atoms, back to back in the layout above) join
+ * the owned atoms with their positions, velocities and accelerations; each
+ * takes its type from the table of cv2_set_type_table by its id, as every
+ * rank of the reference holds every atom's type.  The owned order stays
+ * ascending by id when it was (atoms uploaded in ascending id order, the
+ * reference's index order when ids are the indices,
+ * add_unique_atom_molecule.cpp:31); otherwise the arrivals follow the
+ * owned atoms in ascending id order.  The list is rebuilt at the next force
+ * pass. */
+int cv2_owned_unpack(cv2_ctx *ctx, int32_t npackets, const int64_t *counts, const double *packets);
+
+/* Type of every atom id (nid entries): Atom_struct::type of the reference's
+ * shared atoms, which every rank holds; read by cv2_owned_unpack. */
+int cv2_set_type_table(cv2_ctx *ctx, int64_t nid, const int32_t *type_by_id);
+
+/* Ids of the owned atoms in owned order (the order of cv2_download), which
+ * owned exchanges change; ids[n]. */
+int cv2_get_ids(cv2_ctx *ctx, int64_t *ids);
+
 /* Page-locked host memory for packet buffers (hipHostMalloc): device copies
  * from and to it run at the link's full rate.  No reference counterpart
  * (the reference's buffers are std::vector).  *ptr = NULL for bytes = 0. */

@@ -137,6 +137,84 @@ int64_t orc_halo_pack(int64_t n, const double *pos, const double *vel, const int
   return need;
 }
 
+int64_t orc_owned_pack(int64_t n, const double *pos, const double *vel, const double *acc,
+                       const int64_t *id, const int32_t gi[3], const int32_t np[3],
+                       const double lo[3], const double hi[3], const double size_global[3],
+                       const int32_t bc[3], int64_t counts[27], int64_t cap, double *out,
+                       int32_t *stay, double *pos_after) {
+  /* :131-206: one direction per atom outside the domain; me == all[i][j][k]
+   * exactly when every axis the direction moves along has one process */
+  int32_t *dir = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int self[27];
+  for (int d = 0; d < 27; ++d) {
+    const int c[3] = {d / 9, (d / 3) % 3, d % 3};
+    self[d] = 1;
+    for (int a = 0; a < 3; ++a)
+      if (c[a] != 1 && np[a] != 1) self[d] = 0;
+    counts[d] = 0;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    int v[3];
+    for (int a = 0; a < 3; ++a) {
+      const double x = pos[3 * i + a];
+      v[a] = 0;
+      if (x < lo[a]) {
+        v[a] = -1;
+        if (gi[a] == 0) v[a] *= bc[a];
+      } else if (x > hi[a]) {
+        v[a] = +1;
+        if (gi[a] == np[a] - 1) v[a] *= bc[a];
+      }
+    }
+    const int d = (v[0] + 1) * 9 + (v[1] + 1) * 3 + (v[2] + 1);
+    dir[i] = d;
+    if (d != 13 && !self[d]) counts[d]++; /* :199-204 */
+  }
+  /* :308-333 layout: id, position, velocity, acceleration, molecule index,
+   * bond count; packets follow each other in direction order */
+  const int64_t total = 12;
+  int64_t need = 0, base[27];
+  for (int d = 0; d < 27; ++d) {
+    base[d] = need;
+    need += total * counts[d];
+  }
+  if (need <= cap) {
+    int64_t fill[27] = {0};
+    for (int64_t i = 0; i < n; ++i) {
+      const int d = dir[i];
+      const double *p = pos + 3 * i;
+      if (stay) stay[i] = d == 13 || self[d];
+      if (d == 13 || self[d]) { /* :571-583: back into this domain */
+        if (pos_after) {
+          const int c[3] = {d / 9 - 1, (d / 3) % 3 - 1, d % 3 - 1};
+          for (int a = 0; a < 3; ++a) pos_after[3 * i + a] = p[a] - c[a] * (hi[a] - lo[a]);
+        }
+        continue;
+      }
+      const int c[3] = {d / 9, (d / 3) % 3, d % 3};
+      const int64_t N = counts[d], k = fill[d]++;
+      double *pk = out + base[d];
+      pk[k] = (double)id[i];
+      for (int a = 0; a < 3; ++a) {
+        double x = p[a];
+        if (bc[a] == 1) { /* :402-431 */
+          if (gi[a] == 0 && c[a] == 0) x += size_global[a];
+          if (gi[a] == np[a] - 1 && c[a] == 2) x -= size_global[a];
+        }
+        pk[N + 3 * k + a] = x;
+        pk[4 * N + 3 * k + a] = vel[3 * i + a];
+        pk[7 * N + 3 * k + a] = acc[3 * i + a];
+      }
+      pk[10 * N + k] = -1.0; /* molecule index (atoms outside molecules) */
+      pk[11 * N + k] = 0.0;  /* atomic bond count */
+      if (pos_after)
+        for (int a = 0; a < 3; ++a) pos_after[3 * i + a] = p[a];
+    }
+  }
+  free(dir);
+  return need;
+}
+
 int64_t orc_ghosts(int64_t n, const double *pos, const orc_box *b, double g, int64_t cap,
                    double *gpos, int64_t *owner, int32_t *code) {
   const double llow[3] = {b->lo[0] + g, b->lo[1] + g, b->lo[2] + g};

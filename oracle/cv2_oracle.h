@@ -67,6 +67,26 @@ int64_t orc_halo_pack(int64_t n, const double *pos, const double *vel, const int
                       const double size_global[3], const int32_t bc[3], double ghost_cutoff,
                       int32_t with_vel, int64_t counts[27], int64_t cap, double *out);
 
+/* The owned-atom exchange of exchange_owned_mpi_shared_atoms.cpp for one
+ * domain's owned atoms in index order, all outside molecules: an atom
+ * strictly below lo_local on an axis gets -1 there, strictly above hi_local
+ * +1, zeroed at a non-periodic grid edge (:131-190); direction d = i*9 +
+ * j*3 + k.  Where d leads back to the domain (one process on every axis d
+ * moves along: me == all[i][j][k]) the atom stays and its position is
+ * shifted by the local box length (:571-583); otherwise it leaves in packet
+ * d: [N ids][3N positions][3N velocities][3N accelerations][N molecule
+ * indices = -1][N bond counts = 0] (:308-369), positions shifted by
+ * size_global across a periodic grid edge (:402-431), packets concatenated
+ * in d order.  counts[27] (counts[13] = 0); stay[i] (nullable) = 1 for
+ * staying atoms and pos_after (nullable, 3n) their positions after the
+ * shift (the leavers' unshifted).  Returns the doubles needed (writes out,
+ * stay and pos_after only when cap suffices). */
+int64_t orc_owned_pack(int64_t n, const double *pos, const double *vel, const double *acc,
+                       const int64_t *id, const int32_t grid_index[3], const int32_t nprocs[3],
+                       const double lo_local[3], const double hi_local[3],
+                       const double size_global[3], const int32_t bc[3], int64_t counts[27],
+                       int64_t cap, double *out, int32_t *stay, double *pos_after);
+
 /* neighborlist.cpp:247-297 (O(N^2) default).  CSR output: offs[n+1],
  * nbr[...] with ghost g stored as n+g.  Returns the total number of entries
  * (writes at most cap). */

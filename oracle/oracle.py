@@ -79,6 +79,9 @@ def oracle_lib():
         L.orc_halo_pack.argtypes = [C.c_int64, _d, _d, _i64, _i32, _i32, _i32, _d, _d, _d, _i32,
                                     C.c_double, C.c_int32, _i64, C.c_int64, _d]
         L.orc_halo_pack.restype = C.c_int64
+        L.orc_owned_pack.argtypes = [C.c_int64, _d, _d, _d, _i64, _i32, _i32, _d, _d, _d, _i32, _i64,
+                                     C.c_int64, _d, _i32, _d]
+        L.orc_owned_pack.restype = C.c_int64
         L.orc_nlist_brute.argtypes = [C.c_int64, _d, C.c_int64, _d, C.c_double, C.c_double,
                                       C.c_int64, _i64, _u32]
         L.orc_nlist_brute.restype = C.c_int64
@@ -132,6 +135,28 @@ def ghosts(pos, box, ghost_cutoff):
         if k <= cap:
             return g[:k].copy(), o[:k].copy(), c[:k].copy()
         cap = k
+
+
+def owned_pack(pos, vel, acc, id_, grid_index, nprocs, lo_local, hi_local, size_global, bc):
+    """Owned-atom exchange packets (exchange_owned_mpi_shared_atoms.cpp):
+    (counts[27], packets concatenated in direction order, stay[n] as bool,
+    positions after the shifts back into the domain [n, 3])."""
+    p = _f64(pos); v = _f64(vel); a = _f64(acc)
+    n = len(p)
+    ids = np.ascontiguousarray(id_, np.int64)
+    a32 = lambda x: np.ascontiguousarray(x, np.int32)
+    gi, npr, bcs = a32(grid_index), a32(nprocs), a32(bc)
+    lo, hi, sg = _f64(lo_local), _f64(hi_local), _f64(size_global)
+    counts = np.zeros(27, np.int64)
+    L = oracle_lib()
+    args = (n, _p(p, _d), _p(v, _d), _p(a, _d), _p(ids, _i64), _p(gi, _i32), _p(npr, _i32),
+            _p(lo, _d), _p(hi, _d), _p(sg, _d), _p(bcs, _i32), _p(counts, _i64))
+    need = L.orc_owned_pack(*args, 0, None, None, None)
+    out = np.empty(max(need, 1))
+    stay = np.zeros(max(n, 1), np.int32)
+    after = np.zeros((max(n, 1), 3))
+    L.orc_owned_pack(*args, need, _p(out, _d), _p(stay, _i32), _p(after, _d))
+    return counts, out[:need], stay[:n].astype(bool), after[:n]
 
 
 def halo_pack(pos, vel, id_, type_, grid_index, nprocs, lo_local, hi_local, size_global, bc,

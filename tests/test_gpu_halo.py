@@ -45,7 +45,9 @@ def split(counts, out, total):
 
 @pytest.fixture(scope="module")
 def domains():
-    s = lj_fluid(6, seed=31)
+    # domains at least twice the ghost cutoff wide (the halo rule gives an atom
+    # one direction per axis, so a narrower domain misses images)
+    s = lj_fluid(8, seed=31)
     box = O.make_box(s["lo"], s["hi"])
     rng = np.random.default_rng(5)
     pos, _ = O.wrap(s["pos"] + rng.normal(0.0, 0.05, s["pos"].shape), box)
