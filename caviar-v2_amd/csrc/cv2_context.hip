@@ -1792,6 +1792,24 @@ int cv2_get_stats(cv2_ctx *ctx, cv2_stats *out) {
   return CV2_OK;
 }
 
+int cv2_get_build_filter(cv2_ctx *ctx, cv2_build_filter *out) {
+  if (!ctx || !out) return CV2_ERR_BAD_ARG;
+  if (!ctx->box_set || !ctx->nl_set) return fail(ctx, CV2_ERR_STATE, "set the box and neighbour settings first");
+  const BoxDev &b = ctx->b;
+  *out = cv2_build_filter{};
+  for (int d = 0; d < 3; ++d) {
+    out->qunit[d] = b.qinv[d];
+    out->lo[d] = b.lo[d];
+    out->ncell[d] = b.nc[d];
+    out->cell_width[d] = b.cinv[d] > 0.0 ? 1.0 / b.cinv[d] : 0.0;
+  }
+  out->ce = std::sqrt(b.ce_sq);
+  out->cthr_in = b.cthr_in;
+  out->cthr_out = b.cthr_out;
+  out->coarse = b.coarse;
+  return CV2_OK;
+}
+
 int cv2_synchronize(cv2_ctx *ctx) {
   if (!ctx) return CV2_ERR_BAD_ARG;
   HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -32,6 +32,12 @@ class Stats(C.Structure):
                 ("build_kind", C.c_int32), ("force_tasks", C.c_int32)]
 
 
+class BuildFilter(C.Structure):
+    _fields_ = [("qunit", C.c_double * 3), ("lo", C.c_double * 3), ("ce", C.c_double),
+                ("cell_width", C.c_double * 3), ("cthr_in", C.c_float), ("cthr_out", C.c_float),
+                ("coarse", C.c_int32), ("ncell", C.c_int32 * 3)]
+
+
 class Timing(C.Structure):
     _fields_ = [("force_ms", C.c_double), ("force_calls", C.c_int64), ("build_ms", C.c_double),
                 ("integrate_ms", C.c_double)]
@@ -105,6 +111,7 @@ def lib():
             "cv2_download": ([vp, _d, _d, _d], C.c_int),
             "cv2_get_pairs": ([vp, C.c_int64, _i64, _i64], C.c_int),
             "cv2_get_stats": ([vp, C.POINTER(Stats)], C.c_int),
+            "cv2_get_build_filter": ([vp, C.POINTER(BuildFilter)], C.c_int),
             "cv2_synchronize": ([vp], C.c_int),
             "cv2_set_timing": ([vp, C.c_int], C.c_int),
             "cv2_get_timing": ([vp, C.POINTER(Timing)], C.c_int),
@@ -370,6 +377,14 @@ class Context:
         out = np.empty((max(k.value, 1), 3), np.int64)
         self._chk(self.L.cv2_get_pairs(self.h, k.value, C.byref(k), _p(out, _i64)))
         return out[: k.value]
+
+    def build_filter(self):
+        """Fixed-point unit and coarse-filter thresholds of the list build."""
+        f = BuildFilter()
+        self._chk(self.L.cv2_get_build_filter(self.h, C.byref(f)))
+        return dict(qunit=np.array(f.qunit[:]), lo=np.array(f.lo[:]), ce=f.ce,
+                    cell_width=np.array(f.cell_width[:]), cthr_in=np.float32(f.cthr_in),
+                    cthr_out=np.float32(f.cthr_out), coarse=f.coarse, ncell=tuple(f.ncell[:]))
 
     def stats(self):
         s = Stats()

@@ -332,6 +332,27 @@ int cv2_get_pairs(cv2_ctx *ctx, int64_t cap, int64_t *npairs, int64_t *pairs);
 
 int cv2_get_stats(cv2_ctx *ctx, cv2_stats *out);
 
+/* The fixed-point grid and the coarse distance filter of the wave-per-cell
+ * list build, as the library set them up for the current box (for checks of
+ * the filter's soundness against the FP64 predicate).  A coordinate is
+ * round((x - lo) / qunit) kept modulo 2^32 (a separation is the int32
+ * difference of two, converted to float and multiplied by (float)qunit);
+ * a candidate whose float squared distance is below cthr_in is inside the
+ * list cutoff ce for certain, at or above cthr_out outside it for certain,
+ * and in between the FP64 predicate decides.  coarse = 0: the per-atom
+ * build (no coarse filter) is in use.  No counterpart in the reference,
+ * whose list build has no coarse filter. */
+typedef struct {
+  double qunit[3];
+  double lo[3];
+  double ce;
+  double cell_width[3];
+  float cthr_in, cthr_out;
+  int32_t coarse;
+  int32_t ncell[3];
+} cv2_build_filter;
+int cv2_get_build_filter(cv2_ctx *ctx, cv2_build_filter *out);
+
 /* Kernel timing with HIP events recorded on the context's stream around each
  * kernel group (no reference counterpart; the reference only logs the wall
  * time of Md_simulator::run, md_simulator.cpp:325-353). */

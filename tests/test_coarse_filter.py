@@ -5,9 +5,12 @@ squared distance (from the int32 fixed-point coordinates, relative to a
 reference atom of the home cell) is below cthr_in, and rejects it when at or
 above cthr_out.  This test reproduces the device arithmetic in float32 with
 numpy on pairs placed within 3e-5 of the list cutoff, in boxes of several
-shapes, with the thresholds computed by the library's own setup path
-(read back through the same formulas), and checks that neither shortcut
-ever contradicts the FP64 predicate (dx*dx + dy*dy) + dz*dz < ce^2.
+shapes, with the thresholds restated from the library's setup path, and
+checks that neither shortcut ever contradicts the FP64 predicate
+(dx*dx + dy*dy) + dz*dz < ce^2.  tests/test_gpu_coarse_filter.py checks
+that this restatement equals the values the library sets up
+(cv2_get_build_filter), repeats the check with those values, and drives
+the band path on the device.
 """
 import math
 
