@@ -50,7 +50,8 @@ struct cv2_ens {
   double *d_thermo = nullptr;
   int *h_flags = nullptr;
   // launch shapes, graph of one step
-  cv2_ens_tuning tune{256, 128, 0, 0.0};
+  cv2_ens_tuning tune{256, 128, 0, 0.0, 1.0};
+  double cell_scale = 1.0;  // cell width of the current layout, in list cutoffs
   bool use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -140,7 +141,9 @@ void free_all(cv2_ens *ens) {
 // ce per axis the system takes the wave-per-cell build (coarse filter on
 // the fixed-point coordinates); otherwise cells at least half the list cutoff
 // wide for the per-atom build (+-2 cells).
-BoxDev system_box(const double lo[3], const double hi[3], double ce, double rc) {
+// cell_scale >= 1: cells at least cell_scale * ce wide where that still
+// leaves two cells per axis (the wave build's minimum), else ce wide.
+BoxDev system_box(const double lo[3], const double hi[3], double ce, double rc, double cell_scale) {
   BoxDev b{};
   b.ce_sq = ce * ce;
   b.thresh_sq = 0.25 * (ce - rc) * (ce - rc);
@@ -161,9 +164,16 @@ BoxDev system_box(const double lo[3], const double hi[3], double ce, double rc) 
     // positions (pos_rel).
     fast = fast && std::floor(b.L[d] / (ce * (1.0 + 1e-12))) >= 2.0;
   }
+  double scale = 1.0;
+  if (fast && cell_scale > 1.0) {
+    bool wide_ok = true;
+    for (int d = 0; d < 3; ++d)
+      wide_ok = wide_ok && std::floor(b.L[d] / (cell_scale * ce * (1.0 + 1e-12))) >= 2.0;
+    if (wide_ok) scale = cell_scale;
+  }
   double wmax = 0.0;
   for (int d = 0; d < 3; ++d) {
-    int nc = (int)std::floor(b.L[d] / ((fast ? 1.0 : 0.5) * ce * (1.0 + 1e-12)));
+    int nc = (int)std::floor(b.L[d] / ((fast ? scale : 0.5) * ce * (1.0 + 1e-12)));
     nc = std::max(1, std::min(nc, 1024));
     b.nc[d] = nc;
     b.cinv[d] = nc / b.L[d];
@@ -361,53 +371,96 @@ int cv2_ens_set_neighbor(cv2_ens *ens, double cutoff, double extra_coef) {
   return CV2_OK;
 }
 
+namespace {
+// Cell grids of all systems for a cell width of cell_scale list cutoffs
+// (system_box): boxes, cell offsets and system ids, the cell arrays and the
+// per-block statistics.  The atoms keep their arrays; the caller reloads the
+// state (which forces a build).
+int layout_cells(cv2_ens *ens, double cell_scale) {
+  const double ce = ens->nl_cutoff * ens->nl_coef;
+  const int nsys = ens->nsys;
+  std::vector<BoxDev> boxes(nsys);
+  std::vector<int> coff(nsys, 0);
+  int64_t ncell = 0;
+  bool iso = true, per_atom = false;
+  for (int s = 0; s < nsys; ++s) {
+    boxes[s] = system_box(&ens->lo[3 * s], &ens->hi[3 * s], ce, ens->nl_cutoff, cell_scale);
+    iso = iso && boxes[s].qiso;
+    per_atom = per_atom || !boxes[s].coarse;
+    coff[s] = (int)ncell;
+    ncell += (int64_t)boxes[s].nc[0] * boxes[s].nc[1] * boxes[s].nc[2];
+  }
+  if (ncell > (1 << 28)) return efail(ens, CV2_ERR_BAD_ARG, "cell grids too large");
+  drop_graph(ens);
+  std::vector<int> csys(ncell);
+  for (int s = 0; s < nsys; ++s) {
+    const int nc = boxes[s].nc[0] * boxes[s].nc[1] * boxes[s].nc[2];
+    for (int c = 0; c < nc; ++c) csys[coff[s] + c] = s;
+  }
+  DevState &st = ens->s;
+  for (void **p : {(void **)&st.cell_count, (void **)&st.cell_start, (void **)&st.tile_sum,
+                   (void **)&st.wstat, (void **)&ens->d_cell_sys, (void **)&ens->d_cell_off,
+                   (void **)&ens->d_box}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  const int64_t n = ens->n;
+  ens->iso = iso;
+  ens->per_atom = per_atom;
+  ens->ncell = (int)ncell;
+  ens->cell_scale = cell_scale;
+  ens->thresh_sq = boxes[0].thresh_sq;
+  st.ncell = (int)ncell;
+  EHIP(ralloc(&st.cell_count, ncell + 1));
+  EHIP(ralloc(&st.cell_start, ncell + 1));
+  EHIP(ralloc(&st.tile_sum, ncell / 1024 + 2));
+  // per-block statistics: per-atom build blocks (>= 64 atoms), then
+  // wave-per-cell build blocks (4 cells)
+  EHIP(ralloc(&st.wstat, (n + 63) / 64 + (ncell + 3) / 4 + 16));
+  EHIP(ralloc(&ens->d_cell_sys, ncell));
+  EHIP(ralloc(&ens->d_cell_off, nsys));
+  EHIP(ralloc(&ens->d_box, nsys));
+  EHIP(hipMemcpyAsync(ens->d_cell_sys, csys.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, ens->stream));
+  EHIP(hipMemcpyAsync(ens->d_cell_off, coff.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ens->stream));
+  EHIP(hipMemcpyAsync(ens->d_box, boxes.data(), sizeof(BoxDev) * nsys, hipMemcpyHostToDevice, ens->stream));
+  EHIP(hipMemsetAsync(st.cell_count, 0, sizeof(int) * (ncell + 1), ens->stream));
+  EHIP(hipStreamSynchronize(ens->stream));
+  return CV2_OK;
+}
+}  // namespace
+
 int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const double *lo,
                    const double *hi, const double *pos, const double *vel) {
   if (!ens || nsys < 1 || !natoms || !lo || !hi || !pos) return efail(ens, CV2_ERR_BAD_ARG, "bad systems");
   if (!ens->nl_set || !ens->lj_set) return efail(ens, CV2_ERR_STATE, "set LJ and neighbour parameters first");
   (void)hipSetDevice(ens->device);
   const double ce = ens->nl_cutoff * ens->nl_coef;
-  std::vector<BoxDev> boxes(nsys);
   std::vector<int64_t> aoff(nsys + 1, 0);
-  std::vector<int> coff(nsys, 0);
-  int64_t ncell = 0;
   double max_expect = 0.0;
-  bool iso = true, per_atom = false;
   for (int s = 0; s < nsys; ++s) {
     if (natoms[s] < 1) return efail(ens, CV2_ERR_BAD_ARG, "every system needs atoms");
-    for (int d = 0; d < 3; ++d)
+    double vol = 1.0;
+    for (int d = 0; d < 3; ++d) {
       if (!(hi[3 * s + d] - lo[3 * s + d] > 2.0 * ce))
         return efail(ens, CV2_ERR_BAD_ARG, "periodic box must be wider than twice the list cutoff");
-    boxes[s] = system_box(lo + 3 * s, hi + 3 * s, ce, ens->nl_cutoff);
-    iso = iso && boxes[s].qiso;
-    per_atom = per_atom || !boxes[s].coarse;
+      vol *= hi[3 * s + d] - lo[3 * s + d];
+    }
     aoff[s + 1] = aoff[s] + natoms[s];
-    coff[s] = (int)ncell;
-    ncell += (int64_t)boxes[s].nc[0] * boxes[s].nc[1] * boxes[s].nc[2];
-    const double vol = boxes[s].L[0] * boxes[s].L[1] * boxes[s].L[2];
     max_expect = std::max(max_expect, 4.18879020478639 * ce * ce * ce * (double)natoms[s] / vol);
   }
   const int64_t n = aoff[nsys];
   if (n > (int64_t)kIndexMask) return efail(ens, CV2_ERR_BAD_ARG, "too many atoms");
-  if (ncell > (1 << 28)) return efail(ens, CV2_ERR_BAD_ARG, "cell grids too large");
   free_all(ens);
   ens->nsys = nsys;
   ens->n = n;
-  ens->iso = iso;
-  ens->per_atom = per_atom;
   ens->natoms.assign(natoms, natoms + nsys);
   ens->atom_off = aoff;
   ens->lo.assign(lo, lo + 3 * nsys);
   ens->hi.assign(hi, hi + 3 * nsys);
-  ens->ncell = (int)ncell;
-  ens->thresh_sq = boxes[0].thresh_sq;
-  // per-atom and per-cell system ids
-  std::vector<int> sys(n), csys(ncell), aoff32(nsys + 1);
-  for (int s = 0; s < nsys; ++s) {
+  // per-atom system ids
+  std::vector<int> sys(n), aoff32(nsys + 1);
+  for (int s = 0; s < nsys; ++s)
     for (int64_t i = aoff[s]; i < aoff[s + 1]; ++i) sys[i] = s;
-    const int nc = boxes[s].nc[0] * boxes[s].nc[1] * boxes[s].nc[2];
-    for (int c = 0; c < nc; ++c) csys[coff[s] + c] = s;
-  }
   for (int s = 0; s <= nsys; ++s) aoff32[s] = (int)aoff[s];
   DevState &st = ens->s;
   st.n = (int)n;
@@ -415,7 +468,6 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
   int cap = (int)std::ceil(1.6 * max_expect + 64.0);
   st.cap = (cap + 15) / 16 * 16;
   st.stride = (int)((n + 63) / 64 * 64);
-  st.ncell = (int)ncell;
   st.write_q = true;
   EHIP(ralloc(&st.pos, n));
   EHIP(ralloc(&st.pos_alt, n));
@@ -434,42 +486,53 @@ int cv2_ens_upload(cv2_ens *ens, int32_t nsys, const int64_t *natoms, const doub
   EHIP(ralloc(&st.order, n));
   EHIP(ralloc(&st.nnbr, n));
   EHIP(ralloc(&st.nbr, (size_t)st.cap * st.stride));
-  EHIP(ralloc(&st.cell_count, ncell + 1));
-  EHIP(ralloc(&st.cell_start, ncell + 1));
-  EHIP(ralloc(&st.tile_sum, ncell / 1024 + 2));
   EHIP(ralloc(&st.flags, F_COUNT));
-  // per-block statistics: per-atom build blocks (>= 64 atoms), then
-  // wave-per-cell build blocks (4 cells)
-  EHIP(ralloc(&st.wstat, (n + 63) / 64 + (ncell + 3) / 4 + 16));
   EHIP(ralloc(&ens->d_sys, n));
-  EHIP(ralloc(&ens->d_cell_sys, ncell));
-  EHIP(ralloc(&ens->d_cell_off, nsys));
   EHIP(ralloc(&ens->d_atom_off, nsys + 1));
-  EHIP(ralloc(&ens->d_box, nsys));
   EHIP(ralloc(&ens->d_sflag, nsys));
   EHIP(ralloc(&ens->d_srebuilds, nsys));
   EHIP(ralloc(&ens->d_act, 4 * nsys + 8));  // systems and cells of a build (EnsDev::act)
   EHIP(ralloc(&ens->eatom, 2 * n));
   EHIP(ralloc(&ens->d_thermo, 3 * nsys));
   EHIP(hipMemcpyAsync(ens->d_sys, sys.data(), sizeof(int) * n, hipMemcpyHostToDevice, ens->stream));
-  EHIP(hipMemcpyAsync(ens->d_cell_sys, csys.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, ens->stream));
-  EHIP(hipMemcpyAsync(ens->d_cell_off, coff.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ens->stream));
   EHIP(hipMemcpyAsync(ens->d_atom_off, aoff32.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, ens->stream));
-  EHIP(hipMemcpyAsync(ens->d_box, boxes.data(), sizeof(BoxDev) * nsys, hipMemcpyHostToDevice, ens->stream));
   EHIP(hipMemsetAsync(ens->eatom, 0, sizeof(long long) * 2 * n, ens->stream));
   EHIP(hipStreamSynchronize(ens->stream));
+  int rc = layout_cells(ens, ens->tune.cell_scale);
+  if (rc) return rc;
   ens->atoms = true;
   return load_state(ens, pos, vel);
 }
+
+namespace {
+// A new cell width: the systems' cell grids are laid out again and the
+// current state reloaded into them (a build follows).
+int change_cells(cv2_ens *ens, double cell_scale) {
+  if (cell_scale == ens->cell_scale || !ens->atoms) {
+    ens->cell_scale = ens->atoms ? ens->cell_scale : cell_scale;
+    return CV2_OK;
+  }
+  const int64_t n = ens->n;
+  std::vector<double> pos(3 * n), vel(3 * n);
+  int rc = cv2_ens_download(ens, pos.data(), vel.data(), nullptr);
+  if (!rc) rc = layout_cells(ens, cell_scale);
+  if (!rc) rc = load_state(ens, pos.data(), vel.data());
+  return rc;
+}
+}  // namespace
 
 int cv2_ens_set_tuning(cv2_ens *ens, const cv2_ens_tuning *t) {
   if (!ens || !t) return CV2_ERR_BAD_ARG;
   if (!valid_block(t->force_block) || !valid_block(t->build_block) || (t->rebuild_all != 0 && t->rebuild_all != 1))
     return efail(ens, CV2_ERR_BAD_ARG, "block sizes must be 64, 128 or 256 and rebuild_all 0 or 1");
+  if (!(t->cell_scale >= 1.0 && t->cell_scale <= 2.0))
+    return efail(ens, CV2_ERR_BAD_ARG, "cell_scale must lie in [1, 2]");
+  (void)hipSetDevice(ens->device);
   ens->tune.force_block = t->force_block;
   ens->tune.build_block = t->build_block;
   ens->tune.rebuild_all = t->rebuild_all;
-  return CV2_OK;
+  ens->tune.cell_scale = t->cell_scale;
+  return change_cells(ens, t->cell_scale);
 }
 
 int cv2_ens_tune(cv2_ens *ens, double dt, int32_t trial_steps, cv2_ens_tuning *best) {
@@ -481,22 +544,37 @@ int cv2_ens_tune(cv2_ens *ens, double dt, int32_t trial_steps, cv2_ens_tuning *b
   int rc = cv2_ens_download(ens, pos.data(), vel.data(), nullptr);
   if (rc) return rc;
   const int blocks[3] = {64, 128, 256};
-  cv2_ens_tuning win{ens->tune.force_block, ens->tune.build_block, ens->tune.rebuild_all, 1e30};
+  const double scale0 = ens->cell_scale;
+  cv2_ens_tuning win{ens->tune.force_block, ens->tune.build_block, ens->tune.rebuild_all, 1e30, scale0};
+  auto trial = [&](const cv2_ens_tuning &t) -> int {
+    ens->tune = t;
+    int r = load_state(ens, pos.data(), vel.data());
+    if (!r) r = run(ens, 5, dt);  // warm-up (and graph capture)
+    if (r) return r;
+    const auto t0 = std::chrono::steady_clock::now();
+    r = run(ens, trial_steps, dt);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (r) return r;
+    const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / trial_steps;
+    if (ms < win.ms_per_step) {
+      win = t;
+      win.ms_per_step = ms;
+    }
+    return CV2_OK;
+  };
+  // launch shapes and rebuild policy on the current cells, then the cell
+  // width with the best of them
   for (int fb : blocks)
     for (int bb : blocks)
-      for (int pol = 0; pol < 2; ++pol) {
-        ens->tune = cv2_ens_tuning{fb, bb, pol, 0.0};
-        rc = load_state(ens, pos.data(), vel.data());
-        if (!rc) rc = run(ens, 5, dt);  // warm-up (and graph capture)
-        if (rc) return rc;
-        const auto t0 = std::chrono::steady_clock::now();
-        rc = run(ens, trial_steps, dt);
-        const auto t1 = std::chrono::steady_clock::now();
-        if (rc) return rc;
-        const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / trial_steps;
-        if (ms < win.ms_per_step) win = cv2_ens_tuning{fb, bb, pol, ms};
-      }
+      for (int pol = 0; pol < 2; ++pol)
+        if ((rc = trial(cv2_ens_tuning{fb, bb, pol, 0.0, scale0}))) return rc;
+  for (double sc : {1.0, 1.15, 1.3}) {
+    if (sc == scale0) continue;
+    if ((rc = layout_cells(ens, sc))) return rc;
+    if ((rc = trial(cv2_ens_tuning{win.force_block, win.build_block, win.rebuild_all, 0.0, sc}))) return rc;
+  }
   ens->tune = win;
+  if (ens->cell_scale != win.cell_scale && (rc = layout_cells(ens, win.cell_scale))) return rc;
   rc = load_state(ens, pos.data(), vel.data());
   if (rc) return rc;
   if (best) *best = win;

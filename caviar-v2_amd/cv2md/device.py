@@ -51,7 +51,7 @@ class HaloGeom(C.Structure):
 
 class EnsTuning(C.Structure):
     _fields_ = [("force_block", C.c_int32), ("build_block", C.c_int32), ("rebuild_all", C.c_int32),
-                ("ms_per_step", C.c_double)]
+                ("ms_per_step", C.c_double), ("cell_scale", C.c_double)]
 
 
 class EnsStats(C.Structure):
@@ -463,15 +463,15 @@ class Ensemble:
         self.natoms = nat
         self.offsets = np.concatenate([[0], np.cumsum(nat)])
 
-    def set_tuning(self, force_block=256, build_block=128, rebuild_all=0):
-        t = EnsTuning(force_block, build_block, rebuild_all, 0.0)
+    def set_tuning(self, force_block=256, build_block=128, rebuild_all=0, cell_scale=1.0):
+        t = EnsTuning(force_block, build_block, rebuild_all, 0.0, float(cell_scale))
         self._chk(self.L.cv2_ens_set_tuning(self.h, C.byref(t)))
 
     def tune(self, dt, trial_steps=20):
         t = EnsTuning()
         self._chk(self.L.cv2_ens_tune(self.h, dt, int(trial_steps), C.byref(t)))
         return dict(force_block=t.force_block, build_block=t.build_block, rebuild_all=t.rebuild_all,
-                    ms_per_step=t.ms_per_step)
+                    ms_per_step=t.ms_per_step, cell_scale=t.cell_scale)
 
     def use_graph(self, enable=True):
         self._chk(self.L.cv2_ens_use_graph(self.h, int(bool(enable))))

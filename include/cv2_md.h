@@ -388,6 +388,8 @@ typedef struct {
   int32_t build_block;   /* threads per block of the list kernel: 64, 128 or 256 */
   int32_t rebuild_all;   /* 0: rebuild only the systems that ask; 1: rebuild all when any asks */
   double ms_per_step;    /* cv2_ens_tune: measured time per step of the chosen setting */
+  double cell_scale;     /* cell width in list cutoffs, 1 to 2: each system's cells are at least
+                            this wide where that leaves two per axis, else one list cutoff */
 } cv2_ens_tuning;
 
 typedef struct {

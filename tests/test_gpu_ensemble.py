@@ -160,9 +160,32 @@ def test_tuner_keeps_state():
         after = e.download()
         assert best["force_block"] in (64, 128, 256) and best["build_block"] in (64, 128, 256)
         assert best["rebuild_all"] in (0, 1) and best["ms_per_step"] > 0
+        assert best["cell_scale"] in (1.0, 1.15, 1.3)
         for a, b in zip(before, after):
             assert np.array_equal(a, b)
         e.run_vv(50, DT)
+
+
+@pytest.mark.parametrize("scale", [1.15, 1.3])
+def test_cell_width_gives_the_same_bits(scale):
+    """The tuner's cell-width axis: wider cells (fewer cells, more
+    candidates per home atom) give the same pair sets, and with fixed-point
+    sums the same trajectory bit for bit, as cells one list cutoff wide."""
+    systems = small_ensemble(seed=26, nsys=5)
+    out = []
+    for sc in (1.0, scale):
+        with D.Ensemble.from_systems(systems, rc=RC, nl_cutoff=RC, nl_coef=NL_COEF) as e:
+            e.set_tuning(cell_scale=sc)
+            e.run_vv(150, DT)
+            e.build()
+            out.append((e.download(), [sort_rows(e.pairs(k)) for k in range(len(systems))],
+                        e.stats()["system_builds"]))
+    (d0, p0, b0), (d1, p1, b1) = out
+    assert b0 == b1 and b0 > len(systems)
+    for a, b in zip(d0, d1):
+        assert np.array_equal(a, b)
+    for a, b in zip(p0, p1):
+        assert np.array_equal(a, b)
 
 
 def test_c4_workload_512_systems():
