@@ -6,6 +6,10 @@
 //       read the system, run Velocity_verlet or Leap_frog with every writer
 //       (energy, temperature, pressure, potential energy, xyz with id,
 //       velocity and acceleration) at the cadence, print the final state
+//   dropin_checks molrun <input> <steps>
+//       molecule indices (runs of 4 atoms, every fifth run outside
+//       molecules) with Lj::ignore_intra_molecule_flag, run Velocity_verlet,
+//       print the final state
 //   dropin_checks neighlist <input> <all>
 //       build the list (Neighborlist::build(true)), materialise neighlist
 //       (all = 1: the all_atom_test form), print it and the ghosts
@@ -85,6 +89,21 @@ int main(int argc, char **argv) {
     sim.md_simulator.set_integrator_type(lf ? Integrator_t::Leap_frog : Integrator_t::Velocity_verlet);
     sim.md_simulator.set_initial_step(0);
     sim.md_simulator.set_final_step((size_t)steps);
+    sim.md_simulator.set_dt(0.005);
+    sim.md_simulator.run();
+    const auto &o = sim.atom_data.atom_struct_owned;
+    for (size_t i = 0; i < o.position.size(); ++i)
+      std::printf("%a %a %a %a %a %a\n", o.position[i].x, o.position[i].y, o.position[i].z,
+                  o.velocity[i].x, o.velocity[i].y, o.velocity[i].z);
+    return 0;
+  }
+  if (mode == "molrun") {
+    auto &mi = sim.atom_data.atom_struct_owned.molecule_index;
+    for (size_t i = 0; i < mi.size(); ++i) mi[i] = (i / 4) % 5 == 4 ? -1 : (int)(i / 4);
+    lj.ignore_intra_molecule_flag(true);
+    sim.md_simulator.set_integrator_type(Integrator_t::Velocity_verlet);
+    sim.md_simulator.set_initial_step(0);
+    sim.md_simulator.set_final_step((size_t)std::atol(argv[3]));
     sim.md_simulator.set_dt(0.005);
     sim.md_simulator.run();
     const auto &o = sim.atom_data.atom_struct_owned;

@@ -226,6 +226,34 @@ def test_writers_and_leapfrog_match_oracle(integ, tmp_path):
 
 
 @pytest.mark.gpu
+def test_ignore_intra_molecule_dropin_matches_oracle(tmp_path):
+    """A user program that sets Atom_struct::molecule_index and calls
+    Lj::ignore_intra_molecule_flag (lj.cpp:446) runs 40 velocity-Verlet steps
+    on the device; the final state equals the oracle's run with the same
+    molecules."""
+    s = _small_fluid()
+    inp = tmp_path / "in.txt"
+    _write_input(inp, s)
+    exe = _compile(os.path.join(ROOT, "tests", "cpp", "dropin_checks.cpp"), str(tmp_path / "dc"))
+    r = subprocess.run([exe, "molrun", str(inp), "40"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [l.split() for l in r.stdout.split("\n")]
+    fin = np.array([[float.fromhex(x) for x in t] for t in rows
+                    if len(t) == 6 and all(x.lstrip("-").startswith("0x") for x in t)])
+    n = len(s["pos"])
+    assert fin.shape == (n, 6)
+    mol = np.where((np.arange(n) // 4) % 5 == 4, -1, np.arange(n) // 4)
+    with O.OracleSim(dict(s, use_cell_list=1)) as o:
+        o.set_molecules(mol)
+        o.setup()
+        for _ in range(40):
+            o.step()
+        po, vo, _ = o.state()
+    assert np.abs(fin[:, :3] - po).max() < 1e-10 and np.abs(fin[:, 3:] - vo).max() < 1e-9
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("all_pairs", [0, 1])
 def test_host_neighlist_matches_oracle(all_pairs, tmp_path):
     """Neighborlist::update_host_list after a forced build: the ghosts are the
