@@ -101,6 +101,7 @@ struct cv2_ctx {
   int *type_by_id = nullptr; // cv2_set_type_table
   int64_t ntype_ids = 0;
   int *mig_moved = nullptr;  // device flag: some atom was shifted back into the domain
+  unsigned long long *vstate = nullptr;  // CV2_NL_VERIFY: state of the build check (k_verify.hip)
   void *mig_aux = nullptr;   // received ids, sources and packet tables
   size_t mig_aux_bytes = 0;
   double virial = 0.0, pe = 0.0;
@@ -394,7 +395,7 @@ int prepare_list_form(cv2_ctx *ctx) {
   for (int k = 0; k < ctx->lj.nt * ctx->lj.nt; ++k) rc2max = std::max(rc2max, ctx->lj.rc2[k]);
   // (ignore_intra_molecule runs on the gather kernels, 32-bit entries)
   s.slots = ctx->precision != CV2_PREC_FP64 && ctx->b.coarse && s.n > 0 &&
-            rc2max < 1.9 * ctx->b.ce_sq && !s.mol;
+            rc2max < 1.9 * ctx->b.ce_sq && !s.mol && !ctx->vstate;
   if (!s.slots) return CV2_OK;
   const TaskGeom old = s.tg;
   s.tg = task_geom(s.n, ctx->b, ctx->min_mpr);
@@ -439,6 +440,13 @@ int fetch_energy(cv2_ctx *ctx) {
   return CV2_OK;
 }
 
+// A list build as queued on the stream, followed (CV2_NL_VERIFY) by the
+// independent check of the pair set it produced, before anything moves.
+void build_checked(cv2_ctx *ctx, bool keep_acc, int step = 0) {
+  launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, keep_acc, step);
+  if (ctx->vstate) launch_verify(ctx->s, ctx->b, ctx->vstate, ctx->stream);
+}
+
 int queue_build(cv2_ctx *ctx, bool force, bool keep_acc, int step = 0) {
   force = force || ctx->need_build;
   ctx->need_build = false;
@@ -448,7 +456,7 @@ int queue_build(cv2_ctx *ctx, bool force, bool keep_acc, int step = 0) {
   }
   if (force || (ctx->nl_flags & CV2_NL_REBUILD_EVERY_STEP))
     launch_set_flag(ctx->s.flags, F_REBUILD, 1, ctx->stream);
-  launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, keep_acc, step);
+  build_checked(ctx, keep_acc, step);
   return CV2_OK;
 }
 
@@ -766,6 +774,7 @@ int cv2_destroy(cv2_ctx *ctx) {
                   (void *)ctx->halo_buf, (void *)ctx->halo_res})
     if (p) (void)hipFree(p);
   if (ctx->type_by_id) (void)hipFree(ctx->type_by_id);
+  if (ctx->vstate) (void)hipFree(ctx->vstate);
   if (ctx->mig_moved) (void)hipFree(ctx->mig_moved);
   if (ctx->mig_aux) (void)hipFree(ctx->mig_aux);
   if (ctx->lj_dev) (void)hipFree(ctx->lj_dev);
@@ -866,6 +875,20 @@ int cv2_set_neighbor(cv2_ctx *ctx, double cutoff, double extra_coef, unsigned fl
   if (!ctx) return CV2_ERR_BAD_ARG;
   if (cutoff <= 0.0) return fail(ctx, CV2_ERR_BAD_ARG, "cutoff have to non-negative.");
   if (extra_coef < 1.0) return fail(ctx, CV2_ERR_BAD_ARG, "cutoff_extra_coef must be larger than 1.0");
+  if ((flags & CV2_NL_VERIFY) && ctx->host_halo)
+    return fail(ctx, CV2_ERR_STATE, "CV2_NL_VERIFY checks periodic and open boxes, not host-supplied halos");
+  if (flags & CV2_NL_VERIFY) {
+    if (!ctx->vstate) {
+      (void)hipSetDevice(ctx->device);
+      HIPCHK(dalloc(&ctx->vstate, (size_t)verify_state_words()));
+      HIPCHK(hipMemsetAsync(ctx->vstate, 0, sizeof(unsigned long long) * verify_state_words(), ctx->stream));
+      launch_verify_arm(ctx->s.flags, ctx->vstate, ctx->stream);
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+  } else if (ctx->vstate) {
+    (void)hipFree(ctx->vstate);
+    ctx->vstate = nullptr;
+  }
   ctx->nl_cutoff = cutoff;
   ctx->nl_coef = extra_coef;
   ctx->nl_flags = flags;
@@ -955,6 +978,8 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
   HIPCHK(hipMemcpyAsync(s.tag, tg.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(s.id, idv.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemsetAsync(s.flags, 0, sizeof(int) * F_COUNT, ctx->stream));
+  if (ctx->vstate)  // the build count starts again: so do the checks
+    HIPCHK(hipMemsetAsync(ctx->vstate, 0, sizeof(unsigned long long) * verify_state_words(), ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->atoms = true;
   ctx->need_build = true;
@@ -974,6 +999,7 @@ int cv2_set_ghosts(cv2_ctx *ctx, int64_t ng, const double *pos, const int32_t *t
   if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload owned atoms before ghosts");
   if (ctx->s.mol)
     return fail(ctx, CV2_ERR_STATE, "host-supplied ghosts carry no molecule indices (ignore_intra_molecule is on)");
+  if (ctx->vstate) return fail(ctx, CV2_ERR_STATE, "CV2_NL_VERIFY does not check host-supplied halos");
   (void)hipSetDevice(ctx->device);
   const bool count_changed = ng != ctx->ng || !ctx->host_halo;
   if (!ctx->host_halo) {
@@ -1103,6 +1129,7 @@ static int halo_unpack_impl(cv2_ctx *ctx, int32_t npackets, const int64_t *count
   if (!ctx->atoms) return fail(ctx, CV2_ERR_STATE, "upload owned atoms before ghosts");
   if (ctx->s.mol)
     return fail(ctx, CV2_ERR_STATE, "halo packets carry no molecule indices (ignore_intra_molecule is on)");
+  if (ctx->vstate) return fail(ctx, CV2_ERR_STATE, "CV2_NL_VERIFY does not check host-supplied halos");
   (void)hipSetDevice(ctx->device);
   const int64_t total = with_velocity ? 8 : 5;
   long long first[64], pbase[64], pn[64];
@@ -1375,7 +1402,7 @@ int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt) {
   for (int tries = 0; rc == CV2_ERR_CAPACITY && ctx->h_flags[F_OVERFLOW] && tries < 8; ++tries) {
     rc = grow_lists(ctx);
     if (rc) return rc;
-    launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, true);
+    build_checked(ctx, true);
     rc = check_flags(ctx);
   }
   if (rebuilt) *rebuilt = ctx->h_flags[F_REBUILDS] != before;
@@ -1396,7 +1423,7 @@ int forces_pass(cv2_ctx *ctx, bool energy) {
     if (rc != CV2_ERR_CAPACITY || !ctx->h_flags[F_OVERFLOW] || tries == 8) return rc;
     rc = grow_lists(ctx);
     if (rc) return rc;
-    launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, false);
+    build_checked(ctx, false);
   }
 }
 }  // namespace
@@ -1609,7 +1636,7 @@ int cv2_step_end(cv2_ctx *ctx, int energy) {
     if (rc != CV2_ERR_CAPACITY || !ctx->h_flags[F_OVERFLOW] || tries == 8) break;
     rc = grow_lists(ctx);
     if (rc) return rc;
-    launch_build(ctx->s, ctx->b, ctx->stream, ctx->host_halo, false);
+    build_checked(ctx, false);
   }
   if (rc) return rc;
   ctx->step_open = false;
@@ -1789,6 +1816,16 @@ int cv2_get_stats(cv2_ctx *ctx, cv2_stats *out) {
   out->precision = ctx->precision;
   out->build_kind = ctx->b.coarse ? CV2_BUILD_CELL_WAVE : CV2_BUILD_PER_ATOM;
   out->force_tasks = ctx->s.slots ? ctx->h_flags[F_NTASK] : 0;
+  if (ctx->vstate) {
+    unsigned long long v[8];
+    HIPCHK(hipMemcpy(v, ctx->vstate, sizeof(v), hipMemcpyDeviceToHost));
+    out->verified_builds = (int64_t)v[6];
+    out->verify_mismatches = (int64_t)v[7];
+    out->verify_sum[0] = v[2];
+    out->verify_pairs[0] = (int64_t)v[3];
+    out->verify_sum[1] = v[4];
+    out->verify_pairs[1] = (int64_t)v[5];
+  }
   return CV2_OK;
 }
 

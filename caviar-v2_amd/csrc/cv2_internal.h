@@ -582,6 +582,12 @@ void launch_halo_unpack(DevState &s, const double *in, int npk, const long long 
 void launch_ghost_place(DevState &s, const double4 *gpos_in, const int *gid_in, int ng, bool remap,
                         hipStream_t st);
 
+// Independent check of each list build (k_verify.hip, CV2_NL_VERIFY): vs
+// holds verify_state_words() words (zeroed at allocation).
+int verify_state_words();
+void launch_verify(const DevState &s, const BoxDev &b, unsigned long long *vs, hipStream_t st);
+void launch_verify_arm(const int *flags, unsigned long long *vs, hipStream_t st);
+
 // Owned-atom exchange (k_migrate.hip).  selfmask bit d: direction d leads
 // back to this domain.  classify: direction codes by upload index (13 =
 // staying), per-tile counts, *moved |= 1 when an atom is shifted back into

@@ -17,6 +17,7 @@ CV2_PREC_FP64 = 0
 CV2_PREC_FP32_FIXED = 1
 CV2_NL_REBUILD_EVERY_STEP = 1
 CV2_NL_REFERENCE_REBUILDS = 2
+CV2_NL_VERIFY = 4
 
 ERRORS = {1: "HIP", 2: "BAD_ARG", 3: "NAN", 4: "FIXED_OVERFLOW", 5: "CAPACITY", 6: "STATE"}
 
@@ -29,7 +30,9 @@ class Stats(C.Structure):
     _fields_ = [("natoms", C.c_int64), ("nghost", C.c_int64), ("steps", C.c_int64),
                 ("rebuilds", C.c_int64), ("list_entries", C.c_int64), ("max_neighbors", C.c_int32),
                 ("capacity", C.c_int32), ("ncell", C.c_int32 * 3), ("precision", C.c_int32),
-                ("build_kind", C.c_int32), ("force_tasks", C.c_int32)]
+                ("build_kind", C.c_int32), ("force_tasks", C.c_int32), ("verified_builds", C.c_int64),
+                ("verify_mismatches", C.c_int64), ("verify_sum", C.c_uint64 * 2),
+                ("verify_pairs", C.c_int64 * 2)]
 
 
 class BuildFilter(C.Structure):
@@ -224,12 +227,12 @@ class Context:
 
     @classmethod
     def from_sysdef(cls, s: dict, precision=CV2_PREC_FP64, device=0, rebuild_every_step=False,
-                    reference_rebuilds=False):
+                    reference_rebuilds=False, verify=False):
         c = cls(device)
         c.set_box(s["lo"], s["hi"], s.get("bc", (1, 1, 1)), s["ghost_cutoff"])
         c.set_lj(s["eps"], s["sigma"], s["lj_cutoff"], s["mass"], s.get("cutoff_list"))
         flags = (CV2_NL_REBUILD_EVERY_STEP if rebuild_every_step else 0) | \
-            (CV2_NL_REFERENCE_REBUILDS if reference_rebuilds else 0)
+            (CV2_NL_REFERENCE_REBUILDS if reference_rebuilds else 0) | (CV2_NL_VERIFY if verify else 0)
         c.set_neighbor(s["nl_cutoff"], s["nl_coef"], flags)
         c.set_precision(precision)
         c.upload(s["pos"], s.get("vel"), s.get("type"))
@@ -393,7 +396,9 @@ class Context:
                     list_entries=s.list_entries, max_neighbors=s.max_neighbors, capacity=s.capacity,
                     ncell=tuple(s.ncell), precision=s.precision,
                     build_kind=('cell_wave' if s.build_kind == 1 else 'per_atom'),
-                    force_tasks=s.force_tasks)
+                    force_tasks=s.force_tasks, verified_builds=s.verified_builds,
+                    verify_mismatches=s.verify_mismatches, verify_sum=tuple(s.verify_sum),
+                    verify_pairs=tuple(s.verify_pairs))
 
     def set_timing(self, enable=True, period=1):
         """HIP-event timing: every kernel group (period 1), or only the force

@@ -36,6 +36,10 @@ class Neighborlist {
   // triggers, an atom's wrap and a changed ghost set, so that the rebuild
   // steps are the reference's (the device's images need neither)
   void set_reference_rebuilds(bool v) { reference_rebuilds = v; }
+  // Addition (default false): check every list build on the device against
+  // an all-pairs recomputation of the pair set (O(N^2) per build, for
+  // tests); a build that differs stops the run with an error
+  void set_verify_builds(bool v) { verify_builds = v; }
   void set_dt(double v);
 
   // neighborlist.cpp:303-390 (cutoff_extra and the rebuild threshold)
@@ -51,6 +55,7 @@ class Neighborlist {
   bool all_atom_test = false;
   bool rebuild_test = false;
   bool reference_rebuilds = false;
+  bool verify_builds = false;
   double cutoff = 0.0;
   double cutoff_extra = 0.0;
   double cutoff_extra_coef = 1.12347;

@@ -118,7 +118,8 @@ void configure(DeviceSession &s, Precision precision, int device) {
   Neighborlist &nl = cv->neighborlist;
   s.check(cv2_set_neighbor(s.ctx, nl.cutoff, nl.cutoff_extra_coef,
                            (nl.rebuild_test ? CV2_NL_REBUILD_EVERY_STEP : 0u) |
-                               (nl.reference_rebuilds ? CV2_NL_REFERENCE_REBUILDS : 0u)),
+                               (nl.reference_rebuilds ? CV2_NL_REFERENCE_REBUILDS : 0u) |
+                               (nl.verify_builds ? CV2_NL_VERIFY : 0u)),
           "cv2_set_neighbor");
   s.check(cv2_set_precision(s.ctx, precision == Precision::fp64 ? CV2_PREC_FP64 : CV2_PREC_FP32_FIXED),
           "cv2_set_precision");
@@ -608,6 +609,14 @@ void Md_simulator::advance(int64_t nsteps) {
     caviar_->log.error_all(FC_FILE_LINE_FUNC, "an atom has NaN acceleration");
   s->check(rc, "device step");
   s->device_current = true;
+  if (caviar_->neighborlist.verify_builds) {
+    cv2_stats st;
+    s->check(cv2_get_stats(s->ctx, &st), "cv2_get_stats");
+    if (st.verify_mismatches != 0)
+      caviar_->log.error_all(FC_FILE_LINE_FUNC, std::to_string(st.verify_mismatches) + " of " +
+                                                    std::to_string(st.verified_builds) +
+                                                    " neighbour list builds differ from the all-pairs check");
+  }
 }
 
 void Md_simulator::write_outputs(int64_t step_i, double t) {

@@ -44,6 +44,16 @@ extern "C" {
  * atom crosses a face on almost every step, so the reference rebuilds on
  * almost every step; the pair set within the cutoff does not depend on it. */
 #define CV2_NL_REFERENCE_REBUILDS 2u
+/* Check every list build: right after each build the device recomputes the
+ * pair set from scratch -- every ordered pair of owned atoms at the minimum
+ * periodic image, kept where the reference's single-domain ghost of that
+ * image exists, the FP64 predicate of neighborlist.cpp:261-266, no cells,
+ * coarse filter or fixed point -- and compares a 64-bit checksum and the
+ * count of its pairs with those of the built list (cv2_stats
+ * verified_builds, verify_mismatches).  O(N^2) per build: for tests.  The
+ * list is kept in its 32-bit form while it is on; host-supplied halos are
+ * refused. */
+#define CV2_NL_VERIFY 4u
 
 typedef struct cv2_ctx cv2_ctx;
 
@@ -58,6 +68,15 @@ typedef struct {
   int32_t precision;
   int32_t build_kind;       /* CV2_BUILD_CELL_WAVE or CV2_BUILD_PER_ATOM */
   int32_t force_tasks;      /* force tasks of the current build (FP32 form, wave-build boxes), else 0 */
+  int64_t verified_builds;  /* CV2_NL_VERIFY: builds checked */
+  int64_t verify_mismatches; /* CV2_NL_VERIFY: checked builds whose pair set differed */
+  /* CV2_NL_VERIFY, the last checked build: [0] the built list, [1] the
+   * recomputation -- sum over ordered pairs (i, j, image) of
+   * splitmix64((id_i << 38) ^ (id_j << 6) ^ image), image = (vx + 1) |
+   * (vy + 1) << 2 | (vz + 1) << 4 for j's position x_j - v L, and the count
+   * of pairs */
+  uint64_t verify_sum[2];
+  int64_t verify_pairs[2];
 } cv2_stats;
 
 /* list build in use (cv2_stats.build_kind): one wavefront per cell with a

@@ -71,6 +71,8 @@ int main(int argc, char **argv) {
     const bool lf = std::strcmp(argv[3], "lf") == 0;
     const long steps = std::atol(argv[4]);
     const int cad = std::atoi(argv[5]);
+    // optional: every list build of the run checked on the device
+    sim.neighborlist.set_verify_builds(argc > 6 && std::strcmp(argv[6], "verify") == 0);
     writer::Atom_writer out(&sim);
     out.set_energy_step(cad);
     out.set_temperature_step(cad);

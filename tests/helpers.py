@@ -81,3 +81,33 @@ def force_error(a, ref):
     m = np.linalg.norm(ref, axis=1)
     rms = np.sqrt((m ** 2).mean())
     return float((d / np.maximum(m, rms)).max()), float(np.sqrt((d ** 2).mean()) / rms)
+
+
+_U = np.uint64
+
+
+def splitmix64(z):
+    """The splitmix64 output function of state + 0x9e3779b97f4a7c15
+    (Steele, Lea and Flood, OOPSLA 2014), on uint64 arrays, wrapping."""
+    z = np.asarray(z, np.uint64) + _U(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> _U(30))) * _U(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> _U(27))) * _U(0x94D049BB133111EB)
+    return z ^ (z >> _U(31))
+
+
+def pair_checksum(rows, ids=None):
+    """(sum mod 2^64, count) over the ordered pairs of a half-list pair set
+    (i, j, image_code) rows, as CV2_NL_VERIFY reports them: owned pairs
+    (code 13) count in both directions, image pairs as listed; a pair adds
+    splitmix64((id_i << 38) ^ (id_j << 6) ^ image) with image = (vx + 1) |
+    (vy + 1) << 2 | (vz + 1) << 4 and code = (vx + 1) + 3 (vy + 1) + 9 (vz + 1)."""
+    a = np.asarray(rows, np.int64).reshape(-1, 3)
+    own = a[:, 2] == 13
+    i = np.concatenate([a[:, 0], a[own, 1]])
+    j = np.concatenate([a[:, 1], a[own, 0]])
+    c = np.concatenate([a[:, 2], a[own, 2]])
+    if ids is not None:
+        i, j = np.asarray(ids, np.int64)[i], np.asarray(ids, np.int64)[j]
+    img = (c % 3) | ((c // 3 % 3) << 2) | ((c // 9) << 4)
+    key = (i.astype(np.uint64) << _U(38)) ^ (j.astype(np.uint64) << _U(6)) ^ img.astype(np.uint64)
+    return int(np.add.reduce(splitmix64(key), dtype=np.uint64)) if len(key) else 0, len(key)

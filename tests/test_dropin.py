@@ -163,13 +163,16 @@ def test_writers_and_leapfrog_match_oracle(integ, tmp_path):
     - kinetic energy, temperature, pressure (kinetic_energy.cpp:206-303),
       potential energy and positions agree with the oracle's run;
     - Leap_frog (kick-drift-kick, md_simulator.cpp:871-980) gives the
-      velocity-Verlet trajectory."""
+      velocity-Verlet trajectory;
+    - the velocity-Verlet run has Neighborlist::set_verify_builds on: every
+      list build is checked on the device and none differs (a difference
+      would end the run with an error)."""
     s = _small_fluid()
     inp = tmp_path / "in.txt"
     _write_input(inp, s)
     exe = _compile(os.path.join(ROOT, "tests", "cpp", "dropin_checks.cpp"), str(tmp_path / "dc"))
-    r = subprocess.run([exe, "run", str(inp), integ, "50", "10"], capture_output=True, text=True,
-                       cwd=tmp_path, timeout=120)
+    r = subprocess.run([exe, "run", str(inp), integ, "50", "10"] + (["verify"] if integ == "vv" else []),
+                       capture_output=True, text=True, cwd=tmp_path, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     # the final state: lines of six hex floats (the log's info lines share stdout)
     rows = [l.split() for l in r.stdout.split("\n")]

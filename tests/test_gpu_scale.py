@@ -3,6 +3,10 @@
 256,000-atom FP64 LJ fluid (40^3 FCC cells, rho* = 0.8442, T* = 1.44, rc 2.5,
 skin 0.3, velocity Verlet dt = 0.005, 10^4 steps):
 
+- Every list build of the run is checked on the device (CV2_NL_VERIFY: an
+  all-pairs FP64 recomputation of the pair set, compared by checksum and
+  count; the check itself is pinned against the oracle in
+  test_gpu_verify.py): no build of the 10^4 steps differs.
 - Pair sets are bit-exact against the oracle at the device's positions of
   each of the first rebuilds, and after the 10^4 steps (a build forced at
   the final positions).  Trajectories of the two diverge, so each check is a
@@ -52,7 +56,7 @@ def test_256k_pairs_at_rebuilds_and_energy():
     s = lj_fluid(40, temperature=1.44, seed=1002)
     n = len(s["pos"])
     assert n == 256000
-    with D.Context.from_sysdef(s, precision=D.CV2_PREC_FP64) as c:
+    with D.Context.from_sysdef(s, precision=D.CV2_PREC_FP64, verify=True) as c:
         c.compute_forces()
         check_pairs(c, s, "setup")
         e = [energy(c)]
@@ -75,6 +79,11 @@ def test_256k_pairs_at_rebuilds_and_energy():
         rebuilds = c.stats()["rebuilds"]
         c.build(True)
         check_pairs(c, s, f"after {STEPS} steps ({rebuilds} rebuilds)")
+        st = c.stats()
+    print(f"[c2] {st['verified_builds']} of {st['rebuilds']} builds checked on the device, "
+          f"{st['verify_mismatches']} differed", flush=True)
+    assert st["verified_builds"] == st["rebuilds"] == rebuilds + 1
+    assert st["verify_mismatches"] == 0
     e = np.asarray(e)
     de = np.abs(e - e[0]).max() / n
     print(f"[c2] E0 {e[0]:.4f}, max |E - E0| per atom {de:.3e}", flush=True)
