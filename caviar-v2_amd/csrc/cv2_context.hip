@@ -86,6 +86,7 @@ struct cv2_ctx {
   int64_t pos_cap = 0;  // capacity of the owned+ghost position arrays
   bool host_halo = false;
   bool forces_valid = false;
+  bool energy_ok = false;  // pe and virial are those of the current positions
   bool need_build = true;  // configuration changed since the last forced build
   bool atoms = false;
   int64_t steps = 0;
@@ -575,7 +576,7 @@ int adopt_owned(cv2_ctx *ctx, int64_t n) {
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->need_build = true;
-  if (!ctx->step_open) ctx->forces_valid = false;
+  if (!ctx->step_open) ctx->forces_valid = ctx->energy_ok = false;
   return CV2_OK;
 }
 
@@ -806,7 +807,7 @@ int cv2_set_box(cv2_ctx *ctx, const double lo[3], const double hi[3], const int 
   ctx->ghost_cut = ghost_cutoff;
   ctx->box_set = true;
   ctx->need_build = true;
-  ctx->forces_valid = false;
+  ctx->forces_valid = ctx->energy_ok = false;
   if (ctx->nl_set) return setup_box(ctx);
   return CV2_OK;
 }
@@ -825,7 +826,7 @@ int cv2_set_lj(cv2_ctx *ctx, int ntypes, const double *eps, const double *sigma,
   }
   ctx->lj_set = true;
   ctx->need_build = true;
-  ctx->forces_valid = false;
+  ctx->forces_valid = ctx->energy_ok = false;
   hipError_t e = hipMemcpyAsync(ctx->lj_dev, &ctx->lj, sizeof(LJDev), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return fail(ctx, CV2_ERR_HIP, hipGetErrorString(e));
@@ -835,7 +836,7 @@ int cv2_set_lj(cv2_ctx *ctx, int ntypes, const double *eps, const double *sigma,
 int cv2_set_jump_tol(cv2_ctx *ctx, double jump_tol) {
   if (!ctx) return CV2_ERR_BAD_ARG;
   ctx->lj.jump_tol = jump_tol > 0.0 ? jump_tol : 0.0;
-  ctx->forces_valid = false;
+  ctx->forces_valid = ctx->energy_ok = false;
   if (!ctx->lj_set) return CV2_OK;  // uploaded with the LJ table
   hipError_t e = hipMemcpyAsync(ctx->lj_dev, &ctx->lj, sizeof(LJDev), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -867,7 +868,7 @@ int cv2_set_molecules(cv2_ctx *ctx, int64_t n, const int32_t *molecule_index,
   }
   // the list form follows (32-bit entries while the flag is on)
   ctx->need_build = true;
-  ctx->forces_valid = false;
+  ctx->forces_valid = ctx->energy_ok = false;
   return CV2_OK;
 }
 
@@ -894,7 +895,7 @@ int cv2_set_neighbor(cv2_ctx *ctx, double cutoff, double extra_coef, unsigned fl
   ctx->nl_flags = flags;
   ctx->nl_set = true;
   ctx->need_build = true;
-  ctx->forces_valid = false;
+  ctx->forces_valid = ctx->energy_ok = false;
   if (ctx->box_set) return setup_box(ctx);
   return CV2_OK;
 }
@@ -905,7 +906,7 @@ int cv2_set_precision(cv2_ctx *ctx, int precision) {
     return fail(ctx, CV2_ERR_BAD_ARG, "unknown precision");
   ctx->precision = precision;
   ctx->s.write_q = precision != CV2_PREC_FP64;
-  ctx->forces_valid = false;
+  ctx->forces_valid = ctx->energy_ok = false;
   ctx->need_build = true;  // the list form follows the precision
   return CV2_OK;
 }
@@ -983,7 +984,7 @@ int cv2_upload_atoms(cv2_ctx *ctx, int64_t n, const double *pos, const double *v
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->atoms = true;
   ctx->need_build = true;
-  ctx->forces_valid = false;
+  ctx->forces_valid = ctx->energy_ok = false;
   ctx->step_open = false;
   ctx->steps = 0;
   ctx->own_cap = n;
@@ -1396,7 +1397,7 @@ int cv2_build_neighbors(cv2_ctx *ctx, int force, int *rebuilt) {
   if (!force) launch_check_disp(ctx->s, ctx->b, ref_rebuilds(ctx), ctx->stream);
   rc = queue_build(ctx, force != 0, true);
   if (rc) return rc;
-  ctx->forces_valid = false;
+  ctx->forces_valid = ctx->energy_ok = false;
   rc = check_flags(ctx);
   // a list cut at its capacity: grow the lists and build again
   for (int tries = 0; rc == CV2_ERR_CAPACITY && ctx->h_flags[F_OVERFLOW] && tries < 8; ++tries) {
@@ -1441,9 +1442,30 @@ int cv2_compute_forces(cv2_ctx *ctx) {
   if (rc) return rc;
   rc = fetch_energy(ctx);
   if (rc) return rc;
-  ctx->forces_valid = true;
+  ctx->forces_valid = ctx->energy_ok = true;
   return CV2_OK;
 }
+
+namespace {
+// Potential energy and virial at the current positions, for cv2_thermo,
+// when the last force pass did not sum them: one energy pass whose
+// accelerations go to scratch, so that the state keeps the bits the step
+// loop computed (on the FP32 path the energy kernel tests the cutoff in
+// FP64 and the step kernel on r^-2 in FP32).
+int energy_pass(cv2_ctx *ctx) {
+  if (!ctx->forces_valid || ctx->need_build) return cv2_compute_forces(ctx);
+  if (ctx->precision != CV2_PREC_FP64) launch_pack_q(ctx->s, ctx->b, ctx->stream);
+  DevState tmp = ctx->s;
+  tmp.acc = ctx->s.acc_alt;
+  launch_forces(tmp, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, true, ctx->stream);
+  int rc = check_flags(ctx);
+  if (rc) return rc;
+  rc = fetch_energy(ctx);
+  if (rc) return rc;
+  ctx->energy_ok = true;
+  return CV2_OK;
+}
+}  // namespace
 
 int cv2_compute_accelerations(cv2_ctx *ctx) {
   if (!ctx) return CV2_ERR_BAD_ARG;
@@ -1456,14 +1478,14 @@ int cv2_compute_accelerations(cv2_ctx *ctx) {
   }
   rc = forces_pass(ctx, false);
   if (rc) return rc;
-  ctx->forces_valid = true;  // the energy and virial of cv2_thermo stay those of the last energy pass
+  ctx->forces_valid = true;
   return CV2_OK;
 }
 
 // Shared step loop.  On entry the forces of the current positions are
 // known; each iteration closes the previous step and opens the next one in
 // one integrator pass, runs the (device-conditional) list build, and
-// computes forces (energy and virial on the last step only).  Where the
+// computes forces (no energy: cv2_thermo sums it when asked).  Where the
 // FP32 task kernel runs, the integrator pass of the next iteration is fused
 // into its epilogue (launch_forces_fused), so that iteration starts with its
 // build.
@@ -1503,7 +1525,7 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
         }
       }
       if (!opened)
-        launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, k + 1 == nsteps, ctx->stream);
+        launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, false, ctx->stream);
       span_end(ctx, t, 0);
       if (ctx->spans.size() > 4000) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1534,7 +1556,9 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
     break;
   }
   ctx->steps += nsteps;
-  if (nsteps > 0) return fetch_energy(ctx);
+  // the potential energy and virial of the new positions are summed when
+  // cv2_thermo asks for them (energy_pass)
+  if (nsteps > 0) ctx->energy_ok = false;
   return CV2_OK;
 }
 
@@ -1607,7 +1631,7 @@ int cv2_step_begin(cv2_ctx *ctx, double dt, int langevin, double gamma, double k
   launch_update(ctx->s, ctx->b, u, ctx->stream);
   ctx->open_args = u;
   ctx->step_open = true;
-  ctx->forces_valid = false;
+  ctx->forces_valid = ctx->energy_ok = false;
   // no host round trip: a NaN raised here halts the device and is reported
   // by cv2_step_end
   return CV2_OK;
@@ -1646,6 +1670,7 @@ int cv2_step_end(cv2_ctx *ctx, int energy) {
     if (rc) return rc;
   }
   ctx->forces_valid = true;
+  ctx->energy_ok = energy != 0;
   return CV2_OK;
 }
 
@@ -1664,6 +1689,11 @@ int cv2_thermo(cv2_ctx *ctx, int n_r_df, double *ke, double *pe, double *virial)
     if (n_r_df >= 1 && n_r_df <= 3 && r[4] > 0.0)  // kinetic_energy.cpp:56-70
       e -= (r[1] * r[1] + r[2] * r[2] + r[3] * r[3]) / r[4];
     *ke = 0.5 * e;
+  }
+  // (while a step is open: those of the last energy pass)
+  if ((pe || virial) && !ctx->step_open && !(ctx->energy_ok && ctx->forces_valid)) {
+    rc = energy_pass(ctx);
+    if (rc) return rc;
   }
   if (pe) *pe = ctx->pe;
   if (virial) *virial = ctx->virial;

@@ -287,15 +287,17 @@ int cv2_compute_forces(cv2_ctx *ctx);
 
 /* Accelerations only, through the same force kernel the step loop runs
  * (for CV2_PREC_FP32_FIXED on wave-build boxes: the branchless pair
- * arithmetic with the cutoff tested on r^-2 in FP32); the potential energy
- * and virial that cv2_thermo reports stay those of the last
- * cv2_compute_forces.  Same reference counterpart as cv2_compute_forces
- * (lj.cpp:393-517 without the virial sum of :487-490,516). */
+ * arithmetic with the cutoff tested on r^-2 in FP32); cv2_thermo sums the
+ * potential energy and virial when asked.  Same reference counterpart as
+ * cv2_compute_forces (lj.cpp:393-517 without the virial sum of
+ * :487-490,516). */
 int cv2_compute_accelerations(cv2_ctx *ctx);
 
 /* nsteps of velocity Verlet.  Md_simulator::integrate_velocity_verlet
  * (md_simulator.cpp:617-729) including re_calculate_acc (:422-437).  The
- * state must be consistent (forces computed) on entry; it is on exit. */
+ * state must be consistent (forces computed) on entry; it is on exit.  No
+ * force pass of the run sums the potential energy: cv2_thermo does, at the
+ * positions the run ends on, when asked. */
 int cv2_run_vv(cv2_ctx *ctx, int64_t nsteps, double dt);
 
 /* nsteps of the Langevin integrator of Md_simulator::
@@ -318,16 +320,19 @@ int cv2_run_langevin(cv2_ctx *ctx, int64_t nsteps, double dt, double gamma, doub
  * the second velocity half (:678 / :860).  Together they give the bits of
  * cv2_run_vv / cv2_run_langevin with nsteps = 1.  cv2_run_* refuse to run
  * while a step is open.  energy != 0: the force pass also sums the potential
- * energy and virial that cv2_thermo reports (the last step of cv2_run_*);
- * energy = 0: the force pass of the other steps of a run, and cv2_thermo's
- * energy and virial stay those of the last energy pass.  A NaN raised by
+ * energy and virial that cv2_thermo reports; energy = 0: the force pass of
+ * the steps of a run, and cv2_thermo sums them when asked.  A NaN raised by
  * cv2_step_begin's half is reported by cv2_step_end. */
 int cv2_step_begin(cv2_ctx *ctx, double dt, int langevin, double gamma, double kbt, uint64_t seed);
 int cv2_step_end(cv2_ctx *ctx, int energy);
 
 /* Thermodynamics of the current state: kinetic energy (kinetic_energy.cpp:
- * 26-70, n_r_df as set), potential energy and virial of the last force pass
- * (Atom_data::virialForce, lj.cpp:487-490,516).  Any pointer may be NULL. */
+ * 26-70, n_r_df as set), potential energy and virial (Atom_data::
+ * virialForce, lj.cpp:487-490,516) of the current positions: when the last
+ * force pass did not sum them (a run, cv2_compute_accelerations,
+ * cv2_step_end with energy = 0), an energy pass does, without changing the
+ * state's accelerations; while a step is open, those of the last energy
+ * pass.  Any pointer may be NULL. */
 int cv2_thermo(cv2_ctx *ctx, int n_r_df, double *ke, double *pe, double *virial);
 
 /* Multiply every owned velocity by lambda: the velocity rescale of the NVE
