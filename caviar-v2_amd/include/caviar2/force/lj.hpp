@@ -15,6 +15,11 @@ class Lj : public Force_field {
   explicit Lj(Caviar2 *c);
   ~Lj() override = default;
 
+  // lj.cpp:180-391: per-atom parameters mixed into the tables (sigma_ij =
+  // (sigma_i + sigma_j) / 2, epsilon_ij = sqrt(epsilon_i epsilon_j)), tables
+  // sized to the atom types present, WCA cutoffs 2^(1/6) sigma_ij, then the
+  // lambda_s / lambda_e scalings.  The reference's Md_simulator does not
+  // call it; a program that does gets the tables it forms on the device.
   void verify_settings() override;
   void calculate_acceleration() override;
   double energy() override;
@@ -46,6 +51,10 @@ class Lj : public Force_field {
   std::vector<double> epsilon_atom, sigma_atom;
   bool cutoff_list_activated = false, wca = false, jump_fix = false, monitor_jump = false;
   bool ignore_intra_molecule = false, make_off_diagonal_vectors = false;
+  // which inputs were given (lj.cpp:28-176); verify_settings forms the
+  // tables from them
+  bool input_by_array = false, input_by_atom = false, lambda_s_is_set = false,
+       lambda_e_is_set = false;
   double jump_tol = 1e-6;
 };
 

@@ -424,10 +424,7 @@ void Lj::set_jump_tol(double v) {
 void Lj::ignore_intra_molecule_flag(bool f) { ignore_intra_molecule = f; }
 void Lj::enable_wca(bool f) {
   wca = f;
-  if (f) {
-    cutoff_list_activated = true;
-    caviar_->log.warning("Lj::enable_wca: WCA cutoffs come from verify_settings, which the reference never calls");
-  }
+  if (f) cutoff_list_activated = true;  // the cutoffs are formed by verify_settings
 }
 void Lj::set_cutoff_list(const std::vector<std::vector<double>> &values) {
   for (auto &v : values)
@@ -440,34 +437,103 @@ void Lj::set_epsilon(const int t1, const int t2, const double value) {
   if ((int)epsilon.size() < t1 + 1) epsilon.resize(t1 + 1);
   if ((int)epsilon[t1].size() < t2 + 1) epsilon[t1].resize(t2 + 1, 0.0);
   epsilon[t1][t2] = value;
+  input_by_array = true;
 }
 void Lj::set_sigma(const int t1, const int t2, const double value) {
   if ((int)sigma.size() < t1 + 1) sigma.resize(t1 + 1);
   if ((int)sigma[t1].size() < t2 + 1) sigma[t1].resize(t2 + 1, 0.0);
   sigma[t1][t2] = value;
+  input_by_array = true;
 }
 void Lj::set_lambda_s(const std::vector<std::vector<double>> &v) {
   for (auto &r : v)
     for (double x : r)
       if (x < 0.0) throw std::runtime_error("Lambda_s values must be non-negative.");
   lambda_s = v;
+  lambda_s_is_set = true;
 }
 void Lj::set_lambda_e(const std::vector<std::vector<double>> &v) {
   for (auto &r : v)
     for (double x : r)
       if (x < 0.0) throw std::runtime_error("Lambda_e values must be non-negative.");
   lambda_e = v;
+  lambda_e_is_set = true;
 }
 void Lj::set_epsilon_atom(const int t, const double v) {
   if ((int)epsilon_atom.size() < t + 1) epsilon_atom.resize(t + 1, 0.0);
   epsilon_atom[t] = v;
+  input_by_atom = true;
 }
 void Lj::set_sigma_atom(const int t, const double v) {
   if ((int)sigma_atom.size() < t + 1) sigma_atom.resize(t + 1, 0.0);
   sigma_atom[t] = v;
+  input_by_atom = true;
 }
 
-void Lj::verify_settings() {}
+void Lj::verify_settings() {
+  if (input_by_atom && input_by_array) caviar_->log.error_all(FC_FILE_LINE_FUNC, "(input_by_atom && input_by_array)");
+  // every table covers the types of the owned atoms
+  size_t ntypes = 0;
+  for (auto t : atom_data->atom_struct_owned.type) ntypes = std::max(ntypes, (size_t)t + 1);
+  auto square = [](std::vector<std::vector<double>> &m, size_t k, double fill) {
+    if (m.size() < k) m.resize(k);
+    for (auto &r : m)
+      if (r.size() < k) r.resize(k, fill);
+  };
+  size_t k = 0;
+  if (input_by_atom) {  // lj.cpp:204-258
+    if (epsilon_atom.size() != sigma_atom.size()) caviar_->log.warning("lj:: (epsilon_atom_size != sigma_atom_size)");
+    k = std::max(epsilon_atom.size(), sigma_atom.size());
+    if (k < ntypes) {
+      caviar_->log.warning("lj:: (atom_max_size < atom_data_type_max + 1)");
+      k = ntypes;
+    }
+    epsilon_atom.resize(k, 0.0);
+    sigma_atom.resize(k, 0.0);
+    epsilon.assign(k, std::vector<double>(k, 0.0));
+    sigma.assign(k, std::vector<double>(k, 0.0));
+    for (size_t i = 0; i < k; ++i)
+      for (size_t j = 0; j < k; ++j) {
+        // the diagonal takes the atom values as given (no mixing round-off)
+        sigma[i][j] = i == j ? sigma_atom[i] : 0.5 * (sigma_atom[i] + sigma_atom[j]);
+        epsilon[i][j] = i == j ? epsilon_atom[i] : std::sqrt(epsilon_atom[i] * epsilon_atom[j]);
+      }
+  } else if (input_by_array) {  // lj.cpp:260-309
+    if (epsilon.size() != sigma.size()) caviar_->log.warning("lj:: (epsilon_size != sigma_size)");
+    k = std::max(epsilon.size(), sigma.size());
+    if (k < ntypes) {
+      caviar_->log.warning("lj:: (es_max_size < atom_data_type_max + 1)");
+      k = ntypes;
+    }
+    square(epsilon, k, 0.0);
+    square(sigma, k, 0.0);
+  }
+  if (wca) {  // lj.cpp:312-328: cutoffs at the potential's minimum
+    const double coef = std::pow(2.0, 1.0 / 6.0);
+    cutoff_list.assign(k, std::vector<double>(k, 0.0));
+    for (size_t i = 0; i < k; ++i)
+      for (size_t j = 0; j < k; ++j) {
+        cutoff_list[i][j] = coef * sigma[i][j];
+        caviar_->log.info("lj wca cutoff " + std::to_string(i) + "," + std::to_string(j) + " : " +
+                          std::to_string(cutoff_list[i][j]));
+      }
+  } else if (cutoff_list_activated) {  // lj.cpp:330-344
+    if (cutoff_list.size() < k) caviar_->log.warning("lj:: (cutoff_list.size() < atom_max_size)");
+    square(cutoff_list, k, 0.0);
+  }
+  // lj.cpp:347-383: scalings of sigma and epsilon, missing entries 1
+  auto scale = [&](std::vector<std::vector<double>> &lam, std::vector<std::vector<double>> &m) {
+    square(lam, ntypes, 1.0);
+    square(m, lam.size(), 0.0);
+    for (size_t i = 0; i < lam.size(); ++i)
+      for (size_t j = 0; j < lam[i].size(); ++j) {
+        if (m[i].size() <= j) m[i].resize(j + 1, 0.0);
+        m[i][j] *= lam[i][j];
+      }
+  };
+  if (lambda_s_is_set) scale(lambda_s, sigma);
+  if (lambda_e_is_set) scale(lambda_e, epsilon);
+}
 
 void Lj::tables(int nt, std::vector<double> &eps, std::vector<double> &sig, std::vector<double> &cl,
                 bool &has_cl) const {

@@ -349,6 +349,8 @@ def ref_lib():
         L.ref_sim_nlist.argtypes = [_vp, C.c_int64, _i64, _u32]; L.ref_sim_nlist.restype = C.c_int64
         L.ref_sim_set_molecules.argtypes = [_vp, _i32, C.c_int]
         L.ref_random_velocity.argtypes = [C.c_int64, C.c_uint, C.c_double, _d]
+        L.ref_lj_tables.argtypes = ([C.c_int64, _i32, C.c_int, _d, C.c_int, _d, C.c_int, _i32, _d, C.c_int,
+                                     _i32, _d, C.c_int] + [C.c_int, _i32, _d] * 3 + [C.c_int, _d, _d, _d])
         _ref = L
     return _ref
 
@@ -416,6 +418,39 @@ class RefSim(_SimBase):
             self.h = None
 
     __del__ = close
+
+
+def ref_lj_tables(types, k, eps_atom=(), sigma_atom=(), eps_set=(), sigma_set=(), wca=False,
+                  cutoff_list=None, lambda_s=None, lambda_e=None):
+    """Lj::verify_settings (lj.cpp:180-391) of the compiled reference on the
+    given inputs; the epsilon, sigma and cutoff-list tables it forms as dense
+    k x k arrays.  eps_set / sigma_set: (a, b, value) set_epsilon / set_sigma
+    calls; cutoff_list, lambda_s, lambda_e: ragged lists of rows."""
+    def rag(m):
+        if m is None:
+            return 0, None, None
+        ln = np.array([len(r) for r in m], np.int32)
+        v = _f64(np.array([x for r in m for x in r], np.float64)) if ln.sum() else _f64(np.zeros(1))
+        return len(m), ln, v
+    ty = np.ascontiguousarray(types, np.int32)
+    ea = _f64(np.array(eps_atom, np.float64)) if len(eps_atom) else None
+    sa = _f64(np.array(sigma_atom, np.float64)) if len(sigma_atom) else None
+    eab = np.array([(a, b) for a, b, _ in eps_set], np.int32).reshape(-1, 2)
+    ev = _f64(np.array([v for _, _, v in eps_set], np.float64))
+    sab = np.array([(a, b) for a, b, _ in sigma_set], np.int32).reshape(-1, 2)
+    sv = _f64(np.array([v for _, _, v in sigma_set], np.float64))
+    out = [np.zeros((k, k)) for _ in range(3)]
+    args = [len(ty), _p(ty, _i32), len(eps_atom), _p(ea, _d), len(sigma_atom), _p(sa, _d),
+            len(eps_set), _p(np.ascontiguousarray(eab), _i32), _p(ev, _d),
+            len(sigma_set), _p(np.ascontiguousarray(sab), _i32), _p(sv, _d), int(bool(wca))]
+    keep = []
+    for m in (cutoff_list, lambda_s, lambda_e):
+        nr, ln, v = rag(m)
+        keep += [ln, v]
+        args += [nr, _p(ln, _i32), _p(v, _d)]
+    args += [k] + [_p(o, _d) for o in out]
+    ref_lib().ref_lj_tables(*args)
+    return out
 
 
 def ref_random_velocity(n, seed, amplitude):

@@ -35,6 +35,22 @@ struct NlSpy : caviar2::Neighborlist {
   }
 };
 
+// Read access to Lj's private tables (epsilon, sigma, cutoff_list) through
+// pointers to members named in explicit template instantiations, where
+// access checking does not apply ([temp.spec.general]): the harness reads
+// what Lj::verify_settings formed without changing the reference.
+using Table = std::vector<std::vector<double>>;
+template <typename Tag, typename Tag::type M>
+struct MemberOf {
+  friend typename Tag::type member(Tag) { return M; }
+};
+struct LjEps { using type = Table caviar2::force_field::Lj::*; friend type member(LjEps); };
+struct LjSig { using type = Table caviar2::force_field::Lj::*; friend type member(LjSig); };
+struct LjCut { using type = Table caviar2::force_field::Lj::*; friend type member(LjCut); };
+template struct MemberOf<LjEps, &caviar2::force_field::Lj::epsilon>;
+template struct MemberOf<LjSig, &caviar2::force_field::Lj::sigma>;
+template struct MemberOf<LjCut, &caviar2::force_field::Lj::cutoff_list>;
+
 struct RefSim {
   caviar2::Caviar2 *cv = nullptr;
   caviar2::force_field::Lj *lj = nullptr;
@@ -249,6 +265,62 @@ void ref_random_velocity(int64_t n, unsigned seed, double amplitude, double *vel
   c.atom_data.add_type_mass(0, 1.0);
   c.atom_data.add_random_velocity(seed, amplitude);
   copy_vec(c.atom_data.atom_struct_owned.velocity, vel);
+}
+
+// Lj::verify_settings (lj.cpp:180-391) as a user program would call it:
+// owned atoms of the given types (the tables cover them), then the setters
+// in the order epsilon_atom / sigma_atom (per-atom input), set_epsilon /
+// set_sigma (a, b, value triples), enable_wca, set_cutoff_list, set_lambda_s,
+// set_lambda_e (ragged rows: lengths, values concatenated; none when the
+// row count is 0), verify_settings.  The tables it leaves are returned as
+// dense k x k (missing entries 0).  Both kinds of input together end the
+// process in the reference (error_all): callers do not pass both.
+static std::vector<std::vector<double>> ragged(int rows, const int32_t *len, const double *v) {
+  std::vector<std::vector<double>> m((size_t)rows);
+  for (int r = 0, at = 0; r < rows; ++r)
+    for (int k = 0; k < len[r]; ++k) m[(size_t)r].push_back(v[at++]);
+  return m;
+}
+
+void ref_lj_tables(int64_t n, const int32_t *type, int neatom, const double *eatom, int nsatom,
+                   const double *satom, int ne, const int32_t *eab, const double *ev, int ns,
+                   const int32_t *sab, const double *sv, int wca, int ncl, const int32_t *cl_len,
+                   const double *cl_v, int nls, const int32_t *ls_len, const double *ls_v, int nle,
+                   const int32_t *le_len, const double *le_v, int k, double *eps_out,
+                   double *sig_out, double *cl_out) {
+  caviar2::Caviar2 c;
+  c.domain.nprocs_x = c.domain.nprocs_y = c.domain.nprocs_z = 1;
+  c.domain.grid_index_x = c.domain.grid_index_y = c.domain.grid_index_z = 0;
+  c.domain.me = 0;
+  c.domain.nprocs = 1;
+  c.domain.set_lower_global(Vector3d<double>{0, 0, 0});
+  c.domain.set_upper_global(Vector3d<double>{10, 10, 10});
+  c.domain.set_boundary_condition(Vector3d<int>{1, 1, 1});
+  c.domain.generate();
+  c.atom_data.set_record_owned_position_old(false);
+  c.atom_data.set_record_owned_velocity_old(false);
+  c.atom_data.set_record_owned_acceleration_old(false);
+  for (int64_t i = 0; i < n; ++i)
+    c.atom_data.add_atom((size_t)i, (size_t)type[i], Vector3d<double>{0.5, 0.5, 0.5},
+                         Vector3d<double>{0, 0, 0});
+  caviar2::force_field::Lj lj(&c);
+  for (int t = 0; t < neatom; ++t) lj.set_epsilon_atom(t, eatom[t]);
+  for (int t = 0; t < nsatom; ++t) lj.set_sigma_atom(t, satom[t]);
+  for (int q = 0; q < ne; ++q) lj.set_epsilon(eab[2 * q], eab[2 * q + 1], ev[q]);
+  for (int q = 0; q < ns; ++q) lj.set_sigma(sab[2 * q], sab[2 * q + 1], sv[q]);
+  if (wca) lj.enable_wca(true);
+  if (ncl > 0) lj.set_cutoff_list(ragged(ncl, cl_len, cl_v));
+  if (nls > 0) lj.set_lambda_s(ragged(nls, ls_len, ls_v));
+  if (nle > 0) lj.set_lambda_e(ragged(nle, le_len, le_v));
+  lj.verify_settings();
+  auto dense = [k](const std::vector<std::vector<double>> &m, double *out) {
+    for (int a = 0; a < k; ++a)
+      for (int b = 0; b < k; ++b)
+        out[a * k + b] = (a < (int)m.size() && b < (int)m[(size_t)a].size()) ? m[(size_t)a][(size_t)b] : 0.0;
+  };
+  dense(lj.*member(LjEps{}), eps_out);
+  dense(lj.*member(LjSig{}), sig_out);
+  dense(lj.*member(LjCut{}), cl_out);
 }
 
 }  // extern "C"

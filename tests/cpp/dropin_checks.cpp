@@ -13,6 +13,14 @@
 //   dropin_checks neighlist <input> <all>
 //       build the list (Neighborlist::build(true)), materialise neighlist
 //       (all = 1: the all_atom_test form), print it and the ghosts
+//   dropin_checks ljtables <input> <atom|array|both>
+//       types i % 3, Lj parameters given per atom (with WCA and lambda
+//       scalings) or as arrays, Lj::verify_settings, print the 3 x 3
+//       epsilon, sigma and cutoff-list tables (no device)
+//   dropin_checks ljrun <input> <steps>
+//       types i % 2 (masses 1, 1.5), per-atom epsilon / sigma, WCA,
+//       lambda_e, Lj::verify_settings, run Velocity_verlet, print the final
+//       state
 //
 // Input: n L rc coef ghost_cutoff, then n lines "x y z vx vy vz" (box [0, L)^3,
 // periodic, one species eps = sigma = mass = 1).
@@ -21,6 +29,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <vector>
 
 #include "caviar2/caviar2.hpp"
 #include "caviar2/tools/all.hpp"
@@ -64,9 +73,70 @@ int main(int argc, char **argv) {
   load(sim, argv[2], rc, coef);
   force_field::Lj lj(&sim);
   lj.set_cutoff(rc);
-  lj.set_sigma(0, 0, 1.0);
-  lj.set_epsilon(0, 0, 1.0);
+  const bool own_params = mode == "ljtables" || mode == "ljrun";
+  if (!own_params) {
+    lj.set_sigma(0, 0, 1.0);
+    lj.set_epsilon(0, 0, 1.0);
+  }
   sim.add_force_field(&lj);
+  auto print_state = [&sim]() {
+    const auto &o = sim.atom_data.atom_struct_owned;
+    for (size_t i = 0; i < o.position.size(); ++i)
+      std::printf("%a %a %a %a %a %a\n", o.position[i].x, o.position[i].y, o.position[i].z,
+                  o.velocity[i].x, o.velocity[i].y, o.velocity[i].z);
+  };
+  if (mode == "ljtables") {
+    auto &ty = sim.atom_data.atom_struct_owned.type;
+    for (size_t i = 0; i < ty.size(); ++i) ty[i] = i % 3;
+    const std::string c = argv[3];
+    if (c == "atom" || c == "both") {
+      const double e[3] = {1.0, 0.5, 0.8}, sg[3] = {1.0, 0.88, 1.2};
+      for (int t = 0; t < 3; ++t) {
+        lj.set_epsilon_atom(t, e[t]);
+        lj.set_sigma_atom(t, sg[t]);
+      }
+      lj.enable_wca(true);
+      lj.set_lambda_s({{1.0, 0.9}, {0.9}});  // partial rows: the rest is 1
+      lj.set_lambda_e({{1.0, 0.5, 1.0}, {0.5, 1.0, 1.0}, {1.0, 1.0, 0.7}});
+    }
+    if (c == "array" || c == "both") {
+      lj.set_epsilon(0, 0, 1.0);
+      lj.set_epsilon(0, 1, 0.5);
+      lj.set_epsilon(1, 0, 0.5);
+      lj.set_epsilon(1, 1, 0.8);
+      lj.set_sigma(0, 0, 1.0);
+      lj.set_sigma(1, 1, 0.9);
+      lj.set_cutoff_list({{2.5}});
+    }
+    lj.verify_settings();
+    std::vector<double> e, sg, cl;
+    bool has_cl = false;
+    lj.tables(3, e, sg, cl, has_cl);
+    std::printf("has_cutoff_list %d\n", has_cl ? 1 : 0);
+    for (const auto *tab : {&e, &sg, &cl}) {
+      for (int k = 0; k < 9; ++k) std::printf("%a%c", (*tab)[k], k == 8 ? '\n' : ' ');
+    }
+    return 0;
+  }
+  if (mode == "ljrun") {
+    auto &ty = sim.atom_data.atom_struct_owned.type;
+    for (size_t i = 0; i < ty.size(); ++i) ty[i] = i % 2;
+    sim.atom_data.add_type_mass(1, 1.5);
+    lj.set_epsilon_atom(0, 1.0);
+    lj.set_epsilon_atom(1, 0.6);
+    lj.set_sigma_atom(0, 1.0);
+    lj.set_sigma_atom(1, 0.9);
+    lj.enable_wca(true);
+    lj.set_lambda_e({{1.0, 0.8}, {0.8, 1.0}});
+    lj.verify_settings();
+    sim.md_simulator.set_integrator_type(Integrator_t::Velocity_verlet);
+    sim.md_simulator.set_initial_step(0);
+    sim.md_simulator.set_final_step((size_t)std::atol(argv[3]));
+    sim.md_simulator.set_dt(0.005);
+    sim.md_simulator.run();
+    print_state();
+    return 0;
+  }
   if (mode == "run") {
     const bool lf = std::strcmp(argv[3], "lf") == 0;
     const long steps = std::atol(argv[4]);

@@ -107,12 +107,34 @@ def molecules(n):
     return np.where(m % 5 == 4, -1, m).astype(np.int32)
 
 
+# Lj::verify_settings inputs of tests/cpp/dropin_checks.cpp's ljtables mode
+# (12 owned atoms of types i % 3)
+LJ_ATOM = dict(eps_atom=(1.0, 0.5, 0.8), sigma_atom=(1.0, 0.88, 1.2), wca=True,
+               lambda_s=[[1.0, 0.9], [0.9]],
+               lambda_e=[[1.0, 0.5, 1.0], [0.5, 1.0, 1.0], [1.0, 1.0, 0.7]])
+LJ_ARRAY = dict(eps_set=[(0, 0, 1.0), (0, 1, 0.5), (1, 0, 0.5), (1, 1, 0.8)],
+                sigma_set=[(0, 0, 1.0), (1, 1, 0.9)], cutoff_list=[[2.5]])
+
+
+def lj_tables():
+    """The tables the reference's Lj::verify_settings forms: per-atom input
+    with WCA and ragged lambda rows ("atom"), arrays with a short cutoff list
+    ("array")."""
+    types = np.arange(12) % 3
+    d = {"types": types.astype(np.int32)}
+    for case, kw in (("atom", LJ_ATOM), ("array", LJ_ARRAY)):
+        e, sg, cl = O.ref_lj_tables(types, 3, **kw)
+        d[case + "_eps"], d[case + "_sigma"], d[case + "_cutoff_list"] = e, sg, cl
+    return "lj_tables", d
+
+
 def main(only=None):
     O.ref_lib()
     if only:  # regenerate the named fixtures, keep the others and their hashes
         gen = {"mol_traj": lambda: traj_fixture("mol_traj", lj_fluid(5, temperature=1.0, seed=1006), 20,
                                                 keep=(1, 10, 20), use_cell_list=True,
-                                                mol=molecules(500))}
+                                                mol=molecules(500)),
+               "lj_tables": lj_tables}
         mpath = os.path.join(HERE, "golden.json")
         meta = json.load(open(mpath))
         for name in only:
@@ -141,6 +163,7 @@ def main(only=None):
     fixtures.append(("random_velocity", dict(n=np.asarray(512), seed=np.asarray(1),
                                              amplitude=np.asarray(0.5),
                                              vel=O.ref_random_velocity(512, 1, 0.5))))
+    fixtures.append(lj_tables())
     manifest = {}
     for name, d in fixtures:
         path = os.path.join(HERE, name + ".npz")

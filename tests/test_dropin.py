@@ -287,3 +287,92 @@ def test_host_neighlist_matches_oracle(all_pairs, tmp_path):
     offs, nbr = O.nlist_brute(s["pos"], g_ref, s["nl_cutoff"], s["nl_coef"])
     for i, l in enumerate(lists):
         assert l == sorted(int(x) for x in nbr[offs[i]:offs[i + 1]])
+
+
+# ---- Lj::verify_settings: per-atom mixing, WCA, lambda scalings --------------
+
+def _ljtables(exe, inp, case, tmp_path):
+    r = subprocess.run([exe, "ljtables", str(inp), case], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=60)
+    if r.returncode != 0:
+        return r, None
+    rows = [l.split() for l in r.stdout.split("\n") if l and not l.startswith("[")]
+    assert rows[0] == ["has_cutoff_list", "1"]
+    t = [np.array([float.fromhex(x) for x in row]).reshape(3, 3) for row in rows[1:4]]
+    return r, t
+
+
+def test_lj_verify_settings_tables_match_reference(golden, tmp_path):
+    """Lj::verify_settings (lj.cpp:180-391) in a user program, no device:
+    the epsilon, sigma and cutoff-list tables equal bit for bit those the
+    compiled reference forms from the same inputs (tests/golden/lj_tables,
+    made by make_golden.py through oracle/ref_driver.cpp), and follow the
+    closed forms: Lorentz-Berthelot mixing of the per-atom values, WCA
+    cutoffs 2^(1/6) sigma_ij before the lambda_s scaling, lambda_s / lambda_e
+    products with missing entries 1, array tables zero-filled to the atom
+    types present.  Per-atom and array input together are refused."""
+    g = golden("lj_tables")
+    s = _small_fluid()
+    inp = tmp_path / "in.txt"
+    _write_input(inp, s)
+    exe = _compile(os.path.join(ROOT, "tests", "cpp", "dropin_checks.cpp"), str(tmp_path / "dc"))
+    for case in ("atom", "array"):
+        r, (e, sg, cl) = _ljtables(exe, inp, case, tmp_path)
+        assert np.array_equal(e, g[case + "_eps"]), case
+        assert np.array_equal(sg, g[case + "_sigma"]), case
+        assert np.array_equal(cl, g[case + "_cutoff_list"]), case
+    # closed forms of the per-atom case
+    ea, sa = np.array([1.0, 0.5, 0.8]), np.array([1.0, 0.88, 1.2])
+    lam_s = np.array([[1.0, 0.9, 1.0], [0.9, 1.0, 1.0], [1.0, 1.0, 1.0]])
+    lam_e = np.array([[1.0, 0.5, 1.0], [0.5, 1.0, 1.0], [1.0, 1.0, 0.7]])
+    s_mix = 0.5 * (sa[:, None] + sa[None, :])
+    e_mix = np.sqrt(ea[:, None] * ea[None, :])
+    np.fill_diagonal(s_mix, sa)
+    np.fill_diagonal(e_mix, ea)
+    e, sg, cl = (g["atom_eps"], g["atom_sigma"], g["atom_cutoff_list"])
+    assert np.allclose(sg, s_mix * lam_s, rtol=1e-15, atol=0)
+    assert np.allclose(e, e_mix * lam_e, rtol=1e-15, atol=0)
+    assert np.allclose(cl, 2.0 ** (1.0 / 6.0) * s_mix, rtol=1e-15, atol=0)
+    # the array case: entries as set, the rest 0; the cutoff list [[2.5]] zero-filled
+    assert np.array_equal(g["array_eps"], [[1.0, 0.5, 0.0], [0.5, 0.8, 0.0], [0.0, 0.0, 0.0]])
+    assert np.array_equal(g["array_sigma"], [[1.0, 0.0, 0.0], [0.0, 0.9, 0.0], [0.0, 0.0, 0.0]])
+    assert np.array_equal(g["array_cutoff_list"], [[2.5, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    r, _ = _ljtables(exe, inp, "both", tmp_path)
+    assert r.returncode != 0 and "input_by_atom && input_by_array" in (r.stdout + r.stderr)
+
+
+@pytest.mark.gpu
+def test_lj_verify_settings_run_matches_oracle(tmp_path):
+    """A user program with two species (masses 1, 1.5) given per atom
+    (epsilon 1, 0.6; sigma 1, 0.9), WCA cutoffs and a lambda_e scaling, which
+    calls Lj::verify_settings: 40 velocity-Verlet steps on the device follow
+    the oracle's run with the tables of the closed forms (mixing, 2^(1/6)
+    sigma_ij cutoffs, lambda_e products)."""
+    s = _small_fluid()
+    inp = tmp_path / "in.txt"
+    _write_input(inp, s)
+    exe = _compile(os.path.join(ROOT, "tests", "cpp", "dropin_checks.cpp"), str(tmp_path / "dc"))
+    r = subprocess.run([exe, "ljrun", str(inp), "40"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [l.split() for l in r.stdout.split("\n")]
+    fin = np.array([[float.fromhex(x) for x in t] for t in rows
+                    if len(t) == 6 and all(x.lstrip("-").startswith("0x") for x in t)])
+    n = len(s["pos"])
+    assert fin.shape == (n, 6)
+    sa, ea = np.array([1.0, 0.9]), np.array([1.0, 0.6])
+    sig = 0.5 * (sa[:, None] + sa[None, :])
+    eps = np.sqrt(ea[:, None] * ea[None, :])
+    np.fill_diagonal(sig, sa)
+    np.fill_diagonal(eps, ea)
+    eps = eps * np.array([[1.0, 0.8], [0.8, 1.0]])
+    cl = 2.0 ** (1.0 / 6.0) * sig
+    o_sys = dict(s, type=(np.arange(n) % 2).astype(np.int32), eps=eps, sigma=sig, cutoff_list=cl,
+                 mass=np.array([1.0, 1.5]), use_cell_list=1)
+    with O.OracleSim(o_sys) as o:
+        o.setup()
+        for _ in range(40):
+            o.step()
+        po, vo, _ = o.state()
+    print(f"[ljrun] max |dx| {np.abs(fin[:, :3] - po).max():.2e}, |dv| {np.abs(fin[:, 3:] - vo).max():.2e}")
+    assert np.abs(fin[:, :3] - po).max() < 1e-10 and np.abs(fin[:, 3:] - vo).max() < 1e-9
