@@ -194,16 +194,8 @@ struct CellStarts {
 
 constexpr int kRowWaves = 4;  // x-rows partitioned per work-group (one wavefront each)
 constexpr int kMaxRowCells = 1024;
-constexpr int kDpCells = 256;  // rows up to this many cells get the wave-count cut
-// cut cost: per wavefront, per task.  A task costs more than the four
-// wavefronts it can save, so the cut keeps the fewest tasks and among those
-// takes the fewest wavefronts.  (Measured on the 2,048,000-atom fluid: a
-// cut with 11 % more tasks and 4.5 % fewer wavefronts made the force pass
-// 5.5 % slower in the liquid; at the fewest tasks the lattice start's force
-// pass is 1 % faster than the greedy cut's, the liquid's unchanged.)
-constexpr unsigned kDpWave = 16u, kDpTask = 64u;
 
-__device__ void task_row_partition(int row, int lane, int *P, int *H, int *D,
+__device__ void task_row_partition(int row, int lane, int *P, int *H,
                                    const int *__restrict__ count, const CellStarts &cs,
                                    const TaskGeom &g, int4 *__restrict__ tpiece,
                                    int4 *__restrict__ tinfo, int *__restrict__ ctask,
@@ -225,10 +217,9 @@ __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ 
     // exclusive prefix sums over x of one row's 9-row column counts (P) and
     // of its own counts (H), per wavefront
     __shared__ int P[kRowWaves][kMaxRowCells + 1], H[kRowWaves][kMaxRowCells + 1];
-    __shared__ int D[kRowWaves][kDpCells + 1];
     const int wv = (int)(threadIdx.x >> 6);
     task_row_partition((b - nbs - nfix) * kRowWaves + wv, (int)(threadIdx.x & 63), P[wv], H[wv],
-                       D[wv], count, cs, g, tpiece, tinfo, ctask, rowtasks, flags, gstart, n);
+                       count, cs, g, tpiece, tinfo, ctask, rowtasks, flags, gstart, n);
     return;
   }
   if (b >= nbs) {
@@ -1263,21 +1254,13 @@ __global__ void __launch_bounds__(256)
 // starts of a build are known).  P and H (LDS, nx + 1 entries each) receive
 // the exclusive prefix sums over x of the row's 9-row column counts and of
 // its own counts; from them the home atoms and the staged neighbourhood of
-// any run of cells [x0, x1) are O(1).  A task is admissible when it keeps
-// at most kTaskBlock home atoms and a neighbourhood (plus the sentinel)
-// within kStage, or is a single cell.  The force kernel runs one wavefront
-// per 64 home atoms of a task, each until its longest list is done, and
-// each task pays for staging its neighbourhood, so the row is cut into the
-// fewest tasks and, among those cuts, the fewest wavefronts: a dynamic
-// program over the cut ends, D[e] = the best cut of cells [0, e) as
-// cost << 12 | the start of its last task, the admissible starts of a task
-// ending at e being a suffix of [0, e) (both limits grow with the run).
-// Rows wider than kDpCells cells, and rows whose best cut needs more than
-// the row's task slots, are cut greedily instead (longest admissible run
-// from x0).  The last slot of the row takes whatever remains.  Each task slot gets its
+// any run of cells [x0, x1) are O(1).  The row is cut greedily: each task
+// takes the longest run from x0 that keeps at most kTaskBlock home atoms and
+// a neighbourhood (plus the sentinel) within kStage, at least one cell; the
+// last slot of the row takes whatever remains.  Each task slot gets its
 // piece table (first atom, slot base, x-cells of every piece) and tinfo
 // (home atoms [h0, h1), staged count S, x0 | x1 << 16), each cell its slot.
-__device__ void task_row_partition(int row, int lane, int *P, int *H, int *D,
+__device__ void task_row_partition(int row, int lane, int *P, int *H,
                                    const int *__restrict__ count, const CellStarts &cs,
                                    const TaskGeom &g, int4 *__restrict__ tpiece,
                                    int4 *__restrict__ tinfo, int *__restrict__ ctask,
@@ -1336,64 +1319,9 @@ __device__ void task_row_partition(int row, int lane, int *P, int *H, int *D,
   bool ovf = false;
   // open grids: a row without owned atoms (the ghost margin) has no task
   if (g.open && H[nx] == 0) x0 = nx;
-  // the wavefront-count cut; afterwards D[x] holds the end of the task that
-  // starts at x, for the starts of the chosen cut
-  bool dp = x0 < nx && nx <= kDpCells;
-  if (dp) {
-    volatile int *Dv = D;
-    if (lane == 0) Dv[0] = 0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int e = 1; e <= nx; ++e) {  // uniform over the wavefront
-      unsigned best = 0xffffffffu;
-      for (int b0 = 0; b0 < e; b0 += 64) {
-        const int s0 = e - 1 - b0 - lane;  // a start of the task ending at e
-        bool ok = false;
-        if (s0 >= 0) {
-          const int h = H[e] - H[s0];
-          ok = s0 == e - 1 || (h <= kTaskBlock && staged(s0, e) + 1 <= kStage);
-          if (ok) {
-            const unsigned c = min(((unsigned)Dv[s0] >> 12) + (unsigned)((h + 63) >> 6) * kDpWave + kDpTask,
-                                   0xfffffu);  // (saturates only for absurd cell counts)
-            best = min(best, (c << 12) | (unsigned)s0);
-          }
-        }
-        // admissible starts are a suffix: stop at the first inadmissible one
-        if (__ballot(s0 >= 0 && !ok)) break;
-      }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) best = min(best, (unsigned)__shfl_xor((int)best, o, 64));
-      if (lane == 0) Dv[e] = (int)best;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    // count the tasks of the best cut, then turn its back links into
-    // forward links (every lane walks the same path; lane 0 writes, each
-    // entry read before it is overwritten)
-    int ntask = 0;
-    for (int e = nx; e > 0; e = Dv[e] & 0xfff) ++ntask;
-    dp = ntask <= g.mpr;
-    if (dp) {
-      int cur = nx, prev = Dv[nx] & 0xfff;
-      while (true) {
-        const int pp = prev > 0 ? (Dv[prev] & 0xfff) : -1;
-        if (lane == 0) Dv[prev] = cur;
-        if (prev == 0) break;
-        cur = prev;
-        prev = pp;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
   while (x0 < nx) {  // uniform over the wavefront
     int x1 = x0 + 1;
-    if (t + 1 < g.mpr && dp) {
-      x1 = min(max(D[x0], x0 + 1), nx);  // (the clamp only guards the loop's progress)
-    } else if (t + 1 < g.mpr) {
+    if (t + 1 < g.mpr) {
       // both limits grow with x1, so the admissible ends are a prefix
       for (int e0 = x0 + 1; e0 <= nx; e0 += 64) {
         const int e = e0 + lane;
