@@ -301,8 +301,8 @@ def main():
                     help="run only the force kernel and a calibration copy (for rocprofv3 --pmc)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); multi-GPU sharding is a design "
-              "note in DESIGN.md, not a measured configuration", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): one GPU holds the whole box, "
+              "and no sharding across GPUs is built", file=sys.stderr)
         sys.exit(2)
 
     if a.traffic_probe:

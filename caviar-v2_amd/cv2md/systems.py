@@ -36,7 +36,7 @@ def lj_fluid(ncell: int, rho: float = 0.8442, temperature: float = 1.44, seed: i
              rc: float = 2.5, skin: float = 0.3, dt: float = 0.005, integrator: int = 0,
              gamma: float = 0.0):
     """Single-species LJ fluid on an FCC start (the reference's
-    tests/t002_lj.cpp set-up at BASELINE densities)."""
+    tests/t002_lj.cpp set-up, by default at reduced density 0.8442)."""
     pos, L = fcc_lattice(ncell, rho)
     n = len(pos)
     vel = maxwell_velocities(n, temperature, seed)

@@ -17,7 +17,7 @@ from helpers import force_error, oracle_forces, oracle_pairs, sort_rows
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12
-# FP32 pair arithmetic: per-atom |da| / max(|a|, rms|a|) (precision)
+# FP32 pair arithmetic, any kernel, small systems: per-atom |da| / max(|a|, rms|a|)
 FP32_TOL_MAX = 2e-5
 FP32_TOL_RMS = 5e-6
 
