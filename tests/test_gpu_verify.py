@@ -105,3 +105,21 @@ def test_host_halo_refused():
         with pytest.raises(D.Cv2Error) as e:
             c.set_ghosts(np.zeros((1, 3)) + 1.0, np.zeros(1, np.int32), np.zeros(1, np.int64))
         assert e.value.code == 6  # CV2_ERR_STATE
+
+
+def test_c3_every_build_checked():
+    # the 2,048,000-atom FP32 + fixed-point Langevin workload (80^3 FCC start,
+    # T* = 1, gamma = 1, dt = 0.005): every list build of 300 steps, the
+    # lattice melting included, checked on the device (the coarse-filter wave
+    # build in its 32-bit form)
+    s = lj_fluid(80, temperature=1.0, seed=1003, integrator=1, gamma=1.0)
+    with D.Context.from_sysdef(s, precision=D.CV2_PREC_FP32_FIXED, verify=True) as c:
+        c.compute_forces()
+        c.run_langevin(300, s["dt"], 1.0, 1.0, 1003)
+        st = c.stats()
+    print(f"[verify] C3: {st['rebuilds']} rebuilds, {st['verified_builds']} checked, "
+          f"{st['verify_mismatches']} differed, last {st['verify_pairs'][0]} ordered pairs", flush=True)
+    assert st["build_kind"] == "cell_wave"
+    assert st["rebuilds"] > 30
+    assert st["verified_builds"] == st["rebuilds"]
+    assert st["verify_mismatches"] == 0
