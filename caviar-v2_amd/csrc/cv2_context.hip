@@ -1497,6 +1497,7 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
   bool opened = false;  // this iteration's integrator pass already ran (fused)
   launch_set_flag(ctx->s.flags, F_STEP, 0, ctx->stream);  // steps are counted from this run's first
   for (int attempt = 0;; ++attempt) {
+    bool closed = false;  // the last step's finishing half ran fused into its force pass
     for (int64_t k = k0; k < nsteps; ++k) {
       if (!opened) {
         u.finish = k > 0;
@@ -1514,17 +1515,21 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
       span_end(ctx, t, 1);
       t = span_begin(ctx, 0);
       opened = false;
-      if (ctx->fuse && k + 1 < nsteps && ctx->precision != CV2_PREC_FP64) {
+      if (ctx->fuse && ctx->precision != CV2_PREC_FP64) {
         UpdateArgs f = u;
         f.finish = true;
-        f.start = true;
+        f.start = k + 1 < nsteps;   // the last step closes only
         f.step = s0 + (uint64_t)k;  // what iteration k + 1 would close and open
         if (launch_forces_fused(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, f, ctx->stream)) {
-          std::swap(ctx->s.q, ctx->s.q_alt);
-          opened = true;
+          if (f.start) {
+            std::swap(ctx->s.q, ctx->s.q_alt);
+            opened = true;
+          } else {
+            closed = true;
+          }
         }
       }
-      if (!opened)
+      if (!opened && !closed)
         launch_forces(ctx->s, ctx->b, ctx->lj_dev, ctx->lj, ctx->precision, false, ctx->stream);
       span_end(ctx, t, 0);
       if (ctx->spans.size() > 4000) {
@@ -1532,7 +1537,7 @@ int run_steps(cv2_ctx *ctx, int64_t nsteps, UpdateArgs u) {
         spans_collect(ctx);
       }
     }
-    if (nsteps > 0) {
+    if (nsteps > 0 && !closed) {
       u.finish = true;
       u.start = false;
       u.step = s0 + (uint64_t)nsteps - 1;

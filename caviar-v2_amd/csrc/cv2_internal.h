@@ -529,7 +529,9 @@ StepParams step_params(const BoxDev &b, const UpdateArgs &u);
 // into the epilogue: forces of the current positions, then u's finish +
 // start for every owned atom (accelerations are not stored); the
 // fixed-point coordinates of the new positions go to s.q_alt, and the host
-// swaps s.q and s.q_alt after the launch.  Returns false (nothing launched)
+// swaps s.q and s.q_alt after the launch.  u.start = false (the last step
+// of a run): the finishing velocity half only, the accelerations stored,
+// positions and q unchanged (no swap).  Returns false (nothing launched)
 // where the fused form does not apply.
 bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                          const UpdateArgs &u, hipStream_t st);

@@ -486,6 +486,7 @@ struct FuseArgs {
   int4 *q_out;
   StepParams p;
   int langevin;
+  int start;  // 0: the last step of a run, its finishing velocity half only
 };
 
 // One work-group per force task, in the compact order of the build
@@ -529,11 +530,23 @@ __global__ void __launch_bounds__(kTaskBlock) __attribute__((amdgpu_waves_per_eu
   double *at_sh = reinterpret_cast<double *>(sx);
   static_assert(3 * kTaskBlock * sizeof(double) <= sizeof(int) * kStage, "staging area too small");
   auto fuse_step = [&](int ia, double4 x, double v[3], const double4 &r, int idv, const double *aa) {
-    if (fz.langevin)
-      integrate_values<true, true, true, true>(x, v, idv, aa, fz.p);
-    else
-      integrate_values<false, true, true, true>(x, v, idv, aa, fz.p);
-    integrate_store<true, true, false>(ia, n, x, v, r, fz.pos, fz.vel, fz.q_out, fz.p, flags);
+    if (fz.start) {
+      if (fz.langevin)
+        integrate_values<true, true, true, true>(x, v, idv, aa, fz.p);
+      else
+        integrate_values<false, true, true, true>(x, v, idv, aa, fz.p);
+      integrate_store<true, true, false>(ia, n, x, v, r, fz.pos, fz.vel, fz.q_out, fz.p, flags);
+    } else {  // positions and q stay (other tasks may still stage them); the
+              // accelerations are kept for the next run's first half step
+      if (fz.langevin)
+        integrate_values<true, true, false, true>(x, v, idv, aa, fz.p);
+      else
+        integrate_values<false, true, false, true>(x, v, idv, aa, fz.p);
+      integrate_store<false, false, false>(ia, n, x, v, r, fz.pos, fz.vel, fz.q_out, fz.p, flags);
+      acc[ia] = aa[0];
+      acc[n + ia] = aa[1];
+      acc[2 * n + ia] = aa[2];
+    }
   };
 
   // two species without jump_fix: the branchless entry (task_atom)
@@ -908,9 +921,10 @@ void launch_forces_ens(DevState &s, const EnsDev &e, const LJDev *lj_dev, int bl
 bool launch_forces_fused(DevState &s, const BoxDev &b, const LJDev *lj_dev, const LJDev &lj,
                          const UpdateArgs &u, hipStream_t st) {
   // (the fused epilogue has no owned wrap and no reference rebuild triggers)
-  if (!s.slots || s.n == 0 || s.ng > 0 || !s.q_alt || !u.finish || !u.start) return false;
+  if (!s.slots || s.n == 0 || s.ng > 0 || !s.q_alt || !u.finish) return false;
   if (u.ref_rebuild || u.wrap_owned) return false;
-  FuseArgs fz{s.pos, s.pos_ref, s.vel, s.id, s.q_alt, step_params(b, u), u.langevin ? 1 : 0};
+  FuseArgs fz{s.pos, s.pos_ref, s.vel, s.id, s.q_alt, step_params(b, u), u.langevin ? 1 : 0,
+              u.start ? 1 : 0};
   long long *part = reinterpret_cast<long long *>(s.partials);
   const int tgrid = s.task_grid;
   const bool tail = tgrid < (s.tg.nslot + 7) / 8 * 8;
