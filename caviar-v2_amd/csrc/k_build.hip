@@ -1917,7 +1917,9 @@ void launch_build(DevState &s, const BoxDev &b, hipStream_t st, bool host_halo, 
     cb.open = b.open;
     cb.n = n;
     const int real = nblk(nc, 4), grid = (real + 7) / 8 * 8;
-    const int ncopy = copy_in_list ? (nblk(n, 256) + 7) / 8 * 8 : 0;
+    // (the copy-back work-groups stride over the atoms: a fixed 1024 of them
+    // keep the idle launch of steps without a rebuild short)
+    const int ncopy = copy_in_list ? std::min((nblk(n, 256) + 7) / 8 * 8, 1024) : 0;
     // the list kernel reads the new-order positions where they were moved
     const double4 *pos_new = copy_in_list ? s.pos_alt : s.pos;
 #define CV2_NLC(G, S)                                                                            \
