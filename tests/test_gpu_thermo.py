@@ -9,6 +9,8 @@ energy = 0): an energy pass at the current positions sums them on demand.
   loop left them, and the run continues with the same bits as a run that
   never asked (thermo between two runs changes nothing).
 - While a step is open the last summed values are reported unchanged.
+- A run split into several runs gives the bits of one run (each run closes
+  its last step and the next opens the following one).
 """
 import numpy as np
 import pytest
@@ -61,3 +63,23 @@ def test_open_step_reports_last_values():
         _, _, vir, pe = oracle_forces(c.download()[0], s)
         assert abs(th2["pe"] - pe) < 1e-10 * max(abs(pe), len(s["pos"]))
         assert th2["pe"] != th0["pe"]
+
+
+@pytest.mark.parametrize("prec", sorted(PRECS))
+def test_runs_in_pieces_give_the_same_bits(prec):
+    # a run closes its last step (on the FP32 task path inside the fused
+    # force kernel) and the next run opens the following one: 1 + 1 + 3 + 7
+    # + 12 steps in five runs give the bits of one 24-step run, rebuilds
+    # included
+    p_id, _ = PRECS[prec]
+    s = lj_fluid(10, temperature=1.2, seed=93, integrator=1, gamma=1.0)
+    out = []
+    for pieces in ((24,), (1, 1, 3, 7, 12)):
+        with D.Context.from_sysdef(s, precision=p_id) as c:
+            c.compute_forces()
+            for k in pieces:
+                c.run_langevin(k, s["dt"], 1.0, 1.2, 11)
+            out.append((c.download(), c.stats()["rebuilds"]))
+    assert out[0][1] == out[1][1] and out[0][1] > 1
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
